@@ -1,0 +1,164 @@
+/*
+ * smg.h -- C ABI of the MI355X (gfx950) particle-relaxation engine.
+ *
+ * This is the drop-in boundary for the adaptive particle-relaxation hot path
+ * of SPH-mesh-generation.  In the reference the path is the compile-time
+ * SOLVER class driven by `Solver` (src/solver.h:28-45, src/solver.cpp:257-321);
+ * there is no FFI.  Each entry point below names the reference member it
+ * replaces.  Conventions:
+ *   - plain C types, caller-owned host buffers, library-owned device state;
+ *   - every call returns an int status (SMG_OK or a negative SMG_ERR_*); no
+ *     exceptions cross the ABI, smg_last_error() gives the message;
+ *   - calls on one handle are single-threaded; each handle owns a HIP stream;
+ *   - per-particle host arrays are indexed by particle id (0..n-1), the order
+ *     the particles were created in, whatever order the device keeps them in.
+ */
+#ifndef SMG_H
+#define SMG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMG_ABI_VERSION 1
+
+enum {
+  SMG_OK = 0,
+  SMG_ERR_ARG = -1,       /* bad argument / config                          */
+  SMG_ERR_HIP = -2,       /* HIP runtime error (no GPU, launch failure, ...) */
+  SMG_ERR_CAPACITY = -3,  /* neighbour capacity or table size exceeded       */
+  SMG_ERR_STATE = -4,     /* call out of order (e.g. step before levelset)  */
+  SMG_ERR_REMAP = -5,     /* re-seeding found no positive-phase position    */
+  SMG_ERR_NONFINITE = -6  /* a NaN/Inf appeared in the particle state        */
+};
+
+/* particle types, src/glbparam.h:55-62 */
+enum { SMG_REAL = 0, SMG_SURFACE = 1, SMG_SINGULARITY = 2, SMG_SEGMENT = 3 };
+
+/* target-spacing profiles: tanh is Levelset::Get_scale
+ * (src/level_set_solver/level_set.cpp:311-351); linear is the build's
+ * extension dx = dx0 + min(max(phi,0)/band,1)*(dx1-dx0). */
+enum { SMG_PROFILE_TANH = 0, SMG_PROFILE_LINEAR = 1 };
+
+typedef struct smg_config {
+  int32_t dim;               /* 2 or 3                                        */
+  int32_t sym[3];            /* mirror-ghost walls per axis (SYM_DIM_X/Y/Z)   */
+  double box_l[3];           /* SPH and level-set box corner (SPH::box_l)     */
+  double domain[3];          /* box extent (SPH::domain)                      */
+  double particle_box_l[3];  /* seeding / re-seeding box (particle_box_l/r)   */
+  double particle_box_r[3];
+  int32_t num_pkg[3];        /* level-set packages per axis, 4 cells each     */
+  double cell_ratio;         /* CELL_RATIO, h = dx*cell_ratio/cut_off         */
+  double cut_off;            /* CUT_OFF                                       */
+  double angle_thres;        /* SPH_mesh_generation::angle_thres (0.4*PI)     */
+  double nu_coeff;           /* nu_coeff_start == nu_coeff_end (0.1)          */
+  int32_t profile;           /* SMG_PROFILE_*                                 */
+  double min_dl, max_dl;     /* tanh profile: minimum_dl / maximum_dl         */
+  double max_phi;            /* tanh profile: Levelset::maximum_phi           */
+  double lin_dx0, lin_dx1;   /* linear profile end spacings                   */
+  double lin_band;           /* linear profile band width in phi              */
+  uint64_t seed;             /* counter-based RNG key (replaces rand())       */
+  int32_t remap_max_attempts;/* bound on the re-seeding rejection loop        */
+  int32_t max_neighbors;     /* per-particle list capacity, 0 = automatic     */
+} smg_config;
+
+/* Level-set statistics from smg_lset_build (Levelset::Calculate_total_volume_mass,
+ * level_set.cpp:930-972, and Parallel_get_global_scale, parallelfunc_mesh.cpp:46-54). */
+typedef struct smg_lset_stats {
+  double maximum_phi;
+  double total_volume;
+  double total_area;
+  double total_mass;
+  double total_mass_surface;
+  int64_t n_interface_cell;
+  int64_t num_particle_mesh;     /* SPH_mesh_generation::Get_num_particles */
+  int64_t num_particle_surface;
+  int64_t num_particle;
+} smg_lset_stats;
+
+/* Per-step diagnostics (the values SPH_mesh_generation keeps as members). */
+typedef struct smg_stats {
+  int64_t step;          /* relaxation steps taken (run_time)                   */
+  int64_t n_particle;
+  int64_t n_real, n_surface;
+  int64_t n_remapped;    /* re-seeded in the last mapping pass                  */
+  int64_t n_neighbor_entries;
+  int32_t max_neighbors_seen;
+  int32_t num_level;     /* cell-list levels in use                             */
+  double dt1, dt2;       /* glbl_timestep after Set_timestep(1) / (2)           */
+  double max_v;          /* glbl_max_v after Set_timestep(2)                    */
+  double residual;       /* max_i |v_i| dt2 / dx_i of the last step             */
+  double max_disp;       /* max_i |v_i| dt2 of the last step                    */
+} smg_stats;
+
+typedef struct smg_job smg_job;
+
+int smg_abi_version(void);
+const char* smg_last_error(void);
+
+/* Reference constants for `dim` (glbparam.h:171-185, sph_mesh_generation.cpp:8-152,680-743). */
+int smg_default_config(int dim, smg_config* cfg);
+
+/* Host-only level-set finishing: normals (Levelset_package::Get_normal / Get_BC_normal,
+ * lset_package.cpp:332-389), cut-cell tags (Get_interface_tag / Is_cut_cell,
+ * :450-469, :664-761), maximum_phi and particle count
+ * (SPH_mesh_generation::Get_num_particles, sph_mesh_generation.cpp:900-941).
+ * phi_ext: cell-centred phi on the grid with a full margin package on each side,
+ *          (4*num_pkg+8) cells per active axis, x fastest.
+ * zero_margin: 1 = restart-mode margins (phi = n = 0 outside the box), 0 = analytic.
+ * Outputs on the grid with a one-cell halo, (4*num_pkg+2) cells per active axis:
+ * phi (double), normal (float, 3 per cell), tag (int8, 1 = CUT_CELL). */
+int smg_lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin,
+                   double* phi, float* normal, int8_t* tag, smg_lset_stats* stats);
+
+/* SPH_mesh_generation constructor + Initialize_case (sph_mesh_generation.cpp:8, :594). */
+int smg_create(const smg_config* cfg, smg_job** job);
+int smg_destroy(smg_job* job);
+
+/* Level set on the one-cell-halo grid of smg_lset_build (copied to the device);
+ * replaces Load_rst_case / Levelset::Load_lset_restart_file (:323-422). */
+int smg_set_levelset(smg_job* job, const double* phi, const float* normal, const int8_t* tag);
+
+/* Particles at given positions (n*3 doubles, unused axes ignored); type may be NULL
+ * (all REAL).  h is set from the level set (Set_particle_info). */
+int smg_init_particles(smg_job* job, int64_t n, const double* x, const int32_t* type);
+/* Random_particle_distribution (sph_mesh_generation.cpp:1302-1367) with the
+ * counter-based RNG keyed by cfg->seed. */
+int smg_seed_particles(smg_job* job, int64_t n);
+/* Pre_run (sph_mesh_generation.cpp:1010-1052): map/project once, zero v and nu. */
+int smg_prerun(smg_job* job);
+/* nsteps x SPH_run_mesh_generation (sph_mesh_generation.cpp:1371-1463). */
+int smg_step(smg_job* job, int64_t nsteps);
+/* Solver::Get_timestep (solver.cpp:269). */
+int smg_get_timestep(smg_job* job, double* dt);
+int smg_get_stats(smg_job* job, smg_stats* st);
+
+/* State by particle id; any pointer may be NULL.  x, v: n*3 doubles; h: dx*cell_ratio/cut_off. */
+int smg_get_state(smg_job* job, double* x, double* v, int32_t* type, double* h, float* nu);
+/* Last step's per-particle pair results: sum (kernel summation), a1 (stage-1
+ * acceleration), a2 (stage-2 acceleration), n*3 floats for a1/a2. */
+int smg_get_pair_terms(smg_job* job, float* sum, float* a1, float* a2);
+/* Neighbour sets of the last step (Level_info::Refresh_neighbor_info,
+ * level_infor.cpp:469-506): for particle id i, entries offs[i]..offs[i+1]-1 hold
+ * (src id, mirror code), sorted.  code = 0 for a real particle, else 2 bits per axis
+ * (1 = image across the low wall, 2 = across the high wall) for mirror ghosts
+ * (Level_info::Construct_symmetric_BC_particles, level_infor.cpp:786-856).
+ * With ids == NULL only offs is filled (to size the buffers). */
+int smg_get_neighbors(smg_job* job, int64_t* offs, int32_t* ids, int32_t* codes, int64_t capacity);
+
+/* Kernel timing: average device time in ms of each kernel over the steps taken since
+ * the last reset, measured with HIP events on the job's stream. */
+int smg_profile_enable(smg_job* job, int enable);
+int smg_profile_get(smg_job* job, const char** names, double* avg_ms, int64_t* counts, int max_kernels, int* n_kernels);
+
+/* Diagnostic with no counterpart in the reference: device FP64 sqrt(a), a/b,
+ * floor(a/b) and tanh(a) for n inputs (4 outputs per input), to check that the FP64
+ * operations the bit-exact paths use round as the host's do. */
+int smg_fp64_probe(int64_t n, const double* a, const double* b, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,265 @@
+// smg_api.cpp -- the extern "C" boundary declared in include/smg.h.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "smg.h"
+#include "smg_engine.hpp"
+#include "smg_internal.hpp"
+#include "smg_kernels.hpp"
+
+namespace smg {
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace smg
+
+using smg::set_error;
+
+struct smg_job {
+  smg_config cfg;
+  std::vector<double> phi;
+  std::vector<float> normal;
+  std::vector<int8_t> tag;
+  double max_phi = 0.;
+  bool has_lset = false;
+  std::unique_ptr<smg::Engine> eng;
+  bool profile = false;
+};
+
+namespace {
+
+int64_t halo_cells(const smg_config& c) {
+  int64_t m = 1;
+  for (int a = 0; a < c.dim; a++) m *= 4 * int64_t(c.num_pkg[a]) + 2;
+  return m;
+}
+
+int check_config(const smg_config* c) {
+  if (!c) { set_error("config is NULL"); return SMG_ERR_ARG; }
+  if (c->dim != 2 && c->dim != 3) { set_error("dim must be 2 or 3"); return SMG_ERR_ARG; }
+  for (int a = 0; a < c->dim; a++) {
+    if (c->num_pkg[a] <= 0) { set_error("num_pkg must be positive"); return SMG_ERR_ARG; }
+    if (!(c->domain[a] > 0.)) { set_error("domain must be positive"); return SMG_ERR_ARG; }
+    if (!(c->particle_box_r[a] > c->particle_box_l[a])) { set_error("particle box is empty"); return SMG_ERR_ARG; }
+  }
+  if (!(c->cell_ratio > 0.) || !(c->cut_off > 0.)) { set_error("cell_ratio/cut_off must be positive"); return SMG_ERR_ARG; }
+  if (c->profile != SMG_PROFILE_TANH && c->profile != SMG_PROFILE_LINEAR) { set_error("unknown spacing profile"); return SMG_ERR_ARG; }
+  if (c->profile == SMG_PROFILE_TANH && !(c->min_dl > 0. && c->max_dl >= c->min_dl)) {
+    set_error("tanh profile needs 0 < min_dl <= max_dl");
+    return SMG_ERR_ARG;
+  }
+  if (c->profile == SMG_PROFILE_LINEAR && !(c->lin_dx0 > 0. && c->lin_dx1 > 0. && c->lin_band > 0.)) {
+    set_error("linear profile needs positive lin_dx0, lin_dx1, lin_band");
+    return SMG_ERR_ARG;
+  }
+  return SMG_OK;
+}
+
+int build_engine(smg_job* job, int64_t n) {
+  if (!job->has_lset) { set_error("set the level set first (smg_set_levelset)"); return SMG_ERR_STATE; }
+  smg::JobInput in;
+  in.cfg = job->cfg;
+  in.n = n;
+  in.max_phi = job->max_phi;
+  in.phi = job->phi.data();
+  in.normal = job->normal.data();
+  in.tag = job->tag.data();
+  job->eng.reset(new smg::Engine());
+  job->eng->profile_enable(job->profile);
+  const int rc = job->eng->setup({in}, job->cfg.max_neighbors, nullptr);
+  if (rc != SMG_OK) job->eng.reset();
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int smg_abi_version(void) { return SMG_ABI_VERSION; }
+const char* smg_last_error(void) { return smg::g_err.c_str(); }
+
+int smg_default_config(int dim, smg_config* c) {
+  if (!c || (dim != 2 && dim != 3)) { set_error("smg_default_config: bad argument"); return SMG_ERR_ARG; }
+  std::memset(c, 0, sizeof(*c));
+  c->dim = dim;
+  // SYMMETRIC = DIM2 with free slip (Makefile.in:183-196); all axes in 3D
+  c->sym[0] = 1; c->sym[1] = 1; c->sym[2] = dim == 3 ? 1 : 0;
+  for (int a = 0; a < 3; a++) {
+    c->box_l[a] = 0.;
+    c->domain[a] = 1.;
+    c->particle_box_l[a] = 0.;
+    c->particle_box_r[a] = 1.;
+    c->num_pkg[a] = a < dim ? 32 : 1;
+  }
+  // glbparam.h:171-178
+  c->cell_ratio = dim == 3 ? 1.45 : 2.0;
+  c->cut_off = 2.0;
+  c->angle_thres = 3.1415926 * 0.4;   // sph_mesh_generation.cpp:689 with PI of glbparam.h:185
+  c->nu_coeff = 0.1;                  // sph_mesh_generation.cpp:691-692
+  c->profile = SMG_PROFILE_TANH;
+  // restart-mode spacing factors 2 / 16 times the level-set cell (sph_mesh_generation.cpp:659-661)
+  const double dl = 1.0 / (4.0 * 32.0);
+  c->min_dl = 2.0 * dl;
+  c->max_dl = 16.0 * dl;
+  c->max_phi = 0.;
+  c->lin_dx0 = c->lin_dx1 = c->lin_band = 0.;
+  c->seed = 20261203ull;
+  c->remap_max_attempts = 100000;
+  c->max_neighbors = 0;
+  return SMG_OK;
+}
+
+int smg_lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin, double* phi, float* normal,
+                   int8_t* tag, smg_lset_stats* stats) {
+  if (check_config(cfg) != SMG_OK) return SMG_ERR_ARG;
+  if (!phi_ext || !phi || !normal || !tag) { set_error("smg_lset_build: NULL buffer"); return SMG_ERR_ARG; }
+  return smg::lset_build(cfg, phi_ext, zero_margin, phi, normal, tag, stats);
+}
+
+int smg_create(const smg_config* cfg, smg_job** job) {
+  if (!job) { set_error("smg_create: job is NULL"); return SMG_ERR_ARG; }
+  *job = nullptr;
+  const int rc = check_config(cfg);
+  if (rc != SMG_OK) return rc;
+  smg_job* j = new smg_job();
+  j->cfg = *cfg;
+  *job = j;
+  return SMG_OK;
+}
+
+int smg_destroy(smg_job* job) {
+  delete job;
+  return SMG_OK;
+}
+
+int smg_set_levelset(smg_job* job, const double* phi, const float* normal, const int8_t* tag) {
+  if (!job || !phi || !normal || !tag) { set_error("smg_set_levelset: NULL argument"); return SMG_ERR_ARG; }
+  const int64_t m = halo_cells(job->cfg);
+  job->phi.assign(phi, phi + m);
+  job->normal.assign(normal, normal + 3 * m);
+  job->tag.assign(tag, tag + m);
+  // maximum_phi over interior cells, as Parallel_get_global_scale (parallelfunc_mesh.cpp:46-54)
+  const smg_config& c = job->cfg;
+  const int nx = 4 * c.num_pkg[0], ny = c.dim > 1 ? 4 * c.num_pkg[1] : 1, nz = c.dim > 2 ? 4 * c.num_pkg[2] : 1;
+  const int gx = nx + 2, gy = c.dim > 1 ? ny + 2 : 1;
+  double mp = 0.;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        const int64_t o = ((int64_t(c.dim > 2 ? k + 1 : 0) * gy) + (c.dim > 1 ? j + 1 : 0)) * gx + i + 1;
+        if (job->phi[o] > 0.) mp = std::max(mp, std::fabs(job->phi[o]));
+      }
+  job->max_phi = mp;
+  job->has_lset = true;
+  return SMG_OK;
+}
+
+int smg_init_particles(smg_job* job, int64_t n, const double* x, const int32_t* type) {
+  if (!job || !x || n <= 0) { set_error("smg_init_particles: bad argument"); return SMG_ERR_ARG; }
+  int rc = build_engine(job, n);
+  if (rc != SMG_OK) return rc;
+  rc = job->eng->init_positions(0, x, type);
+  if (rc != SMG_OK) return rc;
+  return job->eng->init_finish();
+}
+
+int smg_seed_particles(smg_job* job, int64_t n) {
+  if (!job || n <= 0) { set_error("smg_seed_particles: bad argument"); return SMG_ERR_ARG; }
+  int rc = build_engine(job, n);
+  if (rc != SMG_OK) return rc;
+  rc = job->eng->seed();
+  if (rc != SMG_OK) return rc;
+  return job->eng->prerun();
+}
+
+int smg_prerun(smg_job* job) {
+  if (!job || !job->eng) { set_error("smg_prerun: no particles"); return SMG_ERR_STATE; }
+  return job->eng->prerun();
+}
+
+int smg_step(smg_job* job, int64_t nsteps) {
+  if (!job || !job->eng) { set_error("smg_step: no particles"); return SMG_ERR_STATE; }
+  if (nsteps < 0) { set_error("smg_step: negative step count"); return SMG_ERR_ARG; }
+  int rc = job->eng->step(nsteps);
+  if (rc != SMG_OK) return rc;
+  rc = job->eng->sync();
+  if (rc != SMG_OK) return rc;
+  return job->eng->check_health();
+}
+
+int smg_get_timestep(smg_job* job, double* dt) {
+  if (!job || !job->eng || !dt) { set_error("smg_get_timestep: bad argument"); return SMG_ERR_STATE; }
+  smg_stats st;
+  const int rc = job->eng->get_stats(0, &st);
+  *dt = st.dt2;
+  return rc;
+}
+
+int smg_get_stats(smg_job* job, smg_stats* st) {
+  if (!job || !job->eng || !st) { set_error("smg_get_stats: bad argument"); return SMG_ERR_STATE; }
+  return job->eng->get_stats(0, st);
+}
+
+int smg_get_state(smg_job* job, double* x, double* v, int32_t* type, double* h, float* nu) {
+  if (!job || !job->eng) { set_error("smg_get_state: no particles"); return SMG_ERR_STATE; }
+  return job->eng->get_state(0, x, v, type, h, nu);
+}
+
+int smg_get_pair_terms(smg_job* job, float* sum, float* a1, float* a2) {
+  if (!job || !job->eng) { set_error("smg_get_pair_terms: no particles"); return SMG_ERR_STATE; }
+  return job->eng->get_pair_terms(0, sum, a1, a2);
+}
+
+int smg_get_neighbors(smg_job* job, int64_t* offs, int32_t* ids, int32_t* codes, int64_t capacity) {
+  if (!job || !job->eng || !offs) { set_error("smg_get_neighbors: bad argument"); return SMG_ERR_STATE; }
+  if (ids && !codes) { set_error("smg_get_neighbors: codes is NULL"); return SMG_ERR_ARG; }
+  return job->eng->get_neighbors(0, offs, ids, codes, capacity);
+}
+
+int smg_profile_enable(smg_job* job, int enable) {
+  if (!job) { set_error("smg_profile_enable: job is NULL"); return SMG_ERR_ARG; }
+  job->profile = enable != 0;
+  if (job->eng) job->eng->profile_enable(job->profile);
+  return SMG_OK;
+}
+
+int smg_profile_get(smg_job* job, const char** names, double* avg_ms, int64_t* counts, int max_kernels, int* n_kernels) {
+  if (!job || !job->eng || !n_kernels) { set_error("smg_profile_get: bad argument"); return SMG_ERR_STATE; }
+  static thread_local std::vector<std::string> keep;
+  std::vector<double> ms;
+  std::vector<int64_t> cn;
+  const int rc = job->eng->profile_get(keep, ms, cn);
+  if (rc != SMG_OK) return rc;
+  *n_kernels = int(keep.size());
+  for (int q = 0; q < int(keep.size()) && q < max_kernels; q++) {
+    if (names) names[q] = keep[q].c_str();
+    if (avg_ms) avg_ms[q] = ms[q];
+    if (counts) counts[q] = cn[q];
+  }
+  return SMG_OK;
+}
+
+int smg_fp64_probe(int64_t n, const double* a, const double* b, double* out) {
+  if (n <= 0 || !a || !b || !out) { set_error("smg_fp64_probe: bad argument"); return SMG_ERR_ARG; }
+  double *da = nullptr, *db = nullptr, *dout = nullptr;
+  if (hipMalloc(&da, sizeof(double) * n) != hipSuccess || hipMalloc(&db, sizeof(double) * n) != hipSuccess ||
+      hipMalloc(&dout, sizeof(double) * 4 * n) != hipSuccess) {
+    (void)hipFree(da); (void)hipFree(db); (void)hipFree(dout);
+    set_error("smg_fp64_probe: hipMalloc failed");
+    return SMG_ERR_HIP;
+  }
+  (void)hipMemcpy(da, a, sizeof(double) * n, hipMemcpyHostToDevice);
+  (void)hipMemcpy(db, b, sizeof(double) * n, hipMemcpyHostToDevice);
+  smg::launch_fp64_probe(nullptr, n, da, db, dout);
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(double) * 4 * n, hipMemcpyDeviceToHost);
+  (void)hipFree(da); (void)hipFree(db); (void)hipFree(dout);
+  if (e != hipSuccess) { set_error(std::string("smg_fp64_probe: ") + hipGetErrorString(e)); return SMG_ERR_HIP; }
+  return SMG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// smg_device.hpp -- device-side data layout of the relaxation engine (gfx950).
+//
+// Particles live in structure-of-arrays form in slot order.  Every job owns a fixed
+// slot range [p0, p0 + cap) with cap a multiple of the 256-slot tile, so a tile (one
+// workgroup) never straddles jobs and the job's parameters are wave-uniform.  Slots
+// [p0, p0 + n) hold live particles in cell-sorted order; the tail holds dead slots
+// that sort into the job's pad cell.
+#pragma once
+#include <cstdint>
+
+namespace smg {
+
+constexpr int kTile = 256;       // slots per workgroup
+constexpr int kMaxLevels = 8;    // cell-list levels (cell size c0 / 2^L)
+constexpr int kJBits = 26;       // neighbour entry = slot | (mirror code << 26)
+constexpr uint32_t kJMask = (1u << kJBits) - 1u;
+constexpr int kScanItems = 4096; // table entries per scan workgroup (256 threads x 16)
+
+// one level-set cell: phi (FP64, used for projection), raw normal, interface tag
+struct alignas(8) LsCell {
+  double phi;
+  float n[3];
+  int32_t tag;
+};
+static_assert(sizeof(LsCell) == 24, "LsCell layout");
+
+struct JobParams {
+  int32_t D, nlev, profile, remap_max;
+  int32_t sym[3], ng[3];
+  int32_t num_pkg[3];
+  int32_t lev_n[kMaxLevels][3];
+  int32_t lev_bits[kMaxLevels];
+  double box_l[3], box_r[3], domain[3], pbox_l[3], pbox_r[3];
+  double dpkg[3], dcell[3], dl;
+  double cell_ratio, cut_off, nu_coeff;
+  double min_dl, max_dl, max_phi, tanh25, lin_dx0, lin_dx1, lin_band;
+  double c0, clen[kMaxLevels], cinv[kMaxLevels];
+  float wl_h[3], wl_l[3], wh_h[3], wh_l[3];   // walls as float pairs
+  float cos_thres, kdx;                       // cos(angle_thres), cut_off/cell_ratio
+  uint64_t seed;
+  int64_t lset_off;                           // first cell of this job's halo grid
+  int64_t tbl_base, tbl_size;                 // cell-table segment (last entry = pad cell)
+  int64_t lev_off[kMaxLevels];
+  int64_t p0, n, cap;                         // slot range
+};
+
+// per-job reductions; doubles stored as their bit patterns (all values >= 0)
+struct JobRed {
+  unsigned long long dt1[2], dt2[2], maxv[2], res[2], maxdisp[2];
+  int32_t nremap[2];
+  int32_t fail, overflow, maxnb, nsurf;
+};
+
+// one buffer set of the fields that follow the particles through the sort
+struct PState {
+  float* xh[3];
+  float* xl[3];
+  double* h;
+  int32_t* type;
+  float* nrm[3];
+  int32_t* id;
+};
+
+// work arrays and tables
+struct Work {
+  const JobParams* jp;
+  const int32_t* tile_job;
+  JobRed* red;
+  const LsCell* ls;
+  float* nu_a;   // nu read by the stage-2 pass (previous step's value)
+  float* nu_b;   // nu written by the stage-2 pass
+  float* a1[3];
+  float* a2[3];
+  float* sum;
+  uint32_t* key;
+  uint32_t* skey;
+  int32_t* perm;
+  int32_t* counts;
+  int32_t* starts;
+  int32_t* bsum;
+  int32_t* nbl;   // neighbour entries, [k * cap_total + slot]
+  int32_t* nbc;   // neighbour count per slot
+  int64_t* step_ctr;   // index of the step in flight (re-seeding RNG stream)
+  int64_t cap_total;
+  int64_t n_tbl;
+  int32_t K;
+};
+
+}  // namespace smg

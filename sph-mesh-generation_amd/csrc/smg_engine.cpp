@@ -1,0 +1,640 @@
+// smg_engine.cpp -- host orchestration: job layout, device buffers, the step
+// sequence of SPH_run_mesh_generation and result download.
+#include "smg_engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "smg_internal.hpp"
+#include "smg_kernels.hpp"
+
+namespace smg {
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                     \
+      return SMG_ERR_HIP;                                                               \
+    }                                                                                   \
+  } while (0)
+
+#define SMG_TRY(expr)        \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != SMG_OK) return r_; \
+  } while (0)
+
+Engine::~Engine() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  for (auto& r : recs_) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : pool_) (void)hipEventDestroy(e);
+  for (void* p : allocs_) (void)hipFree(p);
+  if (own_stream_ && st_) (void)hipStreamDestroy(st_);
+}
+
+template <typename T>
+T* Engine::alloc_(int64_t count) {
+  void* p = nullptr;
+  if (count <= 0) count = 1;
+  if (hipMalloc(&p, size_t(count) * sizeof(T)) != hipSuccess) return nullptr;
+  allocs_.push_back(p);
+  return static_cast<T*>(p);
+}
+
+int64_t Engine::total_particles() const {
+  int64_t s = 0;
+  for (auto& j : jp_) s += j.n;
+  return s;
+}
+
+static int ceil_log2(int64_t v) {
+  int b = 0;
+  while ((int64_t(1) << b) < v) b++;
+  return b;
+}
+
+int Engine::setup(const std::vector<JobInput>& jobs, int max_neighbors, hipStream_t stream) {
+  if (jobs.empty()) { set_error("setup: no jobs"); return SMG_ERR_ARG; }
+  D_ = jobs[0].cfg.dim;
+  if (D_ != 2 && D_ != 3) { set_error("setup: dim must be 2 or 3"); return SMG_ERR_ARG; }
+  K_ = max_neighbors > 0 ? max_neighbors : 64;
+  if (stream) st_ = stream;
+  else {
+    HIP_TRY(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    own_stream_ = true;
+  }
+  jp_.assign(jobs.size(), JobParams{});
+  int64_t p0 = 0, tbl = 0, ls = 0;
+  for (size_t j = 0; j < jobs.size(); j++) {
+    const smg_config& c = jobs[j].cfg;
+    JobParams& J = jp_[j];
+    std::memset(&J, 0, sizeof(J));
+    if (c.dim != D_) { set_error("setup: all jobs of one engine need the same dim"); return SMG_ERR_ARG; }
+    if (jobs[j].n <= 0) { set_error("setup: job without particles"); return SMG_ERR_ARG; }
+    J.D = D_;
+    J.profile = c.profile;
+    J.remap_max = c.remap_max_attempts > 0 ? c.remap_max_attempts : 100000;
+    double dl = 1.e20, dmin = 1.e300;
+    for (int a = 0; a < 3; a++) {
+      const bool on = a < D_;
+      J.sym[a] = on ? c.sym[a] : 0;
+      J.num_pkg[a] = on ? c.num_pkg[a] : 1;
+      if (on && c.num_pkg[a] <= 0) { set_error("setup: num_pkg must be positive"); return SMG_ERR_ARG; }
+      J.ng[a] = on ? 4 * c.num_pkg[a] + 2 : 1;
+      J.box_l[a] = on ? c.box_l[a] : 0.;
+      J.domain[a] = on ? c.domain[a] : 1.;
+      J.box_r[a] = J.box_l[a] + J.domain[a];
+      J.pbox_l[a] = on ? c.particle_box_l[a] : 0.;
+      J.pbox_r[a] = on ? c.particle_box_r[a] : 1.;
+      J.dpkg[a] = on ? c.domain[a] / double(c.num_pkg[a]) : 1.;
+      J.dcell[a] = on ? J.dpkg[a] / 4.0 : 1.;
+      if (on) { dl = std::min(dl, J.dcell[a]); dmin = std::min(dmin, J.domain[a]); }
+      const float lh = float(J.box_l[a]), rh = float(J.box_r[a]);
+      J.wl_h[a] = lh; J.wl_l[a] = float(J.box_l[a] - double(lh));
+      J.wh_h[a] = rh; J.wh_l[a] = float(J.box_r[a] - double(rh));
+    }
+    J.dl = dl;
+    J.cell_ratio = c.cell_ratio;
+    J.cut_off = c.cut_off;
+    J.nu_coeff = c.nu_coeff;
+    J.min_dl = c.min_dl; J.max_dl = c.max_dl;
+    J.max_phi = c.max_phi > 0. ? c.max_phi : jobs[j].max_phi;
+    J.tanh25 = std::tanh(2.5);
+    J.lin_dx0 = c.lin_dx0; J.lin_dx1 = c.lin_dx1; J.lin_band = c.lin_band;
+    J.cos_thres = float(std::cos(c.angle_thres));
+    J.kdx = float(c.cut_off / c.cell_ratio);
+    J.seed = c.seed;
+    if (c.profile == SMG_PROFILE_TANH && !(J.max_phi > 0.) && c.max_dl != c.min_dl) {
+      set_error("setup: tanh profile needs max_phi > 0");
+      return SMG_ERR_ARG;
+    }
+    // cell-list levels: c0 >= 2 h_max (+ margin), c_L = c0 / 2^L >= 2 h_min (+ margin)
+    SpacingProfile prof(c, dl, J.max_phi > 0. ? J.max_phi : 1.0);
+    const double hmax = prof.dx_max() * c.cell_ratio / c.cut_off;
+    const double hmin = prof.dx_min() * c.cell_ratio / c.cut_off;
+    if (!(hmin > 0.)) { set_error("setup: spacing must be positive"); return SMG_ERR_ARG; }
+    const double R0 = hmax * c.cut_off * (1. + 1.e-9) + 1.e-9;
+    int64_t nbox = int64_t(std::floor(dmin / R0));
+    if (nbox < 1) nbox = 1;
+    J.c0 = dmin / double(nbox);
+    const int max_bits = D_ == 2 ? 15 : 10;
+    int nlev = 1;
+    while (nlev < kMaxLevels && std::ldexp(J.c0, -nlev) >= hmin * c.cut_off * (1. + 1.e-9) + 2.e-10) nlev++;
+    int bits0 = 0;
+    for (int a = 0; a < D_; a++) bits0 = std::max(bits0, ceil_log2(int64_t(std::ceil(J.domain[a] / J.c0))));
+    while (nlev > 1 && bits0 + nlev - 1 > max_bits) nlev--;
+    if (bits0 > max_bits) { set_error("setup: domain has too many cells per axis"); return SMG_ERR_CAPACITY; }
+    J.nlev = nlev;
+    int64_t off = 0;
+    for (int L = 0; L < nlev; L++) {
+      J.clen[L] = std::ldexp(J.c0, -L);
+      J.cinv[L] = 1.0 / J.clen[L];
+      for (int a = 0; a < 3; a++) J.lev_n[L][a] = a < D_ ? int32_t(std::ceil(J.domain[a] / J.clen[L])) : 1;
+      J.lev_bits[L] = bits0 + L;
+      J.lev_off[L] = off;
+      off += int64_t(1) << (D_ * J.lev_bits[L]);
+    }
+    J.tbl_size = off + 1;   // + pad cell for dead slots
+    J.tbl_base = tbl;
+    tbl += J.tbl_size;
+    J.lset_off = ls;
+    ls += int64_t(J.ng[0]) * J.ng[1] * J.ng[2];
+    J.n = jobs[j].n;
+    J.cap = (J.n + kTile - 1) / kTile * kTile;
+    J.p0 = p0;
+    p0 += J.cap;
+  }
+  if (tbl >= (int64_t(1) << 31) - 2) { set_error("setup: cell tables exceed 2^31 entries"); return SMG_ERR_CAPACITY; }
+  if (p0 >= (int64_t(1) << kJBits)) { set_error("setup: more than 2^26 particle slots"); return SMG_ERR_CAPACITY; }
+  cap_total_ = p0;
+  ntiles_ = p0 / kTile;
+  n_tbl_ = tbl;
+  n_ls_ = ls;
+  tile_job_h_.assign(ntiles_, 0);
+  for (size_t j = 0; j < jobs.size(); j++)
+    for (int64_t t = jp_[j].p0 / kTile; t < (jp_[j].p0 + jp_[j].cap) / kTile; t++) tile_job_h_[t] = int32_t(j);
+
+  // device buffers
+  d_jp_ = alloc_<JobParams>(int64_t(jp_.size()));
+  d_tile_job_ = alloc_<int32_t>(ntiles_);
+  d_red_ = alloc_<JobRed>(int64_t(jp_.size()));
+  d_ls_ = alloc_<LsCell>(n_ls_);
+  PState* S2[2] = {&A_, &B_};
+  for (PState* S : S2) {
+    for (int a = 0; a < 3; a++) {
+      S->xh[a] = alloc_<float>(cap_total_);
+      S->xl[a] = alloc_<float>(cap_total_);
+      S->nrm[a] = alloc_<float>(cap_total_);
+    }
+    S->h = alloc_<double>(cap_total_);
+    S->type = alloc_<int32_t>(cap_total_);
+    S->id = alloc_<int32_t>(cap_total_);
+  }
+  W_.nu_a = alloc_<float>(cap_total_);
+  W_.nu_b = alloc_<float>(cap_total_);
+  for (int a = 0; a < 3; a++) {
+    W_.a1[a] = alloc_<float>(cap_total_);
+    W_.a2[a] = alloc_<float>(cap_total_);
+  }
+  W_.sum = alloc_<float>(cap_total_);
+  W_.key = alloc_<uint32_t>(cap_total_);
+  W_.skey = alloc_<uint32_t>(cap_total_);
+  W_.perm = alloc_<int32_t>(cap_total_);
+  W_.counts = alloc_<int32_t>(n_tbl_);
+  W_.starts = alloc_<int32_t>(n_tbl_ + 1);
+  const int64_t nb = (n_tbl_ + kScanItems - 1) / kScanItems;
+  W_.bsum = alloc_<int32_t>(nb + 1);
+  W_.nbl = alloc_<int32_t>(int64_t(K_) * cap_total_);
+  W_.nbc = alloc_<int32_t>(cap_total_);
+  W_.step_ctr = alloc_<int64_t>(1);
+  for (void* p : allocs_)
+    if (!p) { set_error("setup: hipMalloc failed"); return SMG_ERR_HIP; }
+  if (std::find(allocs_.begin(), allocs_.end(), nullptr) != allocs_.end() || !W_.nbc) {
+    set_error("setup: device allocation failed");
+    return SMG_ERR_HIP;
+  }
+  W_.jp = d_jp_;
+  W_.tile_job = d_tile_job_;
+  W_.red = d_red_;
+  W_.ls = d_ls_;
+  W_.cap_total = cap_total_;
+  W_.n_tbl = n_tbl_;
+  W_.K = K_;
+
+  // level sets packed to 24-byte cells
+  std::vector<LsCell> hls(n_ls_);
+  for (size_t j = 0; j < jobs.size(); j++) {
+    const JobParams& J = jp_[j];
+    const int64_t m = int64_t(J.ng[0]) * J.ng[1] * J.ng[2];
+    if (!jobs[j].phi || !jobs[j].normal || !jobs[j].tag) { set_error("setup: level set missing"); return SMG_ERR_STATE; }
+    for (int64_t q = 0; q < m; q++) {
+      LsCell& cc = hls[J.lset_off + q];
+      cc.phi = jobs[j].phi[q];
+      cc.n[0] = jobs[j].normal[3 * q + 0];
+      cc.n[1] = jobs[j].normal[3 * q + 1];
+      cc.n[2] = jobs[j].normal[3 * q + 2];
+      cc.tag = jobs[j].tag[q];
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(d_ls_, hls.data(), sizeof(LsCell) * n_ls_, hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipMemcpyAsync(d_jp_, jp_.data(), sizeof(JobParams) * jp_.size(), hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipMemcpyAsync(d_tile_job_, tile_job_h_.data(), sizeof(int32_t) * ntiles_, hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipMemsetAsync(d_red_, 0, sizeof(JobRed) * jp_.size(), st_));
+  HIP_TRY(hipMemsetAsync(W_.counts, 0, sizeof(int32_t) * n_tbl_, st_));
+  HIP_TRY(hipMemsetAsync(W_.nbc, 0, sizeof(int32_t) * cap_total_, st_));
+  HIP_TRY(hipMemsetAsync(W_.step_ctr, 0, sizeof(int64_t), st_));
+  HIP_TRY(hipMemsetAsync(W_.nu_a, 0, sizeof(float) * cap_total_, st_));
+  HIP_TRY(hipMemsetAsync(W_.nu_b, 0, sizeof(float) * cap_total_, st_));
+  HIP_TRY(hipMemsetAsync(W_.sum, 0, sizeof(float) * cap_total_, st_));
+  for (int a = 0; a < 3; a++) {
+    HIP_TRY(hipMemsetAsync(W_.a1[a], 0, sizeof(float) * cap_total_, st_));
+    HIP_TRY(hipMemsetAsync(W_.a2[a], 0, sizeof(float) * cap_total_, st_));
+  }
+  HIP_TRY(hipStreamSynchronize(st_));
+  red_h_.assign(jp_.size(), JobRed{});
+  return SMG_OK;
+}
+
+void Engine::begin_(const char* name) {
+  if (!profiling_) return;
+  if (pool_.size() < 2) {
+    hipEvent_t e;
+    for (int q = 0; q < 32; q++) {
+      if (hipEventCreate(&e) == hipSuccess) pool_.push_back(e);
+    }
+  }
+  if (pool_.size() < 2) return;
+  cur_a_ = pool_.back();
+  pool_.pop_back();
+  cur_name_ = name;
+  (void)hipEventRecord(cur_a_, st_);
+}
+void Engine::end_() {
+  if (!profiling_ || !cur_a_) return;
+  hipEvent_t b = pool_.back();
+  pool_.pop_back();
+  (void)hipEventRecord(b, st_);
+  recs_.push_back({cur_name_, cur_a_, b});
+  cur_a_ = nullptr;
+}
+
+int Engine::sort_particles() {
+  begin_("k_scan");
+  launch_scan(st_, W_);
+  end_();
+  begin_("k_scatter");
+  launch_scatter(st_, ntiles_, W_);
+  end_();
+  begin_("k_segsort");
+  launch_segsort(st_, ntiles_, W_, A_);
+  end_();
+  begin_("k_gather");
+  launch_gather(st_, D_, ntiles_, W_, A_, B_);
+  end_();
+  std::swap(A_, B_);
+  HIP_TRY(hipGetLastError());
+  return SMG_OK;
+}
+
+int Engine::seed() {
+  HIP_TRY(hipMemsetAsync(d_red_, 0, sizeof(JobRed) * jp_.size(), st_));
+  launch_seed(st_, D_, ntiles_, W_, A_);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st_));
+  SMG_TRY(download_red_());
+  for (auto& r : red_h_)
+    if (r.fail) { set_error("seed: no positive-phase position found within remap_max_attempts"); return SMG_ERR_REMAP; }
+  // Set_particle_info at the seeded positions, cell keys, first sort
+  launch_map(st_, D_, ntiles_, W_, A_, kModeInit, 0);
+  HIP_TRY(hipGetLastError());
+  SMG_TRY(sort_particles());
+  HIP_TRY(hipStreamSynchronize(st_));
+  step_ = 0;
+  prerun_done_ = false;
+  return SMG_OK;
+}
+
+int Engine::init_positions(int job, const double* x, const int32_t* type) {
+  const JobParams& J = jp_[job];
+  const int64_t n = J.n, cap = J.cap;
+  std::vector<float> hi(cap), lo(cap);
+  for (int a = 0; a < 3; a++) {
+    for (int64_t i = 0; i < cap; i++) {
+      const double v = (i < n && a < D_) ? x[3 * i + a] : 0.;
+      hi[i] = float(v);
+      lo[i] = float(v - double(hi[i]));
+    }
+    HIP_TRY(hipMemcpyAsync(A_.xh[a] + J.p0, hi.data(), sizeof(float) * cap, hipMemcpyHostToDevice, st_));
+    HIP_TRY(hipMemcpyAsync(A_.xl[a] + J.p0, lo.data(), sizeof(float) * cap, hipMemcpyHostToDevice, st_));
+    HIP_TRY(hipStreamSynchronize(st_));
+  }
+  std::vector<int32_t> ty(cap), id(cap);
+  for (int64_t i = 0; i < cap; i++) {
+    ty[i] = i < n ? (type ? type[i] : SMG_REAL) : -1;
+    id[i] = i < n ? int32_t(i) : -1;
+  }
+  HIP_TRY(hipMemcpyAsync(A_.type + J.p0, ty.data(), sizeof(int32_t) * cap, hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipMemcpyAsync(A_.id + J.p0, id.data(), sizeof(int32_t) * cap, hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipStreamSynchronize(st_));
+  return SMG_OK;
+}
+
+int Engine::init_finish() {
+  // Set_particle_info at the given positions, cell keys, first sort
+  HIP_TRY(hipMemsetAsync(d_red_, 0, sizeof(JobRed) * jp_.size(), st_));
+  launch_map(st_, D_, ntiles_, W_, A_, kModeInit, 0);
+  HIP_TRY(hipGetLastError());
+  SMG_TRY(sort_particles());
+  HIP_TRY(hipStreamSynchronize(st_));
+  step_ = 0;
+  prerun_done_ = false;
+  return SMG_OK;
+}
+
+int Engine::prerun() {
+  // Pre_run: Get_particle_info, Set_particle_scale, Find_and_map_particle; the force
+  // passes that follow see sum = 0 and nu = 0 and leave a = 0, v = 0, nu = 0.
+  HIP_TRY(hipMemsetAsync(d_red_, 0, sizeof(JobRed) * jp_.size(), st_));
+  begin_("k_map");
+  launch_map(st_, D_, ntiles_, W_, A_, kModePrerun, 0);
+  end_();
+  HIP_TRY(hipGetLastError());
+  SMG_TRY(sort_particles());
+  HIP_TRY(hipMemsetAsync(W_.nu_a, 0, sizeof(float) * cap_total_, st_));
+  HIP_TRY(hipMemsetAsync(W_.nu_b, 0, sizeof(float) * cap_total_, st_));
+  HIP_TRY(hipMemsetAsync(W_.step_ctr, 0, sizeof(int64_t), st_));
+  for (int a = 0; a < 3; a++) {
+    HIP_TRY(hipMemsetAsync(W_.a1[a], 0, sizeof(float) * cap_total_, st_));
+    HIP_TRY(hipMemsetAsync(W_.a2[a], 0, sizeof(float) * cap_total_, st_));
+  }
+  HIP_TRY(hipStreamSynchronize(st_));
+  SMG_TRY(download_red_());
+  // the dt slots stay 0 so the first step moves nothing (v = 0 after Pre_run)
+  for (auto& r : red_h_)
+    if (r.fail) { set_error("prerun: re-seeding found no positive-phase position"); return SMG_ERR_REMAP; }
+  step_ = 0;
+  prerun_done_ = true;
+  return SMG_OK;
+}
+
+int Engine::step(int64_t nsteps) {
+  if (!prerun_done_) { set_error("step: call prerun (or seed) first"); return SMG_ERR_STATE; }
+  for (int64_t s = 0; s < nsteps; s++) {
+    const int64_t idx = step_ + 1;
+    const int par = int(idx & 1);
+    launch_reset(st_, W_, int(jp_.size()), par);
+    begin_("k_map");
+    launch_map(st_, D_, ntiles_, W_, A_, kModeStep, par);
+    end_();
+    SMG_TRY(sort_particles());
+    begin_("k_nb");
+    launch_nb(st_, D_, ntiles_, W_, A_);
+    end_();
+    begin_("k_force1");
+    launch_force1(st_, D_, ntiles_, W_, A_, par);
+    end_();
+    begin_("k_force2");
+    launch_force2(st_, D_, ntiles_, W_, A_, par);
+    end_();
+    HIP_TRY(hipGetLastError());
+    step_ = idx;
+  }
+  return SMG_OK;
+}
+
+int Engine::capture_graph(int64_t steps_per_graph) {
+  if (steps_per_graph <= 0 || (steps_per_graph & 1)) { set_error("capture_graph: need an even step count"); return SMG_ERR_ARG; }
+  if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+  if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+  const bool prof = profiling_;
+  profiling_ = false;
+  const int64_t save = step_;
+  HIP_TRY(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+  int rc = step(steps_per_graph);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st_, &g);
+  profiling_ = prof;
+  step_ = save;   // capture records only; nothing ran
+  if (rc != SMG_OK) return rc;
+  if (e != hipSuccess) { set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e)); return SMG_ERR_HIP; }
+  graph_ = g;
+  HIP_TRY(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+  graph_steps_ = steps_per_graph;
+  return SMG_OK;
+}
+
+// Each replay runs graph_steps_ steps; the parity and the A/B buffer roles repeat
+// every two steps, and the re-seeding RNG stream comes from the device step counter,
+// so a replay is the same computation as the eager steps it stands for.
+int Engine::run_graph(int64_t launches) {
+  if (!graph_exec_) { set_error("run_graph: no graph"); return SMG_ERR_STATE; }
+  if (step_ & 1) { set_error("run_graph: step index must be even"); return SMG_ERR_STATE; }
+  for (int64_t q = 0; q < launches; q++) {
+    HIP_TRY(hipGraphLaunch(graph_exec_, st_));
+    step_ += graph_steps_;
+  }
+  return SMG_OK;
+}
+
+int Engine::sync() {
+  HIP_TRY(hipStreamSynchronize(st_));
+  return SMG_OK;
+}
+
+int Engine::download_red_() {
+  HIP_TRY(hipMemcpyAsync(red_h_.data(), d_red_, sizeof(JobRed) * jp_.size(), hipMemcpyDeviceToHost, st_));
+  HIP_TRY(hipStreamSynchronize(st_));
+  return SMG_OK;
+}
+
+int Engine::check_health() {
+  SMG_TRY(download_red_());
+  for (size_t j = 0; j < red_h_.size(); j++) {
+    if (red_h_[j].fail) { set_error("job " + std::to_string(j) + ": re-seeding failed"); return SMG_ERR_REMAP; }
+    if (red_h_[j].overflow) {
+      set_error("job " + std::to_string(j) + ": neighbour list overflow (need " + std::to_string(red_h_[j].maxnb) +
+                " > max_neighbors " + std::to_string(K_) + ")");
+      return SMG_ERR_CAPACITY;
+    }
+  }
+  return SMG_OK;
+}
+
+static double bits_to_double(unsigned long long b) {
+  double d;
+  std::memcpy(&d, &b, sizeof(d));
+  return d;
+}
+
+int Engine::get_state(int job, double* x, double* v, int32_t* type, double* h, float* nu) {
+  const JobParams& J = jp_[job];
+  const int64_t cap = J.cap, n = J.n;
+  std::vector<int32_t> id(cap), ty(cap);
+  std::vector<double> hh(cap);
+  std::vector<float> xh(cap), xl(cap), a1(cap), a2(cap), nn(cap);
+  HIP_TRY(hipStreamSynchronize(st_));
+  HIP_TRY(hipMemcpy(id.data(), A_.id + J.p0, sizeof(int32_t) * cap, hipMemcpyDeviceToHost));
+  if (type) {
+    HIP_TRY(hipMemcpy(ty.data(), A_.type + J.p0, sizeof(int32_t) * cap, hipMemcpyDeviceToHost));
+    for (int64_t s = 0; s < n; s++) type[id[s]] = ty[s];
+  }
+  if (h) {
+    HIP_TRY(hipMemcpy(hh.data(), A_.h + J.p0, sizeof(double) * cap, hipMemcpyDeviceToHost));
+    for (int64_t s = 0; s < n; s++) h[id[s]] = hh[s];
+  }
+  if (nu) {
+    // after a step the current nu is the one the stage-2 pass just wrote
+    HIP_TRY(hipMemcpy(nn.data(), (step_ > 0 ? W_.nu_b : W_.nu_a) + J.p0, sizeof(float) * cap, hipMemcpyDeviceToHost));
+    for (int64_t s = 0; s < n; s++) nu[id[s]] = nn[s];
+  }
+  if (x) {
+    for (int a = 0; a < 3; a++) {
+      if (a >= D_) {
+        for (int64_t i = 0; i < n; i++) x[3 * i + a] = 0.;
+        continue;
+      }
+      HIP_TRY(hipMemcpy(xh.data(), A_.xh[a] + J.p0, sizeof(float) * cap, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(xl.data(), A_.xl[a] + J.p0, sizeof(float) * cap, hipMemcpyDeviceToHost));
+      for (int64_t s = 0; s < n; s++) x[3 * id[s] + a] = double(xh[s]) + double(xl[s]);
+    }
+  }
+  if (v) {
+    SMG_TRY(download_red_());
+    const int par = int(step_ & 1);
+    const double dt1 = step_ > 0 ? bits_to_double(red_h_[job].dt1[par]) : 0.;
+    const double dt2 = step_ > 0 ? bits_to_double(red_h_[job].dt2[par]) : 0.;
+    for (int a = 0; a < 3; a++) {
+      if (a >= D_ || step_ == 0) {
+        for (int64_t i = 0; i < n; i++) v[3 * i + a] = 0.;
+        continue;
+      }
+      HIP_TRY(hipMemcpy(a1.data(), W_.a1[a] + J.p0, sizeof(float) * cap, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(a2.data(), W_.a2[a] + J.p0, sizeof(float) * cap, hipMemcpyDeviceToHost));
+      for (int64_t s = 0; s < n; s++) {
+        double vv = double(a1[s]) * (0.5 * dt1);
+        vv = vv + double(a2[s]) * (0.5 * dt2);
+        v[3 * id[s] + a] = vv;
+      }
+    }
+  }
+  return SMG_OK;
+}
+
+int Engine::get_pair_terms(int job, float* sum, float* a1, float* a2) {
+  const JobParams& J = jp_[job];
+  const int64_t cap = J.cap, n = J.n;
+  std::vector<int32_t> id(cap);
+  std::vector<float> buf(cap);
+  HIP_TRY(hipStreamSynchronize(st_));
+  HIP_TRY(hipMemcpy(id.data(), A_.id + J.p0, sizeof(int32_t) * cap, hipMemcpyDeviceToHost));
+  if (sum) {
+    HIP_TRY(hipMemcpy(buf.data(), W_.sum + J.p0, sizeof(float) * cap, hipMemcpyDeviceToHost));
+    for (int64_t s = 0; s < n; s++) sum[id[s]] = buf[s];
+  }
+  float* outs[2] = {a1, a2};
+  for (int w = 0; w < 2; w++) {
+    if (!outs[w]) continue;
+    for (int a = 0; a < 3; a++) {
+      if (a >= D_) {
+        for (int64_t i = 0; i < n; i++) outs[w][3 * i + a] = 0.f;
+        continue;
+      }
+      HIP_TRY(hipMemcpy(buf.data(), (w == 0 ? W_.a1[a] : W_.a2[a]) + J.p0, sizeof(float) * cap, hipMemcpyDeviceToHost));
+      for (int64_t s = 0; s < n; s++) outs[w][3 * id[s] + a] = buf[s];
+    }
+  }
+  return SMG_OK;
+}
+
+int Engine::get_neighbors(int job, int64_t* offs, int32_t* ids, int32_t* codes, int64_t capacity) {
+  if (step_ == 0) { set_error("get_neighbors: no step taken yet"); return SMG_ERR_STATE; }
+  const JobParams& J = jp_[job];
+  const int64_t cap = J.cap, n = J.n;
+  std::vector<int32_t> id(cap_total_), cnt(cap);
+  HIP_TRY(hipStreamSynchronize(st_));
+  HIP_TRY(hipMemcpy(id.data(), A_.id, sizeof(int32_t) * cap_total_, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(cnt.data(), W_.nbc + J.p0, sizeof(int32_t) * cap, hipMemcpyDeviceToHost));
+  std::vector<int32_t> ent(int64_t(K_) * cap);
+  for (int k = 0; k < K_; k++)
+    HIP_TRY(hipMemcpy(ent.data() + int64_t(k) * cap, W_.nbl + int64_t(k) * cap_total_ + J.p0, sizeof(int32_t) * cap,
+                      hipMemcpyDeviceToHost));
+  std::vector<int64_t> count_by_id(n, 0);
+  std::vector<int64_t> slot_of(n, -1);
+  for (int64_t s = 0; s < n; s++) {
+    count_by_id[id[J.p0 + s]] = cnt[s];
+    slot_of[id[J.p0 + s]] = s;
+  }
+  offs[0] = 0;
+  for (int64_t i = 0; i < n; i++) offs[i + 1] = offs[i] + count_by_id[i];
+  if (!ids) return SMG_OK;
+  if (offs[n] > capacity) { set_error("get_neighbors: capacity too small"); return SMG_ERR_CAPACITY; }
+  std::vector<std::pair<int32_t, int32_t>> tmp;
+  for (int64_t i = 0; i < n; i++) {
+    const int64_t s = slot_of[i];
+    tmp.clear();
+    for (int k = 0; k < cnt[s]; k++) {
+      const uint32_t e = uint32_t(ent[int64_t(k) * cap + s]);
+      const int64_t q = e & kJMask;
+      tmp.push_back({id[q], int32_t(e >> kJBits)});
+    }
+    std::sort(tmp.begin(), tmp.end());
+    for (size_t k = 0; k < tmp.size(); k++) {
+      ids[offs[i] + k] = tmp[k].first;
+      codes[offs[i] + k] = tmp[k].second;
+    }
+  }
+  return SMG_OK;
+}
+
+int Engine::get_stats(int job, smg_stats* st) {
+  std::memset(st, 0, sizeof(*st));
+  const int par = int(step_ & 1);
+  if (step_ > 0) {
+    launch_residual(st_, D_, ntiles_, W_, A_, int(jp_.size()), par);
+    HIP_TRY(hipGetLastError());
+  }
+  SMG_TRY(download_red_());
+  const JobParams& J = jp_[job];
+  const JobRed& R = red_h_[job];
+  st->step = step_;
+  st->n_particle = J.n;
+  std::vector<int32_t> ty(J.cap);
+  HIP_TRY(hipMemcpy(ty.data(), A_.type + J.p0, sizeof(int32_t) * J.cap, hipMemcpyDeviceToHost));
+  for (int64_t s = 0; s < J.n; s++) {
+    st->n_real += ty[s] == SMG_REAL;
+    st->n_surface += ty[s] == SMG_SURFACE;
+  }
+  st->n_remapped = R.nremap[par];
+  st->max_neighbors_seen = R.maxnb;
+  st->num_level = J.nlev;
+  if (step_ > 0) {
+    st->dt1 = bits_to_double(R.dt1[par]);
+    st->dt2 = bits_to_double(R.dt2[par]);
+    st->max_v = bits_to_double(R.maxv[par]);
+    st->residual = bits_to_double(R.res[par]);
+    st->max_disp = bits_to_double(R.maxdisp[par]);
+    std::vector<int32_t> cnt(J.cap);
+    HIP_TRY(hipMemcpy(cnt.data(), W_.nbc + J.p0, sizeof(int32_t) * J.cap, hipMemcpyDeviceToHost));
+    for (int64_t s = 0; s < J.n; s++) st->n_neighbor_entries += cnt[s];
+  }
+  return SMG_OK;
+}
+
+int Engine::profile_get(std::vector<std::string>& names, std::vector<double>& avg_ms, std::vector<int64_t>& counts) {
+  HIP_TRY(hipStreamSynchronize(st_));
+  for (auto& r : recs_) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      auto& p = prof_[r.name];
+      p.first += ms;
+      p.second += 1;
+      last_ms_[r.name] = ms;
+    }
+    pool_.push_back(r.a);
+    pool_.push_back(r.b);
+  }
+  recs_.clear();
+  names.clear();
+  avg_ms.clear();
+  counts.clear();
+  for (auto& kv : prof_) {
+    names.push_back(kv.first);
+    avg_ms.push_back(kv.second.second ? kv.second.first / double(kv.second.second) : 0.);
+    counts.push_back(kv.second.second);
+  }
+  return SMG_OK;
+}
+
+double Engine::last_kernel_ms(const std::string& name) const {
+  auto it = last_ms_.find(name);
+  return it == last_ms_.end() ? 0. : it->second;
+}
+
+}  // namespace smg

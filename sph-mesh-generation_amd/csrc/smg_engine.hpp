@@ -1,0 +1,110 @@
+// smg_engine.hpp -- host orchestration of the relaxation kernels.
+//
+// An Engine holds the device state of one or more jobs of the same dimension in one
+// slot space, on one HIP stream.  A single relaxation job (smg_create) is an Engine
+// with one job; a batch is an Engine with many, launched together.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "smg.h"
+#include "smg_device.hpp"
+
+namespace smg {
+
+struct JobInput {
+  smg_config cfg;
+  int64_t n = 0;
+  double max_phi = 0.;            // level-set maximum_phi (tanh profile)
+  const double* phi = nullptr;    // one-cell-halo level-set grid
+  const float* normal = nullptr;
+  const int8_t* tag = nullptr;
+};
+
+class Engine {
+ public:
+  Engine() = default;
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  // Allocates and uploads everything for the given jobs (all of dimension D).
+  int setup(const std::vector<JobInput>& jobs, int max_neighbors, hipStream_t stream = nullptr);
+
+  int seed();                                                   // Random_particle_distribution
+  int init_positions(int job, const double* x, const int32_t* type);  // upload one job's positions
+  int init_finish();                                            // Set_particle_info + first sort
+  int prerun();                                                 // Pre_run
+  int step(int64_t nsteps);                                     // SPH_run_mesh_generation x n
+  int sync();
+
+  // results (host, by particle id of the job)
+  int get_state(int job, double* x, double* v, int32_t* type, double* h, float* nu);
+  int get_pair_terms(int job, float* sum, float* a1, float* a2);
+  int get_neighbors(int job, int64_t* offs, int32_t* ids, int32_t* codes, int64_t cap);
+  int get_stats(int job, smg_stats* st);
+  int check_health();   // remap failures, neighbour overflow
+
+  void profile_enable(bool on) { profiling_ = on; }
+  int profile_get(std::vector<std::string>& names, std::vector<double>& avg_ms, std::vector<int64_t>& counts);
+
+  int dim() const { return D_; }
+  int64_t step_index() const { return step_; }
+  int num_jobs() const { return int(jp_.size()); }
+  int64_t total_particles() const;
+  const JobParams& params(int j) const { return jp_[j]; }
+  hipStream_t stream() const { return st_; }
+  double last_kernel_ms(const std::string& name) const;
+
+  // graph-capturable step loop body (two steps so the buffer swap returns to A)
+  int capture_graph(int64_t steps_per_graph);
+  int run_graph(int64_t launches);
+  bool has_graph() const { return graph_exec_ != nullptr; }
+
+ private:
+  int sort_particles();
+  void begin_(const char* name);
+  void end_();
+  int download_red_();
+
+  int D_ = 0;
+  hipStream_t st_ = nullptr;
+  bool own_stream_ = false;
+  std::vector<JobParams> jp_;
+  std::vector<int32_t> tile_job_h_;
+  int64_t ntiles_ = 0, cap_total_ = 0, n_tbl_ = 0, n_ls_ = 0;
+  int K_ = 64;
+  int64_t step_ = 0;   // last completed step index (0 = pre-run)
+  bool prerun_done_ = false;
+
+  JobParams* d_jp_ = nullptr;
+  int32_t* d_tile_job_ = nullptr;
+  JobRed* d_red_ = nullptr;
+  LsCell* d_ls_ = nullptr;
+  PState A_{}, B_{};
+  Work W_{};
+  std::vector<void*> allocs_;
+  std::vector<JobRed> red_h_;
+
+  // profiling
+  bool profiling_ = false;
+  struct Rec { std::string name; hipEvent_t a, b; };
+  std::vector<Rec> recs_;
+  std::vector<hipEvent_t> pool_;
+  std::map<std::string, std::pair<double, int64_t>> prof_;
+  std::map<std::string, double> last_ms_;
+  hipEvent_t cur_a_ = nullptr;
+  std::string cur_name_;
+
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  int64_t graph_steps_ = 0;
+
+  template <typename T>
+  T* alloc_(int64_t count);
+};
+
+}  // namespace smg

@@ -1,0 +1,1053 @@
+// smg_kernels.hip -- gfx950 kernels of one relaxation step.
+//
+// Pass structure (one step = SPH_run_mesh_generation, sph_mesh_generation.cpp:1371-1463):
+//   k_reset    per-job reduction slots of this step
+//   k_map      P1: x += v dt (Update_coord_full_step), level-set sample
+//              (Calculate_particle_infor), surface projection / re-seeding
+//              (Find_and_map_particle), target spacing (Set_particle_scale), cell key
+//   k_scan_*   exclusive scan of the per-cell counts (cell starts)
+//   k_scatter  counting-sort scatter by cell key
+//   k_segsort  order inside each cell by particle id (makes the sort deterministic)
+//   k_gather   move the particle fields into the new order
+//   k_nb       neighbour lists with mirror ghosts, FP64 predicate
+//              (Level_info::Refresh_neighbor_info, level_infor.cpp:469-506)
+//   k_force1   P3: kernel summation + repulsion + stage-1 dt
+//              (Get_kernel_summation, Accumulate_particle_force(1), Set_timestep(1))
+//   k_force2   P4: viscous force + stage-2 dt + nu
+//              (Accumulate_particle_force(2), Set_timestep(2))
+// The two velocity half steps fold into the next k_map: v = a1 dt1/2 + a2 dt2/2.
+//
+// FP64 is used where the result must equal the reference's bit for bit (level-set
+// sampling, projection, neighbour predicate) and for the dt / residual reductions;
+// the pair sums run in FP32.  The file is compiled with -ffp-contract=off so that no
+// FMA changes an FP64 rounding.
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "smg_device.hpp"
+#include "smg_kernels.hpp"
+
+namespace smg {
+
+// ---------------------------------------------------------------------------
+// helpers
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t rng_key(uint64_t seed, uint64_t stream, uint64_t id) {
+  const uint64_t k = mix64(seed + 0x632BE59BD9B4E019ull * (stream + 1ull));
+  return mix64(k ^ (id * 0xD1B54A32D192ED03ull + 0x8CB92BA72F3D8DD7ull));
+}
+__device__ __forceinline__ double rng_u(uint64_t key, uint64_t counter) {
+  return double(mix64(key + counter * 0x9E3779B97F4A7C15ull) >> 11) * 0x1.0p-53;
+}
+
+__device__ __forceinline__ double amin(double a, double b) { return (a <= b) ? a : b; }  // AMIN1
+__device__ __forceinline__ double amax(double a, double b) { return (a >= b) ? a : b; }  // AMAX1
+
+__device__ __forceinline__ unsigned long long dbits(double v) { return (unsigned long long)__double_as_longlong(v); }
+__device__ __forceinline__ double bitsd(unsigned long long v) { return __longlong_as_double((long long)v); }
+
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_reduce(T v, Op op) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// reduce over a 256-thread workgroup; the result is valid in thread 0
+template <typename T, typename Op>
+__device__ __forceinline__ T block_reduce(T v, Op op, T* lds4) {
+  v = wave_reduce(v, op);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) lds4[w] = v;
+  __syncthreads();
+  T r = lds4[0];
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++) r = op(r, lds4[q]);
+  }
+  __syncthreads();
+  return r;
+}
+
+struct OpMin { __device__ double operator()(double a, double b) const { return a < b ? a : b; } };
+struct OpMax { __device__ double operator()(double a, double b) const { return a > b ? a : b; } };
+struct OpAdd { __device__ int operator()(int a, int b) const { return a + b; } };
+struct OpMaxI { __device__ int operator()(int a, int b) const { return a > b ? a : b; } };
+
+__device__ __forceinline__ int64_t ls_index(const JobParams& J, int D, int gx, int gy, int gz) {
+  const int64_t a = gx + 1, b = (D > 1) ? gy + 1 : 0, c = (D > 2) ? gz + 1 : 0;
+  return J.lset_off + (c * J.ng[1] + b) * J.ng[0] + a;
+}
+
+// clamped shift into the level-set box (particle_mesh_generation.cpp:182-185) and
+// package index (get_cell_id with glbl_pkg_start = -1, glbl_pkg_end = num_pkg + 1)
+template <int D>
+__device__ __forceinline__ void ls_locate(const JobParams& J, const double* x, double* cs, int* pk) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (a < D) {
+      double t = x[a] - J.box_l[a];
+      t = (t >= 0.) ? t : 0.;
+      const double hi = J.domain[a] - 1.e-8;
+      cs[a] = (t <= hi) ? t : hi;
+      int p = int(floor(cs[a] / J.dpkg[a]));
+      const int phi_ = J.num_pkg[a];
+      p = (p <= phi_) ? p : phi_;
+      p = (-1 >= p) ? -1 : p;
+      pk[a] = p;
+    } else {
+      cs[a] = 0.;
+      pk[a] = 0;
+    }
+  }
+}
+
+// Calculate_particle_infor (particle_mesh_generation.cpp:175-265): bi/trilinear phi
+// and raw normal; the normal is left unnormalised (quirk Q3)
+template <int D>
+__device__ void ls_sample(const JobParams& J, const LsCell* __restrict__ ls, const double* x, double& phi, double* n) {
+  double cs[3];
+  int pk[3];
+  ls_locate<D>(J, x, cs, pk);
+  int m[3] = {0, 0, 0};
+  double f[3] = {0., 0., 0.};
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    double rr = (cs[a] - J.dpkg[a] * pk[a]) / J.dcell[a] - 0.5;
+    rr = (rr <= 3.5) ? rr : 3.5;
+    rr = (-0.5 >= rr) ? -0.5 : rr;
+    m[a] = int(floor(rr));
+    f[a] = rr - double(m[a]);
+  }
+  phi = 0.;
+  n[0] = n[1] = n[2] = 0.;
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int s = 0; s < (D > 1 ? 2 : 1); s++)
+#pragma unroll
+      for (int t = 0; t < (D > 2 ? 2 : 1); t++) {
+        const LsCell c = ls[ls_index(J, D, pk[0] * 4 + m[0] + r, pk[1] * 4 + m[1] + s, pk[2] * 4 + m[2] + t)];
+        double coef = fabs(f[0] - (1.0 - r));
+        if (D > 1) coef = coef * fabs(f[1] - (1.0 - s));
+        if (D > 2) coef = coef * fabs(f[2] - (1.0 - t));
+        phi += c.phi * coef;
+        n[0] += double(c.n[0]) * coef;
+        n[1] += double(c.n[1]) * coef;
+        n[2] += double(c.n[2]) * coef;
+      }
+}
+
+// Levelset_levelinfo::Get_id_pkg_cell (lset_level_infor.cpp:350-366) -> tag_interface
+template <int D>
+__device__ int ls_tag(const JobParams& J, const LsCell* __restrict__ ls, const double* x) {
+  double cs[3];
+  int pk[3];
+  ls_locate<D>(J, x, cs, pk);
+  int g[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    int c = int(floor((cs[a] - J.dpkg[a] * pk[a]) / J.dcell[a]));
+    c = (c <= 3) ? c : 3;
+    c = (0 >= c) ? 0 : c;
+    g[a] = pk[a] * 4 + c;
+  }
+  return ls[ls_index(J, D, g[0], g[1], g[2])].tag;
+}
+
+// Levelset::Get_scale (level_set.cpp:311-351) and the linear-profile extension
+__device__ __forceinline__ double get_scale(const JobParams& J, double phi) {
+  if (J.profile == SMG_PROFILE_LINEAR) {
+    if (phi > 0. || fabs(phi) < J.dl) {
+      double t = phi > 0. ? phi / J.lin_band : 0.;
+      t = t < 1.0 ? t : 1.0;
+      return J.lin_dx0 + t * (J.lin_dx1 - J.lin_dx0);
+    }
+    return J.lin_dx0;
+  }
+  if (phi > 0. || fabs(phi) < J.dl) return J.min_dl + tanh(2.5 * fabs(phi) / J.max_phi) / J.tanh25 * (J.max_dl - J.min_dl);
+  return J.min_dl;
+}
+// Set_particle_info (particle_mesh_generation.cpp:361-381): h only; rho and vol
+// follow from dx = h * cut_off / cell_ratio where they are used
+__device__ __forceinline__ double h_of_phi(const JobParams& J, double phi) {
+  return get_scale(J, phi) * J.cell_ratio / J.cut_off;
+}
+
+__device__ __forceinline__ int level_of(const JobParams& J, double h) {
+  const double r = h * J.cut_off + 2.e-10;
+  int l = 0;
+  while (l + 1 < J.nlev && J.clen[l + 1] >= r) l++;
+  return l;
+}
+
+__device__ __forceinline__ uint32_t spread2(uint32_t v) {
+  v &= 0x0000FFFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x000003FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+template <int D>
+__device__ __forceinline__ uint32_t morton(const int* c) {
+  if (D == 2) return spread2(uint32_t(c[0])) | (spread2(uint32_t(c[1])) << 1);
+  return spread3(uint32_t(c[0])) | (spread3(uint32_t(c[1])) << 1) | (spread3(uint32_t(c[2])) << 2);
+}
+
+template <int D>
+__device__ __forceinline__ void cell_at(const JobParams& J, int L, const double* x, int* c) {
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    int k = int(floor((x[a] - J.box_l[a]) * J.cinv[L]));
+    const int hi = J.lev_n[L][a] - 1;
+    k = k < 0 ? 0 : (k > hi ? hi : k);
+    c[a] = k;
+  }
+}
+
+__device__ __forceinline__ void split_store(double x, float* hi, float* lo, int64_t s) {
+  const float h = float(x);
+  hi[s] = h;
+  lo[s] = float(x - double(h));
+}
+
+// ---------------------------------------------------------------------------
+// hyperbolic kernel in FP32 (particle.cpp:31-401, 405-807 with PI = 3.1415926)
+template <int DK>
+__device__ __forceinline__ float kC() {
+  return DK == 1 ? 1.0f / 7.0f : (DK == 2 ? float(1.0 / (3.0 * 3.1415926)) : float(15.0 / (62.0 * 3.1415926)));
+}
+template <int DK>
+__device__ __forceinline__ float ipow(float h) {
+  return DK == 1 ? h : (DK == 2 ? h * h : (DK == 3 ? h * h * h : h * h * h * h));
+}
+template <int DK>
+__device__ __forceinline__ float Wk(float r, float h) {
+  const float C = kC<DK>() / ipow<DK>(h);
+  const float s = r / h;
+  if (s < 1.f) return C * (6.0f - 6.0f * s + s * s * s);
+  if (s < 2.f) {
+    const float t = 2.f - s;
+    return C * t * t * t;
+  }
+  return 0.f;
+}
+template <int DK>
+__device__ __forceinline__ float dWk(float r, float h) {   // (dW/dr) / (r + 1e-20)
+  const float C = kC<DK>() / ipow<DK + 1>(h);
+  const float s = r / h;
+  float v = 0.f;
+  if (s < 1.f) v = C * (-6.0f + 3.f * s * s);
+  else if (s < 2.f) {
+    const float t = 2.f - s;
+    v = C * (-3.0f) * t * t;
+  }
+  return v / (r + 1.e-20f);
+}
+
+// relative position x_i - image(x_q) in FP32 from the hi/lo pairs; the image of a
+// mirror ghost is 2w - x_q, so its difference is (x_i - w) + (x_q - w)
+template <int D>
+__device__ __forceinline__ void rel_pos(const JobParams& J, const PState& S, int64_t i, int64_t q, uint32_t code,
+                                        const float* xhi, const float* xli, float* dr) {
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    const uint32_t m = (code >> (2 * a)) & 3u;
+    const float hq = S.xh[a][q], lq = S.xl[a][q];
+    if (m == 0) dr[a] = (xhi[a] - hq) + (xli[a] - lq);
+    else if (m == 1) dr[a] = ((xhi[a] - J.wl_h[a]) + (hq - J.wl_h[a])) + ((xli[a] + lq) - 2.f * J.wl_l[a]);
+    else dr[a] = ((xhi[a] - J.wh_h[a]) + (hq - J.wh_h[a])) + ((xli[a] + lq) - 2.f * J.wh_l[a]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reduction slots of this step
+
+__global__ void k_reset(JobRed* red, int njobs, int par, int64_t* step_ctr) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j == 0) *step_ctr += 1;   // re-seeding RNG stream of this step (graph replays advance it too)
+  if (j >= njobs) return;
+  JobRed& R = red[j];
+  R.dt1[par] = 0x7FF0000000000000ull;
+  R.dt2[par] = 0x7FF0000000000000ull;
+  R.maxv[par] = 0ull;
+  R.res[par ^ 1] = 0ull;
+  R.maxdisp[par ^ 1] = 0ull;
+  R.nremap[par] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// seeding: Random_particle_distribution (sph_mesh_generation.cpp:1302-1367)
+
+template <int D>
+__global__ __launch_bounds__(kTile) void k_seed(Work W, PState S) {
+  const int j = W.tile_job[blockIdx.x];
+  const JobParams& J = W.jp[j];
+  const int64_t slot = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const int64_t loc = slot - J.p0;
+  int bad = 0;
+  if (loc < J.n) {
+    const uint64_t key = rng_key(J.seed, 0xFFFFFFFFull, uint64_t(loc));
+    double x[3] = {0., 0., 0.}, phi = 0., n[3], h = 0.;
+    const double pad = J.dpkg[0];
+    bool ok = false;
+    for (int att = 0; att < J.remap_max && !ok; att++) {
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        const double box = J.pbox_r[a] - J.pbox_l[a];
+        const double u0 = rng_u(key, uint64_t(att) * 6 + 2 * a);
+        const double u1 = rng_u(key, uint64_t(att) * 6 + 2 * a + 1);
+        double v = u0 * box + u1 * 1.e-3 + J.pbox_l[a];
+        const double lo = J.pbox_l[a] + 1.e-10, hi = J.pbox_r[a] - 1.e-10;
+        v = (v <= hi) ? v : hi;
+        v = (lo >= v) ? lo : v;
+        const double llo = J.box_l[a] + pad, lhi = J.box_r[a] - pad;
+        v = (lhi <= v) ? lhi : v;
+        v = (llo >= v) ? llo : v;
+        x[a] = v;
+      }
+      ls_sample<D>(J, W.ls, x, phi, n);
+      h = h_of_phi(J, phi);
+      ok = phi > h;
+    }
+    bad = !ok;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      if (a < D) split_store(x[a], S.xh[a], S.xl[a], slot);
+      else { S.xh[a][slot] = 0.f; S.xl[a][slot] = 0.f; }
+      S.nrm[a][slot] = a < D ? float(n[a]) : 0.f;
+      W.a1[a][slot] = 0.f;
+      W.a2[a][slot] = 0.f;
+    }
+    S.h[slot] = h;
+    S.type[slot] = SMG_REAL;
+    S.id[slot] = int32_t(loc);
+    W.nu_b[slot] = 0.f;
+  } else {
+    S.id[slot] = -1;
+    S.type[slot] = -1;
+    S.h[slot] = 0.;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      S.xh[a][slot] = 0.f; S.xl[a][slot] = 0.f; S.nrm[a][slot] = 0.f;
+      W.a1[a][slot] = 0.f; W.a2[a][slot] = 0.f;
+    }
+    W.nu_b[slot] = 0.f;
+  }
+  if (bad) atomicOr(&W.red[j].fail, 1);
+}
+
+// ---------------------------------------------------------------------------
+// P1: integrate, sample, map, scale, key
+
+template <int D>
+__global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int par) {
+  __shared__ double lds_d[4];
+  __shared__ int lds_i[4];
+  const int j = W.tile_job[blockIdx.x];
+  const JobParams& J = W.jp[j];
+  const int64_t slot = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const int64_t loc = slot - J.p0;
+  double res = 0., md = 0.;
+  int nre = 0, bad = 0, nsurf = 0;
+  uint32_t key;
+  if (loc < J.n) {
+    double x[3] = {0., 0., 0.};
+#pragma unroll
+    for (int a = 0; a < D; a++) x[a] = double(S.xh[a][slot]) + double(S.xl[a][slot]);
+    double h = S.h[slot];
+    int type = S.type[slot];
+    if (mode == kModeStep) {
+      // the two Update_velocity_half_step of the previous step, then Update_coord_full_step
+      const double dt1 = bitsd(W.red[j].dt1[par ^ 1]), dt2 = bitsd(W.red[j].dt2[par ^ 1]);
+      double v[3] = {0., 0., 0.}, U = 0.;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        v[a] = double(W.a1[a][slot]) * (0.5 * dt1);
+        v[a] = v[a] + double(W.a2[a][slot]) * (0.5 * dt2);
+        U += v[a] * v[a];
+      }
+      U = sqrt(U);
+      md = U * dt2;
+      res = U * dt2 / (h * J.cut_off / J.cell_ratio);
+      if (type != SMG_SINGULARITY) {
+#pragma unroll
+        for (int a = 0; a < D; a++) x[a] = x[a] + v[a] * dt2;
+      }
+    }
+    double phi, n[3];
+    ls_sample<D>(J, W.ls, x, phi, n);
+    if (mode == kModeInit) {
+      h = h_of_phi(J, phi);
+    } else {
+      if (mode == kModePrerun) h = h_of_phi(J, phi);
+      // Find_and_map_particle (particle_mesh_generation.cpp:408-467)
+      const int tag = ls_tag<D>(J, W.ls, x);
+      if (tag == 1 && (type == SMG_REAL || type == SMG_SURFACE)) {
+        type = SMG_SURFACE;
+        const double dist = fabs(phi);
+        double nn = 0.;
+#pragma unroll
+        for (int a = 0; a < D; a++) nn += n[a] * n[a];
+        const double div = 1.0 / (sqrt(nn) + 1.0e-15);
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+          const double nd = phi < 0.0 ? n[a] * div : -(n[a] * div);
+          x[a] = x[a] + nd * dist;
+        }
+      } else {
+        type = SMG_REAL;
+      }
+      if (type == SMG_SURFACE) {   // Get_dispersed_position(1e-5 h) with the pre-projection phi, n (Q12)
+        const double dx = 0.00001 * h;
+        double nn = 0.;
+#pragma unroll
+        for (int a = 0; a < D; a++) nn += n[a] * n[a];
+        const double div = 1.0 / (sqrt(nn) + 1.0e-15);
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+          const double nd = phi < 0.0 ? n[a] * div : -(n[a] * div);
+          x[a] = x[a] + nd * dx;
+        }
+      }
+      ls_sample<D>(J, W.ls, x, phi, n);
+      if (phi < 0. && type == SMG_REAL) {
+        // Remap_this_particle_to_positive_phase (sph_mesh_generation.cpp:1885-1928)
+        nre = 1;
+        const uint64_t stream = mode == kModeStep ? uint64_t(*W.step_ctr) : 0ull;
+        const uint64_t key_r = rng_key(J.seed, stream, uint64_t(S.id[slot]));
+        bool ok = false;
+        for (int att = 0; att < J.remap_max && !ok; att++) {
+#pragma unroll
+          for (int a = 0; a < D; a++) {
+            const double box = J.pbox_r[a] - J.pbox_l[a];
+            const double u0 = rng_u(key_r, uint64_t(att) * 6 + 2 * a);
+            const double u1 = rng_u(key_r, uint64_t(att) * 6 + 2 * a + 1);
+            double v = u0 * box + u1 * 1.e-3 + J.pbox_l[a];
+            const double lo = J.pbox_l[a] + 1.e-10, hi = J.pbox_r[a] - 1.e-10;
+            v = (v <= hi) ? v : hi;
+            v = (lo >= v) ? lo : v;
+            x[a] = v;
+          }
+          ls_sample<D>(J, W.ls, x, phi, n);
+          ok = phi > h;
+        }
+        if (ok) {
+          type = SMG_REAL;
+          h = h_of_phi(J, phi);
+        } else {
+          bad = 1;
+        }
+      }
+      if (mode == kModeStep) h = h_of_phi(J, phi);   // Set_particle_scale
+    }
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      split_store(x[a], S.xh[a], S.xl[a], slot);
+      S.nrm[a][slot] = float(n[a]);
+    }
+    S.h[slot] = h;
+    S.type[slot] = type;
+    nsurf = type == SMG_SURFACE;
+    const int L = level_of(J, h);
+    int c[3] = {0, 0, 0};
+    cell_at<D>(J, L, x, c);
+    key = uint32_t(J.tbl_base + J.lev_off[L]) + morton<D>(c);
+  } else {
+    key = uint32_t(J.tbl_base + J.tbl_size - 1);
+  }
+  W.key[slot] = key;
+  atomicAdd(&W.counts[key], 1);
+  res = block_reduce(res, OpMax(), lds_d);
+  md = block_reduce(md, OpMax(), lds_d);
+  nre = block_reduce(nre, OpAdd(), lds_i);
+  bad = block_reduce(bad, OpMaxI(), lds_i);
+  nsurf = block_reduce(nsurf, OpAdd(), lds_i);
+  if (threadIdx.x == 0) {
+    JobRed& R = W.red[j];
+    if (mode == kModeStep) {
+      atomicMax(&R.res[par ^ 1], dbits(res));
+      atomicMax(&R.maxdisp[par ^ 1], dbits(md));
+    }
+    if (nre) atomicAdd(&R.nremap[par], nre);
+    if (bad) atomicOr(&R.fail, 1);
+    if (nsurf) atomicAdd(&R.nsurf, nsurf);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// cell starts: exclusive scan of counts over all job tables
+
+__global__ __launch_bounds__(kTile) void k_scan_reduce(const int32_t* __restrict__ counts, int64_t n, int32_t* bsum) {
+  __shared__ int lds[4];
+  const int64_t base = int64_t(blockIdx.x) * kScanItems + int64_t(threadIdx.x) * 16;
+  int s = 0;
+#pragma unroll
+  for (int q = 0; q < 16; q++) s += (base + q < n) ? counts[base + q] : 0;
+  s = block_reduce(s, OpAdd(), lds);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(int32_t* bsum, int64_t nb) {
+  __shared__ int lds[1024];
+  int carry = 0;
+  for (int64_t c0 = 0; c0 < nb; c0 += 1024) {
+    const int64_t i = c0 + threadIdx.x;
+    const int v = i < nb ? bsum[i] : 0;
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int t = threadIdx.x >= (unsigned)o ? lds[threadIdx.x - o] : 0;
+      __syncthreads();
+      lds[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) bsum[i] = carry + lds[threadIdx.x] - v;
+    const int tot = lds[1023];
+    __syncthreads();
+    carry += tot;
+  }
+  if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+__global__ __launch_bounds__(kTile) void k_scan_down(const int32_t* __restrict__ counts, int64_t n,
+                                                     const int32_t* __restrict__ bsum, int64_t nb, int32_t* starts) {
+  __shared__ int lds[kTile];
+  const int64_t base = int64_t(blockIdx.x) * kScanItems + int64_t(threadIdx.x) * 16;
+  int v[16];
+  int s = 0;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    v[q] = (base + q < n) ? counts[base + q] : 0;
+    s += v[q];
+  }
+  lds[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < kTile; o <<= 1) {
+    const int t = threadIdx.x >= (unsigned)o ? lds[threadIdx.x - o] : 0;
+    __syncthreads();
+    lds[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = bsum[blockIdx.x] + lds[threadIdx.x] - s;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    if (base + q < n) starts[base + q] = run;
+    run += v[q];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) starts[n] = bsum[nb];   // end sentinel
+}
+
+// ---------------------------------------------------------------------------
+// counting sort by cell key
+
+__global__ __launch_bounds__(kTile) void k_scatter(Work W, int64_t nslots) {
+  const int64_t p = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  if (p >= nslots) return;
+  const uint32_t k = W.key[p];
+  const int r = atomicSub(&W.counts[k], 1) - 1;
+  const int64_t d = int64_t(W.starts[k]) + r;
+  W.perm[d] = int32_t(p);
+  W.skey[d] = k;
+}
+
+__global__ __launch_bounds__(kTile) void k_segsort(Work W, PState S) {
+  const int j = W.tile_job[blockIdx.x];
+  const JobParams& J = W.jp[j];
+  const int64_t d = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const uint32_t k = W.skey[d];
+  if (k == uint32_t(J.tbl_base + J.tbl_size - 1)) return;   // dead slots: no order needed
+  if (d > J.p0 && W.skey[d - 1] == k) return;                 // not the head of its cell
+  int64_t e = d + 1;
+  const int64_t end = J.p0 + J.cap;
+  while (e < end && W.skey[e] == k) e++;
+  // insertion sort of perm[d, e) by particle id
+  for (int64_t a = d + 1; a < e; a++) {
+    const int32_t pa = W.perm[a];
+    const int32_t ia = S.id[pa];
+    int64_t b = a - 1;
+    while (b >= d && S.id[W.perm[b]] > ia) {
+      W.perm[b + 1] = W.perm[b];
+      b--;
+    }
+    W.perm[b + 1] = pa;
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(kTile) void k_gather(Work W, PState A, PState B) {
+  const int64_t d = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const int64_t p = W.perm[d];
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    B.xh[a][d] = A.xh[a][p];
+    B.xl[a][d] = A.xl[a][p];
+    B.nrm[a][d] = A.nrm[a][p];
+  }
+  B.h[d] = A.h[p];
+  B.type[d] = A.type[p];
+  B.id[d] = A.id[p];
+  W.nu_a[d] = W.nu_b[p];
+}
+
+// ---------------------------------------------------------------------------
+// neighbour lists (Level_info::Refresh_neighbor_info with mirror ghosts)
+
+template <int D>
+__device__ __forceinline__ void image_of(const JobParams& J, uint32_t code, const double* x, double* g) {
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    const uint32_t m = (code >> (2 * a)) & 3u;
+    g[a] = (m == 1) ? 2. * J.box_l[a] - x[a] : ((m == 2) ? 2. * J.box_r[a] - x[a] : x[a]);
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
+  __shared__ int lds_i[4];
+  const int j = W.tile_job[blockIdx.x];
+  const JobParams& J = W.jp[j];
+  const int64_t slot = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const int64_t loc = slot - J.p0;
+  int cnt = 0;
+  if (loc < J.n) {
+    double xi[3] = {0., 0., 0.};
+#pragma unroll
+    for (int a = 0; a < D; a++) xi[a] = double(S.xh[a][slot]) + double(S.xl[a][slot]);
+    const double hi = S.h[slot];
+    const double ri = hi * J.cut_off + 1.e-10;
+    const int li = level_of(J, hi);
+    // mirror options per axis: none, low wall, high wall
+    int opt[3][3], nopt[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      nopt[a] = 1;
+      opt[a][0] = 0;
+      if (a < D && J.sym[a]) {
+        if (xi[a] - J.box_l[a] < J.c0) opt[a][nopt[a]++] = 1;
+        if (J.box_r[a] - xi[a] < J.c0) opt[a][nopt[a]++] = 2;
+      }
+    }
+    const int64_t cap = W.cap_total;
+    const int K = W.K;
+    for (int o0 = 0; o0 < nopt[0]; o0++)
+      for (int o1 = 0; o1 < nopt[1]; o1++)
+        for (int o2 = 0; o2 < nopt[2]; o2++) {
+          const uint32_t code = uint32_t(opt[0][o0] | (opt[1][o1] << 2) | (opt[2][o2] << 4));
+          double y[3] = {0., 0., 0.};
+          image_of<D>(J, code, xi, y);
+          for (int L = 0; L < J.nlev; L++) {
+            const int s = L < li ? L : li;
+            const int shift = D * (L - s);
+            int cy[3] = {0, 0, 0};
+            cell_at<D>(J, s, y, cy);
+            const int64_t base = J.tbl_base + J.lev_off[L];
+            for (int dz = (D > 2 ? -1 : 0); dz <= (D > 2 ? 1 : 0); dz++)
+              for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                  int c[3] = {cy[0] + dx, cy[1] + dy, cy[2] + dz};
+                  if (c[0] < 0 || c[0] >= J.lev_n[s][0] || c[1] < 0 || c[1] >= J.lev_n[s][1]) continue;
+                  if (D > 2 && (c[2] < 0 || c[2] >= J.lev_n[s][2])) continue;
+                  const uint64_t m = morton<D>(c);
+                  const int32_t lo = W.starts[base + int64_t(m << shift)];
+                  const int32_t hi2 = W.starts[base + int64_t((m + 1) << shift)];
+                  for (int32_t q = lo; q < hi2; q++) {
+                    double xq[3] = {0., 0., 0.}, g[3];
+#pragma unroll
+                    for (int a = 0; a < D; a++) xq[a] = double(S.xh[a][q]) + double(S.xl[a][q]);
+                    image_of<D>(J, code, xq, g);
+                    double dist = 0.0;
+#pragma unroll
+                    for (int a = 0; a < D; a++) {
+                      const double dr = xi[a] - g[a];
+                      dist += dr * dr;
+                    }
+                    dist = sqrt(dist);
+                    if (dist <= ri || dist <= S.h[q] * J.cut_off + 1.e-10) {
+                      if (cnt < K) W.nbl[int64_t(cnt) * cap + slot] = int32_t(uint32_t(q) | (code << kJBits));
+                      cnt++;
+                    }
+                  }
+                }
+          }
+        }
+    W.nbc[slot] = cnt < K ? cnt : K;
+  } else {
+    W.nbc[slot] = 0;
+  }
+  const int mx = block_reduce(cnt, OpMaxI(), lds_i);
+  if (threadIdx.x == 0 && mx > 0) {
+    atomicMax(&W.red[j].maxnb, mx);
+    if (mx > W.K) atomicOr(&W.red[j].overflow, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P3: kernel summation + stage-1 force + stage-1 dt
+
+template <int D>
+__global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
+  __shared__ double lds_d[4];
+  const int j = W.tile_job[blockIdx.x];
+  const JobParams& J = W.jp[j];
+  const int64_t slot = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const int64_t loc = slot - J.p0;
+  double dt = 1.0e300;
+  if (loc < J.n) {
+    const int64_t cap = W.cap_total;
+    float xhi[3] = {0.f, 0.f, 0.f}, xli[3] = {0.f, 0.f, 0.f}, ni[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      xhi[a] = S.xh[a][slot];
+      xli[a] = S.xl[a][slot];
+      ni[a] = S.nrm[a][slot];
+    }
+    const double hd = S.h[slot];
+    const float h = float(hd);
+    const int type = S.type[slot];
+    const int cnt = W.nbc[slot];
+    const bool surf = type == SMG_SURFACE;
+    // Get_kernel_summation (particle_mesh_generation.cpp:690-765)
+    float sum = 0.f;
+    int ncount = 0;
+    for (int k = 0; k < cnt; k++) {
+      const uint32_t e = uint32_t(W.nbl[int64_t(k) * cap + slot]);
+      const int64_t q = e & kJMask;
+      const uint32_t code = e >> kJBits;
+      float dr[3] = {0.f, 0.f, 0.f};
+      if (!surf) {
+        rel_pos<D>(J, S, slot, q, code, xhi, xli, dr);
+        float r2 = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; a++) r2 += dr[a] * dr[a];
+        sum += Wk<D>(sqrtf(r2), h);
+        ncount++;
+      } else if (S.type[q] == SMG_SURFACE) {
+        float ca = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; a++) ca += ni[a] * S.nrm[a][q];
+        if (fabsf(ca) > J.cos_thres) {
+          rel_pos<D>(J, S, slot, q, code, xhi, xli, dr);
+          float dn = 0.f;
+#pragma unroll
+          for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
+          float r2 = 0.f;
+#pragma unroll
+          for (int a = 0; a < D; a++) {
+            const float t = dr[a] - ni[a] * dn;
+            r2 += t * t;
+          }
+          sum += Wk<D - 1>(sqrtf(r2), h);
+          ncount++;
+        }
+      }
+    }
+    if (ncount > 1) sum = 1.f / (sum + 1.e-15f);
+    W.sum[slot] = sum;
+    // Accumulate_particle_force_real / _surf, stage 1
+    const float dxi = h * J.kdx;
+    const float vol2i = ipow<D>(dxi) * ipow<D>(dxi);
+    float F[3] = {0.f, 0.f, 0.f};
+    for (int k = 0; k < cnt; k++) {
+      const uint32_t e = uint32_t(W.nbl[int64_t(k) * cap + slot]);
+      const int64_t q = e & kJMask;
+      const uint32_t code = e >> kJBits;
+      if (q == slot && code == 0) continue;
+      const int tq = S.type[q];
+      if (surf && tq == SMG_REAL) continue;
+      float dr[3] = {0.f, 0.f, 0.f};
+      rel_pos<D>(J, S, slot, q, code, xhi, xli, dr);
+      const float hq = float(S.h[q]);
+      const float dxq = hq * J.kdx;
+      const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
+      const float hij = 0.5f * (h + hq);
+      float aw;
+      if (!surf) {
+        float r2 = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; a++) r2 += dr[a] * dr[a];
+        aw = dWk<D>(sqrtf(r2), hij);
+      } else {
+        if (tq == SMG_SURFACE) {
+          float ca = 0.f;
+#pragma unroll
+          for (int a = 0; a < D; a++) ca += ni[a] * S.nrm[a][q];
+          if (fabsf(ca) <= J.cos_thres) continue;
+        }
+        float dn = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
+        float r2 = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+          dr[a] = dr[a] - ni[a] * dn;
+          r2 += dr[a] * dr[a];
+        }
+        aw = dWk<D - 1>(sqrtf(r2), hij);
+      }
+      const float Pij = sum * (vol2i + vol2q);
+#pragma unroll
+      for (int a = 0; a < D; a++) F[a] += dr[a] * aw * -Pij;
+    }
+    if (surf) {   // Map_surface_particle_force with the raw normal
+      float nf = 0.f;
+#pragma unroll
+      for (int a = 0; a < D; a++) nf += F[a] * ni[a];
+#pragma unroll
+      for (int a = 0; a < D; a++) F[a] = F[a] - ni[a] * nf;
+    }
+    double A = 0.;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      W.a1[a][slot] = F[a];
+      A += double(F[a]) * double(F[a]);
+    }
+    A = sqrt(A);
+    // Set_timestep(1): v was zeroed by Reset_particle_force(1), so U = 0
+    const double t1 = 0.25 * sqrt(hd / (A + 1.e-15));
+    const double t2 = hd / 40. / (0. + 1.e-15);
+    dt = amin(t1, t2);
+  }
+  dt = block_reduce(dt, OpMin(), lds_d);
+  if (threadIdx.x == 0 && dt < 1.0e300) atomicMin(&W.red[j].dt1[par], dbits(dt));
+}
+
+// ---------------------------------------------------------------------------
+// P4: stage-2 (viscous) force, stage-2 dt and nu
+
+template <int D>
+__global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
+  __shared__ double lds_d[4];
+  const int j = W.tile_job[blockIdx.x];
+  const JobParams& J = W.jp[j];
+  const int64_t slot = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const int64_t loc = slot - J.p0;
+  double dt = 1.0e300, U = 0.;
+  if (loc < J.n) {
+    const int64_t cap = W.cap_total;
+    const double dt1 = bitsd(W.red[j].dt1[par]);
+    const float hdt = float(0.5 * dt1);
+    float xhi[3] = {0.f, 0.f, 0.f}, xli[3] = {0.f, 0.f, 0.f}, ni[3] = {0.f, 0.f, 0.f}, vi[3] = {0.f, 0.f, 0.f};
+    double vd[3] = {0., 0., 0.};
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      xhi[a] = S.xh[a][slot];
+      xli[a] = S.xl[a][slot];
+      ni[a] = S.nrm[a][slot];
+      const float a1 = W.a1[a][slot];
+      vi[a] = a1 * hdt;
+      vd[a] = double(a1) * (0.5 * dt1);
+    }
+    const double hd = S.h[slot];
+    const float h = float(hd);
+    const int type = S.type[slot];
+    const bool surf = type == SMG_SURFACE;
+    const int cnt = W.nbc[slot];
+    const float dxi = h * J.kdx;
+    const float voli = ipow<D>(dxi);
+    const float rhoi = 1.f / voli;
+    const float etai = W.nu_a[slot] * rhoi;
+    float F[3] = {0.f, 0.f, 0.f};
+    for (int k = 0; k < cnt; k++) {
+      const uint32_t e = uint32_t(W.nbl[int64_t(k) * cap + slot]);
+      const uint32_t code = e >> kJBits;
+      if (code != 0) continue;   // mirror ghosts carry nu = 0, so eta_ij = 0 (Q6)
+      const int64_t q = e & kJMask;
+      if (q == slot) continue;
+      const int tq = S.type[q];
+      if (surf && tq == SMG_REAL) continue;
+      float dr[3] = {0.f, 0.f, 0.f};
+      rel_pos<D>(J, S, slot, q, 0u, xhi, xli, dr);
+      const float hq = float(S.h[q]);
+      const float dxq = hq * J.kdx;
+      const float volq = ipow<D>(dxq);
+      const float rhoq = 1.f / volq;
+      const float hij = 0.5f * (h + hq);
+      float aw;
+      if (!surf) {
+        float r2 = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; a++) r2 += dr[a] * dr[a];
+        aw = dWk<D>(sqrtf(r2), hij);
+      } else {
+        if (tq == SMG_SURFACE) {
+          float ca = 0.f;
+#pragma unroll
+          for (int a = 0; a < D; a++) ca += ni[a] * S.nrm[a][q];
+          if (fabsf(ca) <= J.cos_thres) continue;
+        }
+        float dn = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
+        float r2 = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+          const float t = dr[a] - ni[a] * dn;
+          r2 += t * t;
+        }
+        aw = dWk<D - 1>(sqrtf(r2), hij);
+      }
+      float vq[3];
+      if (!surf && tq != SMG_REAL) {   // Q8: v_j := -v_i
+#pragma unroll
+        for (int a = 0; a < D; a++) vq[a] = -vi[a];
+      } else {
+#pragma unroll
+        for (int a = 0; a < D; a++) vq[a] = W.a1[a][q] * hdt;
+      }
+      const float etaq = W.nu_a[q] * rhoq;
+      const float etaij = 2.f * etaq * etai / (etai + etaq + 1.e-15f);
+      const float temp0 = aw * etaij * (voli * voli + volq * volq);
+#pragma unroll
+      for (int a = 0; a < D; a++) F[a] += (vi[a] - vq[a]) * temp0;
+    }
+    if (surf) {
+      float nf = 0.f;
+#pragma unroll
+      for (int a = 0; a < D; a++) nf += F[a] * ni[a];
+#pragma unroll
+      for (int a = 0; a < D; a++) F[a] = F[a] - ni[a] * nf;
+    }
+    double A = 0.;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      W.a2[a][slot] = F[a];
+      A += double(F[a]) * double(F[a]);
+      U += vd[a] * vd[a];
+    }
+    A = sqrt(A);
+    U = sqrt(U);
+    double t = 0.25 * sqrt(hd / (A + 1.e-15));
+    t = amin(t, hd / 40. / (U + 1.e-15));
+    const double nu = J.nu_coeff * hd * U;
+    t = amin(t, 0.125 * hd * hd / (nu + 1.e-15));
+    W.nu_b[slot] = float(nu);
+    dt = t;
+  }
+  dt = block_reduce(dt, OpMin(), lds_d);
+  U = block_reduce(U, OpMax(), lds_d);
+  if (threadIdx.x == 0 && dt < 1.0e300) {
+    atomicMin(&W.red[j].dt2[par], dbits(dt));
+    atomicMax(&W.red[j].maxv[par], dbits(U));
+  }
+}
+
+// residual of the last step without moving anything (for statistics)
+template <int D>
+__global__ __launch_bounds__(kTile) void k_residual(Work W, PState S, int par) {
+  __shared__ double lds_d[4];
+  const int j = W.tile_job[blockIdx.x];
+  const JobParams& J = W.jp[j];
+  const int64_t slot = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  double res = 0., md = 0.;
+  if (slot - J.p0 < J.n) {
+    const double dt1 = bitsd(W.red[j].dt1[par]), dt2 = bitsd(W.red[j].dt2[par]);
+    double U = 0.;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      double v = double(W.a1[a][slot]) * (0.5 * dt1);
+      v = v + double(W.a2[a][slot]) * (0.5 * dt2);
+      U += v * v;
+    }
+    U = sqrt(U);
+    md = U * dt2;
+    res = md / (S.h[slot] * J.cut_off / J.cell_ratio);
+  }
+  res = block_reduce(res, OpMax(), lds_d);
+  md = block_reduce(md, OpMax(), lds_d);
+  if (threadIdx.x == 0) {
+    atomicMax(&W.red[j].res[par], dbits(res));
+    atomicMax(&W.red[j].maxdisp[par], dbits(md));
+  }
+}
+
+__global__ void k_clear_res(JobRed* red, int njobs, int par) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < njobs) {
+    red[j].res[par] = 0ull;
+    red[j].maxdisp[par] = 0ull;
+  }
+}
+
+// FP64 conformance probe: sqrt, division, floor and tanh of given inputs
+__global__ void k_fp64_probe(int64_t n, const double* a, const double* b, double* out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[4 * i + 0] = sqrt(a[i]);
+  out[4 * i + 1] = a[i] / b[i];
+  out[4 * i + 2] = floor(a[i] / b[i]);
+  out[4 * i + 3] = tanh(a[i]);
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+
+#define SMG_DISPATCH(D, KERNEL, ...)                                      \
+  do {                                                                    \
+    if ((D) == 2) KERNEL<2><<<__VA_ARGS__>>>;                             \
+    else KERNEL<3><<<__VA_ARGS__>>>;                                      \
+  } while (0)
+
+void launch_reset(hipStream_t st, const Work& W, int njobs, int par) {
+  k_reset<<<(njobs + 255) / 256, 256, 0, st>>>(W.red, njobs, par, W.step_ctr);
+}
+void launch_seed(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S) {
+  if (D == 2) k_seed<2><<<ntiles, kTile, 0, st>>>(W, S);
+  else k_seed<3><<<ntiles, kTile, 0, st>>>(W, S);
+}
+void launch_map(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int mode, int par) {
+  if (D == 2) k_map<2><<<ntiles, kTile, 0, st>>>(W, S, mode, par);
+  else k_map<3><<<ntiles, kTile, 0, st>>>(W, S, mode, par);
+}
+void launch_scan(hipStream_t st, const Work& W) {
+  const int64_t nb = (W.n_tbl + kScanItems - 1) / kScanItems;
+  k_scan_reduce<<<nb, kTile, 0, st>>>(W.counts, W.n_tbl, W.bsum);
+  k_scan_top<<<1, 1024, 0, st>>>(W.bsum, nb);
+  k_scan_down<<<nb, kTile, 0, st>>>(W.counts, W.n_tbl, W.bsum, nb, W.starts);
+}
+void launch_scatter(hipStream_t st, int64_t ntiles, const Work& W) {
+  k_scatter<<<ntiles, kTile, 0, st>>>(W, ntiles * kTile);
+}
+void launch_segsort(hipStream_t st, int64_t ntiles, const Work& W, const PState& S) {
+  k_segsort<<<ntiles, kTile, 0, st>>>(W, S);
+}
+void launch_gather(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& A, const PState& B) {
+  if (D == 2) k_gather<2><<<ntiles, kTile, 0, st>>>(W, A, B);
+  else k_gather<3><<<ntiles, kTile, 0, st>>>(W, A, B);
+}
+void launch_nb(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S) {
+  if (D == 2) k_nb<2><<<ntiles, kTile, 0, st>>>(W, S);
+  else k_nb<3><<<ntiles, kTile, 0, st>>>(W, S);
+}
+void launch_force1(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int par) {
+  if (D == 2) k_force1<2><<<ntiles, kTile, 0, st>>>(W, S, par);
+  else k_force1<3><<<ntiles, kTile, 0, st>>>(W, S, par);
+}
+void launch_force2(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int par) {
+  if (D == 2) k_force2<2><<<ntiles, kTile, 0, st>>>(W, S, par);
+  else k_force2<3><<<ntiles, kTile, 0, st>>>(W, S, par);
+}
+void launch_residual(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int njobs, int par) {
+  k_clear_res<<<(njobs + 255) / 256, 256, 0, st>>>(W.red, njobs, par);
+  if (D == 2) k_residual<2><<<ntiles, kTile, 0, st>>>(W, S, par);
+  else k_residual<3><<<ntiles, kTile, 0, st>>>(W, S, par);
+}
+void launch_fp64_probe(hipStream_t st, int64_t n, const double* a, const double* b, double* out) {
+  k_fp64_probe<<<(n + 255) / 256, 256, 0, st>>>(n, a, b, out);
+}
+
+}  // namespace smg

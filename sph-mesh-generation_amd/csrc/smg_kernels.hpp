@@ -1,0 +1,26 @@
+// smg_kernels.hpp -- host-side launchers of the gfx950 kernels (smg_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "smg.h"
+#include "smg_device.hpp"
+
+namespace smg {
+
+enum { kModeInit = 0, kModePrerun = 1, kModeStep = 2 };
+
+void launch_reset(hipStream_t st, const Work& W, int njobs, int par);
+void launch_seed(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S);
+void launch_map(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int mode, int par);
+void launch_scan(hipStream_t st, const Work& W);
+void launch_scatter(hipStream_t st, int64_t ntiles, const Work& W);
+void launch_segsort(hipStream_t st, int64_t ntiles, const Work& W, const PState& S);
+void launch_gather(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& A, const PState& B);
+void launch_nb(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S);
+void launch_force1(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int par);
+void launch_force2(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int par);
+void launch_residual(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int njobs, int par);
+void launch_fp64_probe(hipStream_t st, int64_t n, const double* a, const double* b, double* out);
+
+}  // namespace smg

@@ -1,0 +1,302 @@
+"""Python binding of the MI355X relaxation engine (ctypes over libsmg.so).
+
+The C ABI (include/smg.h) is the product boundary; this module is a thin host
+mirror of it plus the level-set and configuration builders for the benchmark
+geometries.  It loads lib/libsmg.so from this directory and raises if that file is
+missing -- there is no CPU fallback for the relaxation step.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsmg.so")
+
+SMG_OK = 0
+SMG_REAL, SMG_SURFACE = 0, 1
+PROFILE_TANH, PROFILE_LINEAR = 0, 1
+PI_REF = 3.1415926   # glbparam.h:185
+
+
+class SmgConfig(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("sym", C.c_int32 * 3),
+        ("box_l", C.c_double * 3), ("domain", C.c_double * 3),
+        ("particle_box_l", C.c_double * 3), ("particle_box_r", C.c_double * 3),
+        ("num_pkg", C.c_int32 * 3),
+        ("cell_ratio", C.c_double), ("cut_off", C.c_double),
+        ("angle_thres", C.c_double), ("nu_coeff", C.c_double),
+        ("profile", C.c_int32),
+        ("min_dl", C.c_double), ("max_dl", C.c_double), ("max_phi", C.c_double),
+        ("lin_dx0", C.c_double), ("lin_dx1", C.c_double), ("lin_band", C.c_double),
+        ("seed", C.c_uint64), ("remap_max_attempts", C.c_int32), ("max_neighbors", C.c_int32),
+    ]
+
+
+class SmgLsetStats(C.Structure):
+    _fields_ = [
+        ("maximum_phi", C.c_double), ("total_volume", C.c_double), ("total_area", C.c_double),
+        ("total_mass", C.c_double), ("total_mass_surface", C.c_double),
+        ("n_interface_cell", C.c_int64), ("num_particle_mesh", C.c_int64),
+        ("num_particle_surface", C.c_int64), ("num_particle", C.c_int64),
+    ]
+
+
+class SmgStats(C.Structure):
+    _fields_ = [
+        ("step", C.c_int64), ("n_particle", C.c_int64), ("n_real", C.c_int64), ("n_surface", C.c_int64),
+        ("n_remapped", C.c_int64), ("n_neighbor_entries", C.c_int64),
+        ("max_neighbors_seen", C.c_int32), ("num_level", C.c_int32),
+        ("dt1", C.c_double), ("dt2", C.c_double), ("max_v", C.c_double),
+        ("residual", C.c_double), ("max_disp", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libsmg.so; raises FileNotFoundError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is missing: build it with build_lib.py")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.smg_last_error.restype = C.c_char_p
+        L.smg_default_config.argtypes = [C.c_int, P(SmgConfig)]
+        L.smg_lset_build.argtypes = [P(SmgConfig), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, P(SmgLsetStats)]
+        L.smg_create.argtypes = [P(SmgConfig), P(C.c_void_p)]
+        L.smg_destroy.argtypes = [C.c_void_p]
+        L.smg_set_levelset.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.smg_init_particles.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+        L.smg_seed_particles.argtypes = [C.c_void_p, C.c_int64]
+        L.smg_prerun.argtypes = [C.c_void_p]
+        L.smg_step.argtypes = [C.c_void_p, C.c_int64]
+        L.smg_get_timestep.argtypes = [C.c_void_p, P(C.c_double)]
+        L.smg_get_stats.argtypes = [C.c_void_p, P(SmgStats)]
+        L.smg_get_state.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+        L.smg_get_pair_terms.argtypes = [C.c_void_p] + [C.c_void_p] * 3
+        L.smg_get_neighbors.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.smg_profile_enable.argtypes = [C.c_void_p, C.c_int]
+        L.smg_profile_get.argtypes = [C.c_void_p, P(C.c_char_p), P(C.c_double), P(C.c_int64), C.c_int, P(C.c_int)]
+        L.smg_fp64_probe.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+class SmgError(RuntimeError):
+    def __init__(self, rc, where):
+        super().__init__(f"{where} failed ({rc}): {lib().smg_last_error().decode()}")
+        self.rc = rc
+
+
+def _check(rc, where):
+    if rc != SMG_OK:
+        raise SmgError(rc, where)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# configuration and level sets
+
+def default_config(dim):
+    cfg = SmgConfig()
+    _check(lib().smg_default_config(dim, C.byref(cfg)), "smg_default_config")
+    return cfg
+
+
+def config_copy(cfg):
+    c2 = SmgConfig()
+    C.memmove(C.byref(c2), C.byref(cfg), C.sizeof(SmgConfig))
+    return c2
+
+
+def cell_centres_ext(cfg):
+    """Cell centres of the grid with a full margin package (4 cells) on each side."""
+    D = cfg.dim
+    axes = []
+    for a in range(3):
+        if a < D:
+            dpkg = cfg.domain[a] / float(cfg.num_pkg[a])
+            dcell = dpkg / 4.0
+            g = np.arange(-4, 4 * cfg.num_pkg[a] + 4)
+            # pkg*dpkg + (cell+0.5)*dcell + box_l with global cell g = 4*pkg + cell
+            pkg = np.floor_divide(g, 4)
+            cell = g - 4 * pkg
+            axes.append(pkg * dpkg + (cell + 0.5) * dcell + cfg.box_l[a])
+        else:
+            axes.append(np.zeros(1))
+    Z, Y, X = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    return X, Y, Z
+
+
+def lset_build(cfg, phi_ext, zero_margin=True):
+    """Normals, cut tags and statistics of a level set given on the extended grid."""
+    D = cfg.dim
+    halo = [4 * cfg.num_pkg[a] + 2 if a < D else 1 for a in range(3)]
+    m = halo[0] * halo[1] * halo[2]
+    phi = np.zeros(m, np.float64)
+    normal = np.zeros(3 * m, np.float32)
+    tag = np.zeros(m, np.int8)
+    st = SmgLsetStats()
+    ext = np.ascontiguousarray(phi_ext, np.float64).ravel()
+    _check(lib().smg_lset_build(C.byref(cfg), _ptr(ext), int(zero_margin), _ptr(phi), _ptr(normal), _ptr(tag),
+                                C.byref(st)), "smg_lset_build")
+    return {"phi": phi, "normal": normal, "tag": tag, "stats": st}
+
+
+def sdf_circle_hole(center, radius):
+    """phi > 0 outside the circle / sphere (the fluid), SPH-mesh-generation's sign convention."""
+    c = np.asarray(center, float)
+
+    def f(X, Y, Z):
+        r2 = (X - c[0]) ** 2 + (Y - c[1]) ** 2
+        if len(c) > 2:
+            r2 = r2 + (Z - c[2]) ** 2
+        return np.sqrt(r2) - radius
+    return f
+
+
+def sdf_sphere_union(centers, radii, dim):
+    """phi > 0 inside the union of circles / spheres (the fluid is the union)."""
+    cs = np.asarray(centers, float)
+    rs = np.asarray(radii, float)
+
+    def f(X, Y, Z):
+        out = None
+        for c, r in zip(cs, rs):
+            r2 = (X - c[0]) ** 2 + (Y - c[1]) ** 2
+            if dim > 2:
+                r2 = r2 + (Z - c[2]) ** 2
+            v = r - np.sqrt(r2)
+            out = v if out is None else np.maximum(out, v)
+        return out
+    return f
+
+
+def make_levelset(cfg, sdf, zero_margin=True):
+    X, Y, Z = cell_centres_ext(cfg)
+    return lset_build(cfg, sdf(X, Y, Z), zero_margin)
+
+
+def config_c1(h_over_dx=1.0, num_pkg=32, dx=0.01, seed=20261203):
+    """2D: [0,1]^2 minus a circle r = 0.2 at the centre, uniform dx, lset 128^2 cells."""
+    cfg = default_config(2)
+    cfg.num_pkg[0] = cfg.num_pkg[1] = num_pkg
+    cfg.cell_ratio = 2.0 * h_over_dx
+    cfg.cut_off = 2.0
+    cfg.min_dl = cfg.max_dl = dx
+    cfg.seed = seed
+    sdf = sdf_circle_hole((0.5, 0.5), 0.2)
+    return cfg, sdf
+
+
+def config_c2(scale=0.564, num_pkg=64, seed=20261203):
+    """3D: unit cube minus a sphere r = 0.3 at the centre; dx linear in phi from
+    0.008*scale at the surface to 0.024*scale at phi >= 0.2; lset (4*num_pkg)^3."""
+    cfg = default_config(3)
+    for a in range(3):
+        cfg.num_pkg[a] = num_pkg
+    cfg.profile = PROFILE_LINEAR
+    cfg.lin_dx0 = 0.008 * scale
+    cfg.lin_dx1 = 0.024 * scale
+    cfg.lin_band = 0.2
+    cfg.seed = seed
+    sdf = sdf_circle_hole((0.5, 0.5, 0.5), 0.3)
+    return cfg, sdf
+
+
+# ---------------------------------------------------------------------------
+# jobs
+
+class Job:
+    """One relaxation job on the GPU (SPH_mesh_generation driven through the C ABI)."""
+
+    def __init__(self, cfg, lset, profile=False):
+        self.cfg = config_copy(cfg)
+        self.D = cfg.dim
+        h = C.c_void_p()
+        _check(lib().smg_create(C.byref(self.cfg), C.byref(h)), "smg_create")
+        self.h = h
+        self.lset = lset
+        _check(lib().smg_set_levelset(self.h, _ptr(lset["phi"]), _ptr(lset["normal"]), _ptr(lset["tag"])),
+               "smg_set_levelset")
+        if profile:
+            _check(lib().smg_profile_enable(self.h, 1), "smg_profile_enable")
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            lib().smg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def seed(self, n):
+        self.n = int(n)
+        _check(lib().smg_seed_particles(self.h, self.n), "smg_seed_particles")
+
+    def init_particles(self, x, types=None):
+        x3 = np.zeros((len(x), 3), np.float64)
+        x3[:, :x.shape[1]] = x
+        self.n = len(x)
+        tp = None if types is None else np.ascontiguousarray(types, np.int32)
+        _check(lib().smg_init_particles(self.h, self.n, _ptr(x3), None if tp is None else _ptr(tp)),
+               "smg_init_particles")
+
+    def prerun(self):
+        _check(lib().smg_prerun(self.h), "smg_prerun")
+
+    def step(self, n=1):
+        _check(lib().smg_step(self.h, int(n)), "smg_step")
+
+    def state(self):
+        n = self.n
+        x = np.zeros((n, 3)); v = np.zeros((n, 3))
+        t = np.zeros(n, np.int32); hh = np.zeros(n); nu = np.zeros(n, np.float32)
+        _check(lib().smg_get_state(self.h, _ptr(x), _ptr(v), _ptr(t), _ptr(hh), _ptr(nu)), "smg_get_state")
+        return {"x": x, "v": v, "type": t, "h": hh, "nu": nu}
+
+    def pair_terms(self):
+        n = self.n
+        s = np.zeros(n, np.float32); a1 = np.zeros((n, 3), np.float32); a2 = np.zeros((n, 3), np.float32)
+        _check(lib().smg_get_pair_terms(self.h, _ptr(s), _ptr(a1), _ptr(a2)), "smg_get_pair_terms")
+        return {"sum": s, "a1": a1, "a2": a2}
+
+    def neighbors(self):
+        offs = np.zeros(self.n + 1, np.int64)
+        _check(lib().smg_get_neighbors(self.h, _ptr(offs), None, None, 0), "smg_get_neighbors")
+        m = int(offs[-1])
+        ids = np.zeros(max(m, 1), np.int32); codes = np.zeros(max(m, 1), np.int32)
+        _check(lib().smg_get_neighbors(self.h, _ptr(offs), _ptr(ids), _ptr(codes), m), "smg_get_neighbors")
+        return offs, ids[:m], codes[:m]
+
+    def stats(self):
+        st = SmgStats()
+        _check(lib().smg_get_stats(self.h, C.byref(st)), "smg_get_stats")
+        return st.as_dict()
+
+    def profile(self):
+        names = (C.c_char_p * 64)(); ms = (C.c_double * 64)(); cnt = (C.c_int64 * 64)(); nk = C.c_int()
+        _check(lib().smg_profile_get(self.h, names, ms, cnt, 64, C.byref(nk)), "smg_profile_get")
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(nk.value)}
+
+
+def fp64_probe(a, b):
+    a = np.ascontiguousarray(a, np.float64); b = np.ascontiguousarray(b, np.float64)
+    out = np.zeros((len(a), 4))
+    _check(lib().smg_fp64_probe(len(a), _ptr(a), _ptr(b), _ptr(out)), "smg_fp64_probe")
+    return out
