@@ -61,7 +61,7 @@ typedef struct smg_config {
   double lin_band;           /* linear profile band width in phi              */
   uint64_t seed;             /* counter-based RNG key (replaces rand())       */
   int32_t remap_max_attempts;/* bound on the re-seeding rejection loop        */
-  int32_t max_neighbors;     /* per-particle list capacity, 0 = automatic     */
+  int32_t max_neighbors;     /* per-particle list capacity, 0 = 256           */
 } smg_config;
 
 /* Level-set statistics from smg_lset_build (Levelset::Calculate_total_volume_mass,
@@ -148,8 +148,9 @@ int smg_get_pair_terms(smg_job* job, float* sum, float* a1, float* a2);
  * With ids == NULL only offs is filled (to size the buffers). */
 int smg_get_neighbors(smg_job* job, int64_t* offs, int32_t* ids, int32_t* codes, int64_t capacity);
 
-/* Kernel timing: average device time in ms of each kernel over the steps taken since
- * the last reset, measured with HIP events on the job's stream. */
+/* Kernel timing: average device time in ms of each kernel over the launches since
+ * profiling was last enabled (enabling starts a new window), measured with HIP events
+ * recorded on the job's stream around each launch. */
 int smg_profile_enable(smg_job* job, int enable);
 int smg_profile_get(smg_job* job, const char** names, double* avg_ms, int64_t* counts, int max_kernels, int* n_kernels);
 

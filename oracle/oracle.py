@@ -58,6 +58,7 @@ def lib():
                                     P(SmgLsetStats)]
         L.or_num_threads.restype = C.c_int
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_position_rounding.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -86,6 +87,12 @@ def set_threads(n):
 
 def num_threads():
     return lib().or_num_threads()
+
+
+def set_position_rounding(on):
+    """Round positions to the engine's hi + lo float storage where the engine stores
+    them (after seeding and after mapping); off = plain FP64 as the reference."""
+    lib().or_set_position_rounding(int(bool(on)))
 
 
 def lset_build(cfg, phi_ext, zero_margin=True):

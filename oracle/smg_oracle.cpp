@@ -28,6 +28,18 @@ namespace {
 
 const double OR_PI = 3.1415926;   // glbparam.h:185 (quirk Q4: truncated pi)
 
+// When set, positions are rounded to the engine's storage format (an unevaluated
+// pair of floats, hi + lo) at the points where the engine stores them between
+// passes: after seeding and after the mapping pass.  Bit-exact comparisons use it so
+// that both sides see the same stored positions; the arithmetic is unchanged.
+int g_round_positions = 0;
+inline double store_round(double x) {
+  if (!g_round_positions) return x;
+  const float hi = float(x);
+  const float lo = float(x - double(hi));
+  return double(hi) + double(lo);
+}
+
 // ---------------------------------------------------------------------------
 // counter-based RNG (replaces rand(), sph_mesh_generation.cpp:1327-1342, 1909-1921)
 inline uint64_t mix64(uint64_t z) {
@@ -534,6 +546,7 @@ void do_map_all(const Ctx& c, std::vector<P>& ps, uint64_t stream, bool scale_af
     if (!map_particle(c, ps[i], stream, rm)) bad++;
     nre += rm;
     if (scale_after) set_particle_info(c, ps[i]);
+    for (int a = 0; a < c.D; a++) ps[i].x[a] = store_round(ps[i].x[a]);
   }
   o.n_remapped = nre;
   if (bad) o.ok = 0;
@@ -621,10 +634,20 @@ int or_seed(const smg_config* cfg, const double* lphi, const float* lnrm, const 
       if (p.phi > p.h) okk = true;
     }
     if (!okk) bad++;
-    for (int a = 0; a < 3; a++) { s->x[3 * i + a] = a < c.D ? p.x[a] : 0.; s->v[3 * i + a] = 0.; }
+    for (int a = 0; a < 3; a++) { s->x[3 * i + a] = a < c.D ? store_round(p.x[a]) : 0.; s->v[3 * i + a] = 0.; }
     s->h[i] = p.h; s->nu[i] = 0.; s->type[i] = SMG_REAL;
   }
   s->dt = 0.;
+  if (g_round_positions) {   // the engine re-samples h at the stored position
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < s->n; i++) {
+      P p; std::memset(&p, 0, sizeof(P));
+      for (int a = 0; a < 3; a++) p.x[a] = s->x[3 * i + a];
+      sample(c, p.x, p.phi, p.n);
+      set_particle_info(c, p);
+      s->h[i] = p.h;
+    }
+  }
   return bad ? SMG_ERR_REMAP : SMG_OK;
 }
 
@@ -749,6 +772,7 @@ int or_pair_terms(const smg_config* cfg, int64_t n, const double* x, const doubl
 }
 
 int or_num_threads(void) { return omp_get_max_threads(); }
+void or_set_position_rounding(int on) { g_round_positions = on ? 1 : 0; }
 void or_set_threads(int n) { omp_set_num_threads(n); }
 
 }  // extern "C"

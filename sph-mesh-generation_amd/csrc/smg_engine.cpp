@@ -64,7 +64,7 @@ int Engine::setup(const std::vector<JobInput>& jobs, int max_neighbors, hipStrea
   if (jobs.empty()) { set_error("setup: no jobs"); return SMG_ERR_ARG; }
   D_ = jobs[0].cfg.dim;
   if (D_ != 2 && D_ != 3) { set_error("setup: dim must be 2 or 3"); return SMG_ERR_ARG; }
-  K_ = max_neighbors > 0 ? max_neighbors : 64;
+  K_ = max_neighbors > 0 ? max_neighbors : 256;
   if (stream) st_ = stream;
   else {
     HIP_TRY(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
@@ -630,6 +630,20 @@ int Engine::profile_get(std::vector<std::string>& names, std::vector<double>& av
     counts.push_back(kv.second.second);
   }
   return SMG_OK;
+}
+
+void Engine::profile_enable(bool on) {
+  // (re-)enabling starts a new measurement window
+  if (on) {
+    (void)hipStreamSynchronize(st_);
+    for (auto& r : recs_) {
+      pool_.push_back(r.a);
+      pool_.push_back(r.b);
+    }
+    recs_.clear();
+    prof_.clear();
+  }
+  profiling_ = on;
 }
 
 double Engine::last_kernel_ms(const std::string& name) const {

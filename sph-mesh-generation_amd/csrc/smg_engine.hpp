@@ -48,7 +48,7 @@ class Engine {
   int get_stats(int job, smg_stats* st);
   int check_health();   // remap failures, neighbour overflow
 
-  void profile_enable(bool on) { profiling_ = on; }
+  void profile_enable(bool on);
   int profile_get(std::vector<std::string>& names, std::vector<double>& avg_ms, std::vector<int64_t>& counts);
 
   int dim() const { return D_; }

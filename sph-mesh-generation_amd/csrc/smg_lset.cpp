@@ -20,12 +20,13 @@
 // Cut-cell surface area: marching squares (2D) / six tetrahedra per sub-cube (3D)
 // instead of the marching-cube triangle tables; see the particle-count note in the
 // design document.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
-#include <vector>
-#include <algorithm>
 #include <numeric>
+#include <thread>
+#include <vector>
 
 #include "smg.h"
 #include "smg_internal.hpp"
@@ -71,7 +72,21 @@ double corner_value(const Grid& g, int i, int j, int k, int sx, int sy, int sz) 
   return s * 0.125;
 }
 
+// all 3^D cells around (i, j, k) strictly of one sign -> every corner average has that
+// sign too, so the cell cannot be cut
+bool same_sign_block(const Grid& g, int i, int j, int k) {
+  const int s0 = sign3(g.at(i, j, k));
+  if (s0 == 0) return false;
+  const int zr = g.D > 2 ? 1 : 0, yr = g.D > 1 ? 1 : 0;
+  for (int c = -zr; c <= zr; c++)
+    for (int b = -yr; b <= yr; b++)
+      for (int a = -1; a <= 1; a++)
+        if (sign3(g.at(i + a, j + b, k + c)) != s0) return false;
+  return true;
+}
+
 bool cut_cell(const Grid& g, int i, int j, int k) {
+  if (same_sign_block(g, i, j, k)) return false;
   // the corner order of Get_phi_at_corners: (sx, sy, sz) for u = 0..7
   static const int cs[8][3] = {{-1, -1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, -1, -1},
                                {-1, 1, -1}, {-1, 1, 1}, {1, 1, 1}, {1, 1, -1}};
@@ -291,6 +306,22 @@ void cut_cell_measure(const Grid& g, int i, int j, int k, double& vol, double& a
   vol = H * dx * dy * dz;
 }
 
+template <typename F>
+void parallel_range(int lo, int hi, F fn) {
+  const int n = hi - lo;
+  if (n <= 0) return;
+  unsigned nt = std::thread::hardware_concurrency();
+  if (nt == 0) nt = 1;
+  nt = std::min<unsigned>(nt, 32u);
+  nt = std::min<unsigned>(nt, unsigned(n));
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; t++)
+    th.emplace_back([=, &fn]() {
+      for (int q = lo + int(t); q < hi; q += int(nt)) fn(q);
+    });
+  for (auto& x : th) x.join();
+}
+
 }  // namespace
 
 int lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin, double* phi, float* normal,
@@ -311,8 +342,10 @@ int lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin, do
   const int D = g.D;
   const int klo = D > 2 ? -1 : 0, khi = D > 2 ? g.nc[2] + 1 : 1;
   const int jlo = D > 1 ? -1 : 0, jhi = D > 1 ? g.nc[1] + 1 : 1;
-  for (int k = klo; k < khi; k++)
-    for (int j = jlo; j < jhi; j++)
+  // slices are independent: each cell's outputs depend only on the input grid
+  parallel_range(D > 2 ? klo : jlo, D > 2 ? khi : jhi, [&](int outer) {
+  for (int k = (D > 2 ? outer : 0); k < (D > 2 ? outer + 1 : 1); k++)
+    for (int j = (D > 2 ? jlo : outer); j < (D > 2 ? jhi : outer + 1); j++)
       for (int i = -1; i < g.nc[0] + 1; i++) {
         const int64_t o = g.hidx(i, j, k);
         const bool in = g.inside(i, j, k);
@@ -343,6 +376,7 @@ int lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin, do
         phi[o] = blank ? 0.0 : g.at(i, j, k);
         tag[o] = (in && cut_cell(g, i, j, k)) ? 1 : 0;
       }
+  });
 
   double max_phi = 0.0;
   int64_t n_cut = 0;
@@ -358,9 +392,11 @@ int lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin, do
   double t_vol = 0.0, t_area = 0.0, t_mass = 0.0, t_msurf = 0.0;
   const int npk[3] = {cfg->num_pkg[0], D > 1 ? cfg->num_pkg[1] : 1, D > 2 ? cfg->num_pkg[2] : 1};
   // packages in the order Initial_pkg_list_and_reset_tags pushes them (k fastest),
-  // cells inside a package with k fastest, as Calculate_volume_mass loops
-  for (int pi = 0; pi < npk[0]; pi++)
-    for (int pj = 0; pj < npk[1]; pj++)
+  // cells inside a package with k fastest, as Calculate_volume_mass loops; package
+  // sums are formed in parallel and added up in that order
+  std::vector<double> psum(size_t(npk[0]) * npk[1] * npk[2] * 4, 0.0);
+  parallel_range(0, npk[0], [&](int pi) {
+  for (int pj = 0; pj < npk[1]; pj++)
       for (int pk = 0; pk < npk[2]; pk++) {
         double pv = 0.0, pa = 0.0, pm = 0.0, pms = 0.0;
         for (int ci = 0; ci < 4; ci++)
@@ -385,11 +421,19 @@ int lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin, do
               pm += vol / sd;
               pv += vol;
             }
-        t_vol += pv;
-        t_area += pa;
-        t_mass += pm;
-        t_msurf += pms;
+        double* ps = &psum[((size_t(pi) * npk[1] + pj) * npk[2] + pk) * 4];
+        ps[0] = pv;
+        ps[1] = pa;
+        ps[2] = pm;
+        ps[3] = pms;
       }
+  });
+  for (size_t q = 0; q < psum.size(); q += 4) {
+    t_vol += psum[q];
+    t_area += psum[q + 1];
+    t_mass += psum[q + 2];
+    t_msurf += psum[q + 3];
+  }
   if (st) {
     st->maximum_phi = max_phi;
     st->total_volume = t_vol;

@@ -24,7 +24,16 @@ def smg_mod():
 def orc():
     import oracle
     oracle.lib()
+    oracle.set_position_rounding(False)
     return oracle
+
+
+@pytest.fixture()
+def orc_stored(orc):
+    """The oracle seeing positions as the engine stores them (hi + lo floats)."""
+    orc.set_position_rounding(True)
+    yield orc
+    orc.set_position_rounding(False)
 
 
 @pytest.fixture(scope="session")
