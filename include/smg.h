@@ -154,6 +154,40 @@ int smg_get_neighbors(smg_job* job, int64_t* offs, int32_t* ids, int32_t* codes,
 int smg_profile_enable(smg_job* job, int enable);
 int smg_profile_get(smg_job* job, const char** names, double* avg_ms, int64_t* counts, int max_kernels, int* n_kernels);
 
+/* Asynchronous stepping for request loops (no counterpart in the reference): enqueue
+ * nsteps on the job's stream and return; smg_query sets *busy to 1 while the stream
+ * still has work; smg_sync waits.  smg_capture records steps_per_graph (even) steps as
+ * a hipGraph, after which smg_step_async replays it (nsteps must then be a multiple). */
+int smg_step_async(smg_job* job, int64_t nsteps);
+int smg_query(smg_job* job, int32_t* busy);
+int smg_sync(smg_job* job);
+int smg_capture(smg_job* job, int64_t steps_per_graph);
+/* device bytes held by the job (for admission by memory footprint) */
+int smg_footprint(smg_job* job, int64_t* bytes);
+
+/* ---- batched relaxation (many independent jobs per kernel launch) ----
+ * No counterpart in the reference, which relaxes one geometry per MPI job.  Jobs of
+ * the same dimension share one slot space: every kernel launch covers all of them,
+ * with per-job parameter and offset tables (tile -> job); the 2D and the 3D jobs run
+ * on two HIP streams and overlap on the device.  Arrays are indexed by job;
+ * phi/normal/tag are one-cell-halo level sets as for smg_set_levelset. */
+typedef struct smg_batch smg_batch;
+int smg_batch_create(int32_t njobs, const smg_config* cfgs, const int64_t* n_particles,
+                     const double* const* phi, const float* const* normal, const int8_t* const* tag,
+                     smg_batch** batch);
+int smg_batch_destroy(smg_batch* batch);
+/* seed every job (Random_particle_distribution) and Pre_run */
+int smg_batch_seed(smg_batch* batch);
+/* nsteps steps of every job, one launch per pass */
+int smg_batch_step(smg_batch* batch, int64_t nsteps);
+/* capture steps_per_graph (even) steps of every job as one hipGraph per dimension;
+ * smg_batch_run replays it `launches` times */
+int smg_batch_capture(smg_batch* batch, int64_t steps_per_graph);
+int smg_batch_run(smg_batch* batch, int64_t launches);
+int smg_batch_sync(smg_batch* batch);
+int smg_batch_get_stats(smg_batch* batch, int32_t job, smg_stats* st);
+int smg_batch_get_state(smg_batch* batch, int32_t job, double* x, double* v, int32_t* type, double* h, float* nu);
+
 /* Diagnostic with no counterpart in the reference: device FP64 sqrt(a), a/b,
  * floor(a/b) and tanh(a) for n inputs (4 outputs per input), to check that the FP64
  * operations the bit-exact paths use round as the host's do. */

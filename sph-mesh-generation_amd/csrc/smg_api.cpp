@@ -1,6 +1,7 @@
 // smg_api.cpp -- the extern "C" boundary declared in include/smg.h.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -241,6 +242,172 @@ int smg_profile_get(smg_job* job, const char** names, double* avg_ms, int64_t* c
     if (counts) counts[q] = cn[q];
   }
   return SMG_OK;
+}
+
+int smg_step_async(smg_job* job, int64_t nsteps) {
+  if (!job || !job->eng || nsteps < 0) { set_error("smg_step_async: bad argument"); return SMG_ERR_STATE; }
+  if (job->eng->has_graph()) {
+    const int64_t g = job->eng->graph_steps();
+    if (nsteps % g) { set_error("smg_step_async: step count is not a multiple of the captured graph"); return SMG_ERR_ARG; }
+    return job->eng->run_graph(nsteps / g);
+  }
+  return job->eng->step(nsteps);
+}
+
+int smg_query(smg_job* job, int32_t* busy) {
+  if (!job || !job->eng || !busy) { set_error("smg_query: bad argument"); return SMG_ERR_STATE; }
+  bool b = false;
+  const int rc = job->eng->query(&b);
+  *busy = b ? 1 : 0;
+  return rc;
+}
+
+int smg_sync(smg_job* job) {
+  if (!job || !job->eng) { set_error("smg_sync: no particles"); return SMG_ERR_STATE; }
+  const int rc = job->eng->sync();
+  if (rc != SMG_OK) return rc;
+  return job->eng->check_health();
+}
+
+int smg_capture(smg_job* job, int64_t steps_per_graph) {
+  if (!job || !job->eng) { set_error("smg_capture: no particles"); return SMG_ERR_STATE; }
+  return job->eng->capture_graph(steps_per_graph);
+}
+
+int smg_footprint(smg_job* job, int64_t* bytes) {
+  if (!job || !bytes) { set_error("smg_footprint: bad argument"); return SMG_ERR_ARG; }
+  *bytes = job->eng ? job->eng->footprint() : 0;
+  return SMG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched jobs
+
+}  // extern "C"
+
+struct smg_batch {
+  std::unique_ptr<smg::Engine> eng[2];   // [0] = 2D jobs, [1] = 3D jobs
+  std::vector<int> dim_of, local_of;
+  std::vector<std::vector<double>> phi;
+  std::vector<std::vector<float>> normal;
+  std::vector<std::vector<int8_t>> tag;
+};
+
+extern "C" {
+
+int smg_batch_create(int32_t njobs, const smg_config* cfgs, const int64_t* n_particles, const double* const* phi,
+                     const float* const* normal, const int8_t* const* tag, smg_batch** batch) {
+  if (!batch || njobs <= 0 || !cfgs || !n_particles || !phi || !normal || !tag) {
+    set_error("smg_batch_create: bad argument");
+    return SMG_ERR_ARG;
+  }
+  *batch = nullptr;
+  std::unique_ptr<smg_batch> b(new smg_batch());
+  std::vector<smg::JobInput> in[2];
+  b->dim_of.resize(njobs);
+  b->local_of.resize(njobs);
+  b->phi.resize(njobs);
+  b->normal.resize(njobs);
+  b->tag.resize(njobs);
+  for (int j = 0; j < njobs; j++) {
+    int rc = check_config(&cfgs[j]);
+    if (rc != SMG_OK) return rc;
+    if (n_particles[j] <= 0 || !phi[j] || !normal[j] || !tag[j]) {
+      set_error("smg_batch_create: job " + std::to_string(j) + " has no particles or no level set");
+      return SMG_ERR_ARG;
+    }
+    const int64_t m = halo_cells(cfgs[j]);
+    b->phi[j].assign(phi[j], phi[j] + m);
+    b->normal[j].assign(normal[j], normal[j] + 3 * m);
+    b->tag[j].assign(tag[j], tag[j] + m);
+    smg::JobInput ji;
+    ji.cfg = cfgs[j];
+    ji.n = n_particles[j];
+    double mp = 0.;
+    const smg_config& c = cfgs[j];
+    const int nx = 4 * c.num_pkg[0], ny = c.dim > 1 ? 4 * c.num_pkg[1] : 1, nz = c.dim > 2 ? 4 * c.num_pkg[2] : 1;
+    const int gx = nx + 2, gy = c.dim > 1 ? ny + 2 : 1;
+    for (int k = 0; k < nz; k++)
+      for (int jj = 0; jj < ny; jj++)
+        for (int i = 0; i < nx; i++) {
+          const int64_t o = ((int64_t(c.dim > 2 ? k + 1 : 0) * gy) + (c.dim > 1 ? jj + 1 : 0)) * gx + i + 1;
+          if (b->phi[j][o] > 0.) mp = std::max(mp, std::fabs(b->phi[j][o]));
+        }
+    ji.max_phi = mp;
+    ji.phi = b->phi[j].data();
+    ji.normal = b->normal[j].data();
+    ji.tag = b->tag[j].data();
+    const int e = cfgs[j].dim == 3 ? 1 : 0;
+    b->dim_of[j] = e;
+    b->local_of[j] = int(in[e].size());
+    in[e].push_back(ji);
+  }
+  int maxnb = 0;
+  for (int j = 0; j < njobs; j++) maxnb = std::max(maxnb, int(cfgs[j].max_neighbors));
+  for (int e = 0; e < 2; e++) {
+    if (in[e].empty()) continue;
+    b->eng[e].reset(new smg::Engine());
+    const int rc = b->eng[e]->setup(in[e], maxnb, nullptr);
+    if (rc != SMG_OK) return rc;
+  }
+  *batch = b.release();
+  return SMG_OK;
+}
+
+int smg_batch_destroy(smg_batch* batch) {
+  delete batch;
+  return SMG_OK;
+}
+
+#define SMG_FOR_ENGINES(b, call)                 \
+  for (int e_ = 0; e_ < 2; e_++) {               \
+    if (!(b)->eng[e_]) continue;                 \
+    const int rc_ = (b)->eng[e_]->call;          \
+    if (rc_ != SMG_OK) return rc_;               \
+  }
+
+int smg_batch_seed(smg_batch* batch) {
+  if (!batch) { set_error("smg_batch_seed: batch is NULL"); return SMG_ERR_ARG; }
+  SMG_FOR_ENGINES(batch, seed());
+  SMG_FOR_ENGINES(batch, prerun());
+  return SMG_OK;
+}
+
+int smg_batch_step(smg_batch* batch, int64_t nsteps) {
+  if (!batch || nsteps < 0) { set_error("smg_batch_step: bad argument"); return SMG_ERR_ARG; }
+  SMG_FOR_ENGINES(batch, step(nsteps));   // both dimensions enqueue before either waits
+  SMG_FOR_ENGINES(batch, sync());
+  SMG_FOR_ENGINES(batch, check_health());
+  return SMG_OK;
+}
+
+int smg_batch_capture(smg_batch* batch, int64_t steps_per_graph) {
+  if (!batch) { set_error("smg_batch_capture: batch is NULL"); return SMG_ERR_ARG; }
+  SMG_FOR_ENGINES(batch, capture_graph(steps_per_graph));
+  return SMG_OK;
+}
+
+int smg_batch_run(smg_batch* batch, int64_t launches) {
+  if (!batch || launches < 0) { set_error("smg_batch_run: bad argument"); return SMG_ERR_ARG; }
+  SMG_FOR_ENGINES(batch, run_graph(launches));
+  return SMG_OK;
+}
+
+int smg_batch_sync(smg_batch* batch) {
+  if (!batch) { set_error("smg_batch_sync: batch is NULL"); return SMG_ERR_ARG; }
+  SMG_FOR_ENGINES(batch, sync());
+  SMG_FOR_ENGINES(batch, check_health());
+  return SMG_OK;
+}
+
+int smg_batch_get_stats(smg_batch* batch, int32_t job, smg_stats* st) {
+  if (!batch || !st || job < 0 || job >= int32_t(batch->dim_of.size())) { set_error("smg_batch_get_stats: bad argument"); return SMG_ERR_ARG; }
+  return batch->eng[batch->dim_of[job]]->get_stats(batch->local_of[job], st);
+}
+
+int smg_batch_get_state(smg_batch* batch, int32_t job, double* x, double* v, int32_t* type, double* h, float* nu) {
+  if (!batch || job < 0 || job >= int32_t(batch->dim_of.size())) { set_error("smg_batch_get_state: bad argument"); return SMG_ERR_ARG; }
+  return batch->eng[batch->dim_of[job]]->get_state(batch->local_of[job], x, v, type, h, nu);
 }
 
 int smg_fp64_probe(int64_t n, const double* a, const double* b, double* out) {

@@ -37,6 +37,7 @@ struct JobParams {
   double c0, clen[kMaxLevels], cinv[kMaxLevels];
   float wl_h[3], wl_l[3], wh_h[3], wh_l[3];   // walls as float pairs
   float cos_thres, kdx;                       // cos(angle_thres), cut_off/cell_ratio
+  float pf_tol;                               // FP32 prefilter bound on |dr| error
   uint64_t seed;
   int64_t lset_off;                           // first cell of this job's halo grid
   int64_t tbl_base, tbl_size;                 // cell-table segment (last entry = pad cell)
@@ -78,6 +79,11 @@ struct Work {
   int32_t* counts;
   int32_t* starts;
   int32_t* bsum;
+  // neighbour-read records, one 16-byte load each in the pair passes:
+  // rec0 = (xh.x, xh.y, xh.z, h), rec1 = (xl.x, xl.y, xl.z, type bits), rec2 = (a1.xyz, nu)
+  float4* rec0;
+  float4* rec1;
+  float4* rec2;
   int32_t* nbl;   // neighbour entries, [k * cap_total + slot]
   int32_t* nbc;   // neighbour count per slot
   int64_t* step_ctr;   // index of the step in flight (re-seeding RNG stream)

@@ -45,6 +45,7 @@ T* Engine::alloc_(int64_t count) {
   if (count <= 0) count = 1;
   if (hipMalloc(&p, size_t(count) * sizeof(T)) != hipSuccess) return nullptr;
   allocs_.push_back(p);
+  bytes_ += int64_t(count) * int64_t(sizeof(T));
   return static_cast<T*>(p);
 }
 
@@ -110,6 +111,12 @@ int Engine::setup(const std::vector<JobInput>& jobs, int max_neighbors, hipStrea
     J.lin_dx0 = c.lin_dx0; J.lin_dx1 = c.lin_dx1; J.lin_band = c.lin_band;
     J.cos_thres = float(std::cos(c.angle_thres));
     J.kdx = float(c.cut_off / c.cell_ratio);
+    {
+      // bound on the FP32 error of a relative position formed from the hi parts
+      double X = 0.;
+      for (int a = 0; a < D_; a++) X = std::max(X, std::max(std::fabs(J.box_l[a]), std::fabs(J.box_r[a])));
+      J.pf_tol = float(32.0 * std::ldexp(1.0, -23) * (X + 1.0));
+    }
     J.seed = c.seed;
     if (c.profile == SMG_PROFILE_TANH && !(J.max_phi > 0.) && c.max_dl != c.min_dl) {
       set_error("setup: tanh profile needs max_phi > 0");
@@ -184,6 +191,9 @@ int Engine::setup(const std::vector<JobInput>& jobs, int max_neighbors, hipStrea
     W_.a2[a] = alloc_<float>(cap_total_);
   }
   W_.sum = alloc_<float>(cap_total_);
+  W_.rec0 = alloc_<float4>(cap_total_);
+  W_.rec1 = alloc_<float4>(cap_total_);
+  W_.rec2 = alloc_<float4>(cap_total_);
   W_.key = alloc_<uint32_t>(cap_total_);
   W_.skey = alloc_<uint32_t>(cap_total_);
   W_.perm = alloc_<int32_t>(cap_total_);
@@ -233,6 +243,7 @@ int Engine::setup(const std::vector<JobInput>& jobs, int max_neighbors, hipStrea
   HIP_TRY(hipMemsetAsync(W_.nu_a, 0, sizeof(float) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.nu_b, 0, sizeof(float) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.sum, 0, sizeof(float) * cap_total_, st_));
+  HIP_TRY(hipMemsetAsync(W_.rec2, 0, sizeof(float4) * cap_total_, st_));
   for (int a = 0; a < 3; a++) {
     HIP_TRY(hipMemsetAsync(W_.a1[a], 0, sizeof(float) * cap_total_, st_));
     HIP_TRY(hipMemsetAsync(W_.a2[a], 0, sizeof(float) * cap_total_, st_));
@@ -354,6 +365,7 @@ int Engine::prerun() {
     HIP_TRY(hipMemsetAsync(W_.a1[a], 0, sizeof(float) * cap_total_, st_));
     HIP_TRY(hipMemsetAsync(W_.a2[a], 0, sizeof(float) * cap_total_, st_));
   }
+  HIP_TRY(hipMemsetAsync(W_.rec2, 0, sizeof(float4) * cap_total_, st_));
   HIP_TRY(hipStreamSynchronize(st_));
   SMG_TRY(download_red_());
   // the dt slots stay 0 so the first step moves nothing (v = 0 after Pre_run)
@@ -426,6 +438,14 @@ int Engine::run_graph(int64_t launches) {
 int Engine::sync() {
   HIP_TRY(hipStreamSynchronize(st_));
   return SMG_OK;
+}
+
+int Engine::query(bool* busy) {
+  const hipError_t e = hipStreamQuery(st_);
+  if (e == hipSuccess) { *busy = false; return SMG_OK; }
+  if (e == hipErrorNotReady) { *busy = true; return SMG_OK; }
+  set_error(std::string("hipStreamQuery: ") + hipGetErrorString(e));
+  return SMG_ERR_HIP;
 }
 
 int Engine::download_red_() {

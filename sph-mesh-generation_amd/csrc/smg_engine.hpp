@@ -40,6 +40,9 @@ class Engine {
   int prerun();                                                 // Pre_run
   int step(int64_t nsteps);                                     // SPH_run_mesh_generation x n
   int sync();
+  int query(bool* busy);
+  int64_t footprint() const { return bytes_; }
+  int64_t graph_steps() const { return graph_steps_; }
 
   // results (host, by particle id of the job)
   int get_state(int job, double* x, double* v, int32_t* type, double* h, float* nu);
@@ -87,6 +90,7 @@ class Engine {
   PState A_{}, B_{};
   Work W_{};
   std::vector<void*> allocs_;
+  int64_t bytes_ = 0;
   std::vector<JobRed> red_h_;
 
   // profiling

@@ -262,15 +262,16 @@ __device__ __forceinline__ float dWk(float r, float h) {   // (dW/dr) / (r + 1e-
 // relative position x_i - image(x_q) in FP32 from the hi/lo pairs; the image of a
 // mirror ghost is 2w - x_q, so its difference is (x_i - w) + (x_q - w)
 template <int D>
-__device__ __forceinline__ void rel_pos(const JobParams& J, const PState& S, int64_t i, int64_t q, uint32_t code,
+__device__ __forceinline__ void rel_pos(const JobParams& J, const float4& qh, const float4& ql, uint32_t code,
                                         const float* xhi, const float* xli, float* dr) {
+  const float hq[3] = {qh.x, qh.y, qh.z};
+  const float lq[3] = {ql.x, ql.y, ql.z};
 #pragma unroll
   for (int a = 0; a < D; a++) {
     const uint32_t m = (code >> (2 * a)) & 3u;
-    const float hq = S.xh[a][q], lq = S.xl[a][q];
-    if (m == 0) dr[a] = (xhi[a] - hq) + (xli[a] - lq);
-    else if (m == 1) dr[a] = ((xhi[a] - J.wl_h[a]) + (hq - J.wl_h[a])) + ((xli[a] + lq) - 2.f * J.wl_l[a]);
-    else dr[a] = ((xhi[a] - J.wh_h[a]) + (hq - J.wh_h[a])) + ((xli[a] + lq) - 2.f * J.wh_l[a]);
+    if (m == 0) dr[a] = (xhi[a] - hq[a]) + (xli[a] - lq[a]);
+    else if (m == 1) dr[a] = ((xhi[a] - J.wl_h[a]) + (hq[a] - J.wl_h[a])) + ((xli[a] + lq[a]) - 2.f * J.wl_l[a]);
+    else dr[a] = ((xhi[a] - J.wh_h[a]) + (hq[a] - J.wh_h[a])) + ((xli[a] + lq[a]) - 2.f * J.wh_l[a]);
   }
 }
 
@@ -598,10 +599,14 @@ __global__ __launch_bounds__(kTile) void k_gather(Work W, PState A, PState B) {
     B.xl[a][d] = A.xl[a][p];
     B.nrm[a][d] = A.nrm[a][p];
   }
-  B.h[d] = A.h[p];
-  B.type[d] = A.type[p];
+  const double h = A.h[p];
+  const int32_t ty = A.type[p];
+  B.h[d] = h;
+  B.type[d] = ty;
   B.id[d] = A.id[p];
   W.nu_a[d] = W.nu_b[p];
+  W.rec0[d] = make_float4(A.xh[0][p], A.xh[1][p], D > 2 ? A.xh[2][p] : 0.f, float(h));
+  W.rec1[d] = make_float4(A.xl[0][p], A.xl[1][p], D > 2 ? A.xl[2][p] : 0.f, __int_as_float(ty));
 }
 
 // ---------------------------------------------------------------------------
@@ -625,11 +630,14 @@ __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
   const int64_t loc = slot - J.p0;
   int cnt = 0;
   if (loc < J.n) {
+    const float4 r0 = W.rec0[slot], r1 = W.rec1[slot];
+    const float xh[3] = {r0.x, r0.y, r0.z}, xl[3] = {r1.x, r1.y, r1.z};
     double xi[3] = {0., 0., 0.};
 #pragma unroll
-    for (int a = 0; a < D; a++) xi[a] = double(S.xh[a][slot]) + double(S.xl[a][slot]);
+    for (int a = 0; a < D; a++) xi[a] = double(xh[a]) + double(xl[a]);
     const double hi = S.h[slot];
     const double ri = hi * J.cut_off + 1.e-10;
+    const float rfi = float(hi * J.cut_off);
     const int li = level_of(J, hi);
     // mirror options per axis: none, low wall, high wall
     int opt[3][3], nopt[3];
@@ -644,12 +652,21 @@ __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
     }
     const int64_t cap = W.cap_total;
     const int K = W.K;
+    const float cutf = float(J.cut_off);
     for (int o0 = 0; o0 < nopt[0]; o0++)
       for (int o1 = 0; o1 < nopt[1]; o1++)
         for (int o2 = 0; o2 < nopt[2]; o2++) {
           const uint32_t code = uint32_t(opt[0][o0] | (opt[1][o1] << 2) | (opt[2][o2] << 4));
           double y[3] = {0., 0., 0.};
           image_of<D>(J, code, xi, y);
+          // FP32 relative position: dr_a = A_a + s_a * xh_q
+          float A[3] = {0.f, 0.f, 0.f}, sg[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+          for (int a = 0; a < D; a++) {
+            const uint32_t m = (code >> (2 * a)) & 3u;
+            A[a] = m == 0 ? xh[a] : (m == 1 ? xh[a] - 2.f * J.wl_h[a] : xh[a] - 2.f * J.wh_h[a]);
+            sg[a] = m == 0 ? -1.f : 1.f;
+          }
           for (int L = 0; L < J.nlev; L++) {
             const int s = L < li ? L : li;
             const int shift = D * (L - s);
@@ -666,18 +683,37 @@ __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
                   const int32_t lo = W.starts[base + int64_t(m << shift)];
                   const int32_t hi2 = W.starts[base + int64_t((m + 1) << shift)];
                   for (int32_t q = lo; q < hi2; q++) {
-                    double xq[3] = {0., 0., 0.}, g[3];
-#pragma unroll
-                    for (int a = 0; a < D; a++) xq[a] = double(S.xh[a][q]) + double(S.xl[a][q]);
-                    image_of<D>(J, code, xq, g);
-                    double dist = 0.0;
+                    const float4 qr = W.rec0[q];
+                    const float qv[3] = {qr.x, qr.y, qr.z};
+                    float d2 = 0.f;
 #pragma unroll
                     for (int a = 0; a < D; a++) {
-                      const double dr = xi[a] - g[a];
-                      dist += dr * dr;
+                      const float t = A[a] + sg[a] * qv[a];
+                      d2 += t * t;
                     }
-                    dist = sqrt(dist);
-                    if (dist <= ri || dist <= S.h[q] * J.cut_off + 1.e-10) {
+                    const float r = fmaxf(rfi, qr.w * cutf);
+                    const float band = J.pf_tol + 1.e-6f * r;
+                    const float rlo = r - band, rhi = r + band;
+                    bool in;
+                    if (d2 < rlo * rlo) in = true;
+                    else if (d2 > rhi * rhi) in = false;
+                    else {
+                      // near the cutoff: the reference's FP64 predicate on the stored positions
+                      const float4 ql = W.rec1[q];
+                      const double xq[3] = {double(qr.x) + double(ql.x), double(qr.y) + double(ql.y),
+                                            double(qr.z) + double(ql.z)};
+                      double g[3];
+                      image_of<D>(J, code, xq, g);
+                      double dist = 0.0;
+#pragma unroll
+                      for (int a = 0; a < D; a++) {
+                        const double dr = xi[a] - g[a];
+                        dist += dr * dr;
+                      }
+                      dist = sqrt(dist);
+                      in = dist <= ri || dist <= S.h[q] * J.cut_off + 1.e-10;
+                    }
+                    if (in) {
                       if (cnt < K) W.nbl[int64_t(cnt) * cap + slot] = int32_t(uint32_t(q) | (code << kJBits));
                       cnt++;
                     }
@@ -709,18 +745,18 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
   double dt = 1.0e300;
   if (loc < J.n) {
     const int64_t cap = W.cap_total;
-    float xhi[3] = {0.f, 0.f, 0.f}, xli[3] = {0.f, 0.f, 0.f}, ni[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-      xhi[a] = S.xh[a][slot];
-      xli[a] = S.xl[a][slot];
-      ni[a] = S.nrm[a][slot];
-    }
+    const float4 r0 = W.rec0[slot], r1 = W.rec1[slot];
+    const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
+    float ni[3] = {0.f, 0.f, 0.f};
     const double hd = S.h[slot];
-    const float h = float(hd);
-    const int type = S.type[slot];
+    const float h = r0.w;
+    const int type = __float_as_int(r1.w);
     const int cnt = W.nbc[slot];
     const bool surf = type == SMG_SURFACE;
+    if (surf) {
+#pragma unroll
+      for (int a = 0; a < D; a++) ni[a] = S.nrm[a][slot];
+    }
     // Get_kernel_summation (particle_mesh_generation.cpp:690-765)
     float sum = 0.f;
     int ncount = 0;
@@ -728,20 +764,21 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
       const uint32_t e = uint32_t(W.nbl[int64_t(k) * cap + slot]);
       const int64_t q = e & kJMask;
       const uint32_t code = e >> kJBits;
+      const float4 qh = W.rec0[q], ql = W.rec1[q];
       float dr[3] = {0.f, 0.f, 0.f};
       if (!surf) {
-        rel_pos<D>(J, S, slot, q, code, xhi, xli, dr);
+        rel_pos<D>(J, qh, ql, code, xhi, xli, dr);
         float r2 = 0.f;
 #pragma unroll
         for (int a = 0; a < D; a++) r2 += dr[a] * dr[a];
         sum += Wk<D>(sqrtf(r2), h);
         ncount++;
-      } else if (S.type[q] == SMG_SURFACE) {
+      } else if (__float_as_int(ql.w) == SMG_SURFACE) {
         float ca = 0.f;
 #pragma unroll
         for (int a = 0; a < D; a++) ca += ni[a] * S.nrm[a][q];
         if (fabsf(ca) > J.cos_thres) {
-          rel_pos<D>(J, S, slot, q, code, xhi, xli, dr);
+          rel_pos<D>(J, qh, ql, code, xhi, xli, dr);
           float dn = 0.f;
 #pragma unroll
           for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
@@ -767,11 +804,12 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
       const int64_t q = e & kJMask;
       const uint32_t code = e >> kJBits;
       if (q == slot && code == 0) continue;
-      const int tq = S.type[q];
+      const float4 qh = W.rec0[q], ql = W.rec1[q];
+      const int tq = __float_as_int(ql.w);
       if (surf && tq == SMG_REAL) continue;
       float dr[3] = {0.f, 0.f, 0.f};
-      rel_pos<D>(J, S, slot, q, code, xhi, xli, dr);
-      const float hq = float(S.h[q]);
+      rel_pos<D>(J, qh, ql, code, xhi, xli, dr);
+      const float hq = qh.w;
       const float dxq = hq * J.kdx;
       const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
       const float hij = 0.5f * (h + hq);
@@ -816,6 +854,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
       W.a1[a][slot] = F[a];
       A += double(F[a]) * double(F[a]);
     }
+    W.rec2[slot] = make_float4(F[0], F[1], D > 2 ? F[2] : 0.f, W.nu_a[slot]);
     A = sqrt(A);
     // Set_timestep(1): v was zeroed by Reset_particle_force(1), so U = 0
     const double t1 = 0.25 * sqrt(hd / (A + 1.e-15));
@@ -841,79 +880,88 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
     const int64_t cap = W.cap_total;
     const double dt1 = bitsd(W.red[j].dt1[par]);
     const float hdt = float(0.5 * dt1);
-    float xhi[3] = {0.f, 0.f, 0.f}, xli[3] = {0.f, 0.f, 0.f}, ni[3] = {0.f, 0.f, 0.f}, vi[3] = {0.f, 0.f, 0.f};
+    const float4 r0 = W.rec0[slot], r1 = W.rec1[slot], r2 = W.rec2[slot];
+    const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
+    const float a1i[3] = {r2.x, r2.y, r2.z};
+    float ni[3] = {0.f, 0.f, 0.f}, vi[3] = {0.f, 0.f, 0.f};
     double vd[3] = {0., 0., 0.};
 #pragma unroll
     for (int a = 0; a < D; a++) {
-      xhi[a] = S.xh[a][slot];
-      xli[a] = S.xl[a][slot];
-      ni[a] = S.nrm[a][slot];
-      const float a1 = W.a1[a][slot];
-      vi[a] = a1 * hdt;
-      vd[a] = double(a1) * (0.5 * dt1);
+      vi[a] = a1i[a] * hdt;
+      vd[a] = double(a1i[a]) * (0.5 * dt1);
     }
     const double hd = S.h[slot];
-    const float h = float(hd);
-    const int type = S.type[slot];
+    const float h = r0.w;
+    const int type = __float_as_int(r1.w);
     const bool surf = type == SMG_SURFACE;
+    if (surf) {
+#pragma unroll
+      for (int a = 0; a < D; a++) ni[a] = S.nrm[a][slot];
+    }
     const int cnt = W.nbc[slot];
     const float dxi = h * J.kdx;
     const float voli = ipow<D>(dxi);
     const float rhoi = 1.f / voli;
-    const float etai = W.nu_a[slot] * rhoi;
+    const float etai = r2.w * rhoi;
     float F[3] = {0.f, 0.f, 0.f};
-    for (int k = 0; k < cnt; k++) {
-      const uint32_t e = uint32_t(W.nbl[int64_t(k) * cap + slot]);
-      const uint32_t code = e >> kJBits;
-      if (code != 0) continue;   // mirror ghosts carry nu = 0, so eta_ij = 0 (Q6)
-      const int64_t q = e & kJMask;
-      if (q == slot) continue;
-      const int tq = S.type[q];
-      if (surf && tq == SMG_REAL) continue;
-      float dr[3] = {0.f, 0.f, 0.f};
-      rel_pos<D>(J, S, slot, q, 0u, xhi, xli, dr);
-      const float hq = float(S.h[q]);
-      const float dxq = hq * J.kdx;
-      const float volq = ipow<D>(dxq);
-      const float rhoq = 1.f / volq;
-      const float hij = 0.5f * (h + hq);
-      float aw;
-      if (!surf) {
-        float r2 = 0.f;
+    if (etai != 0.f) {   // eta_ij = 0 for every pair when this particle's nu is 0
+      for (int k = 0; k < cnt; k++) {
+        const uint32_t e = uint32_t(W.nbl[int64_t(k) * cap + slot]);
+        const uint32_t code = e >> kJBits;
+        if (code != 0) continue;   // mirror ghosts carry nu = 0, so eta_ij = 0 (Q6)
+        const int64_t q = e & kJMask;
+        if (q == slot) continue;
+        const float4 qh = W.rec0[q], ql = W.rec1[q];
+        const int tq = __float_as_int(ql.w);
+        if (surf && tq == SMG_REAL) continue;
+        const float4 qv = W.rec2[q];
+        const float hq = qh.w;
+        const float dxq = hq * J.kdx;
+        const float volq = ipow<D>(dxq);
+        const float rhoq = 1.f / volq;
+        const float etaq = qv.w * rhoq;
+        if (etaq == 0.f) continue;   // eta_ij = 0
+        float dr[3] = {0.f, 0.f, 0.f};
+        rel_pos<D>(J, qh, ql, 0u, xhi, xli, dr);
+        const float hij = 0.5f * (h + hq);
+        float aw;
+        if (!surf) {
+          float rr = 0.f;
 #pragma unroll
-        for (int a = 0; a < D; a++) r2 += dr[a] * dr[a];
-        aw = dWk<D>(sqrtf(r2), hij);
-      } else {
-        if (tq == SMG_SURFACE) {
-          float ca = 0.f;
+          for (int a = 0; a < D; a++) rr += dr[a] * dr[a];
+          aw = dWk<D>(sqrtf(rr), hij);
+        } else {
+          if (tq == SMG_SURFACE) {
+            float ca = 0.f;
 #pragma unroll
-          for (int a = 0; a < D; a++) ca += ni[a] * S.nrm[a][q];
-          if (fabsf(ca) <= J.cos_thres) continue;
+            for (int a = 0; a < D; a++) ca += ni[a] * S.nrm[a][q];
+            if (fabsf(ca) <= J.cos_thres) continue;
+          }
+          float dn = 0.f;
+#pragma unroll
+          for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
+          float rr = 0.f;
+#pragma unroll
+          for (int a = 0; a < D; a++) {
+            const float t = dr[a] - ni[a] * dn;
+            rr += t * t;
+          }
+          aw = dWk<D - 1>(sqrtf(rr), hij);
         }
-        float dn = 0.f;
+        const float qa[3] = {qv.x, qv.y, qv.z};
+        float vq[3];
+        if (!surf && tq != SMG_REAL) {   // Q8: v_j := -v_i
 #pragma unroll
-        for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
-        float r2 = 0.f;
+          for (int a = 0; a < D; a++) vq[a] = -vi[a];
+        } else {
 #pragma unroll
-        for (int a = 0; a < D; a++) {
-          const float t = dr[a] - ni[a] * dn;
-          r2 += t * t;
+          for (int a = 0; a < D; a++) vq[a] = qa[a] * hdt;
         }
-        aw = dWk<D - 1>(sqrtf(r2), hij);
+        const float etaij = 2.f * etaq * etai / (etai + etaq + 1.e-15f);
+        const float temp0 = aw * etaij * (voli * voli + volq * volq);
+#pragma unroll
+        for (int a = 0; a < D; a++) F[a] += (vi[a] - vq[a]) * temp0;
       }
-      float vq[3];
-      if (!surf && tq != SMG_REAL) {   // Q8: v_j := -v_i
-#pragma unroll
-        for (int a = 0; a < D; a++) vq[a] = -vi[a];
-      } else {
-#pragma unroll
-        for (int a = 0; a < D; a++) vq[a] = W.a1[a][q] * hdt;
-      }
-      const float etaq = W.nu_a[q] * rhoq;
-      const float etaij = 2.f * etaq * etai / (etai + etaq + 1.e-15f);
-      const float temp0 = aw * etaij * (voli * voli + volq * volq);
-#pragma unroll
-      for (int a = 0; a < D; a++) F[a] += (vi[a] - vq[a]) * temp0;
     }
     if (surf) {
       float nf = 0.f;

@@ -86,6 +86,11 @@ def lib():
         L.smg_profile_enable.argtypes = [C.c_void_p, C.c_int]
         L.smg_profile_get.argtypes = [C.c_void_p, P(C.c_char_p), P(C.c_double), P(C.c_int64), C.c_int, P(C.c_int)]
         L.smg_fp64_probe.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.smg_step_async.argtypes = [C.c_void_p, C.c_int64]
+        L.smg_query.argtypes = [C.c_void_p, P(C.c_int32)]
+        L.smg_sync.argtypes = [C.c_void_p]
+        L.smg_capture.argtypes = [C.c_void_p, C.c_int64]
+        L.smg_footprint.argtypes = [C.c_void_p, P(C.c_int64)]
         _lib = L
     return _lib
 
@@ -263,6 +268,25 @@ class Job:
     def step(self, n=1):
         _check(lib().smg_step(self.h, int(n)), "smg_step")
 
+    def step_async(self, n):
+        _check(lib().smg_step_async(self.h, int(n)), "smg_step_async")
+
+    def busy(self):
+        b = C.c_int32()
+        _check(lib().smg_query(self.h, C.byref(b)), "smg_query")
+        return bool(b.value)
+
+    def sync(self):
+        _check(lib().smg_sync(self.h), "smg_sync")
+
+    def capture(self, steps_per_graph):
+        _check(lib().smg_capture(self.h, int(steps_per_graph)), "smg_capture")
+
+    def footprint(self):
+        b = C.c_int64()
+        _check(lib().smg_footprint(self.h, C.byref(b)), "smg_footprint")
+        return int(b.value)
+
     def state(self):
         n = self.n
         x = np.zeros((n, 3)); v = np.zeros((n, 3))
@@ -300,3 +324,126 @@ def fp64_probe(a, b):
     out = np.zeros((len(a), 4))
     _check(lib().smg_fp64_probe(len(a), _ptr(a), _ptr(b), _ptr(out)), "smg_fp64_probe")
     return out
+
+
+# ---------------------------------------------------------------------------
+# batched jobs
+
+def _batch_sigs(L):
+    P = C.POINTER
+    L.smg_batch_create.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   P(C.c_void_p)]
+    L.smg_batch_destroy.argtypes = [C.c_void_p]
+    for f in ("smg_batch_seed", "smg_batch_sync"):
+        getattr(L, f).argtypes = [C.c_void_p]
+    for f in ("smg_batch_step", "smg_batch_capture", "smg_batch_run"):
+        getattr(L, f).argtypes = [C.c_void_p, C.c_int64]
+    L.smg_batch_get_stats.argtypes = [C.c_void_p, C.c_int32, P(SmgStats)]
+    L.smg_batch_get_state.argtypes = [C.c_void_p, C.c_int32] + [C.c_void_p] * 5
+
+
+class Batch:
+    """Many independent jobs relaxed together (smg_batch_* of include/smg.h)."""
+
+    def __init__(self, jobs):
+        """jobs: list of (cfg, lset, n)."""
+        L = lib()
+        _batch_sigs(L)
+        nj = len(jobs)
+        cfgs = (SmgConfig * nj)()
+        for i, (c, _, _) in enumerate(jobs):
+            C.memmove(C.byref(cfgs[i]), C.byref(c), C.sizeof(SmgConfig))
+        ns = np.array([int(n) for _, _, n in jobs], np.int64)
+        self._keep = [ls for _, ls, _ in jobs]
+        phis = (C.c_void_p * nj)(*[_ptr(ls["phi"]).value for ls in self._keep])
+        nrms = (C.c_void_p * nj)(*[_ptr(ls["normal"]).value for ls in self._keep])
+        tags = (C.c_void_p * nj)(*[_ptr(ls["tag"]).value for ls in self._keep])
+        h = C.c_void_p()
+        _check(L.smg_batch_create(nj, C.cast(cfgs, C.c_void_p), _ptr(ns), C.cast(phis, C.c_void_p),
+                                  C.cast(nrms, C.c_void_p), C.cast(tags, C.c_void_p), C.byref(h)), "smg_batch_create")
+        self.h = h
+        self.ns = ns
+        self.dims = [c.dim for c, _, _ in jobs]
+
+    def close(self):
+        if self.h:
+            lib().smg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def seed(self):
+        _check(lib().smg_batch_seed(self.h), "smg_batch_seed")
+
+    def step(self, n):
+        _check(lib().smg_batch_step(self.h, int(n)), "smg_batch_step")
+
+    def capture(self, steps_per_graph):
+        _check(lib().smg_batch_capture(self.h, int(steps_per_graph)), "smg_batch_capture")
+
+    def run(self, launches):
+        _check(lib().smg_batch_run(self.h, int(launches)), "smg_batch_run")
+
+    def sync(self):
+        _check(lib().smg_batch_sync(self.h), "smg_batch_sync")
+
+    def stats(self, j):
+        st = SmgStats()
+        _check(lib().smg_batch_get_stats(self.h, int(j), C.byref(st)), "smg_batch_get_stats")
+        return st.as_dict()
+
+    def state(self, j):
+        n = int(self.ns[j])
+        x = np.zeros((n, 3)); v = np.zeros((n, 3))
+        t = np.zeros(n, np.int32); hh = np.zeros(n); nu = np.zeros(n, np.float32)
+        _check(lib().smg_batch_get_state(self.h, int(j), _ptr(x), _ptr(v), _ptr(t), _ptr(hh), _ptr(nu)),
+               "smg_batch_get_state")
+        return {"x": x, "v": v, "type": t, "h": hh, "nu": nu}
+
+
+def union_job(dim, n_target, rng, seed, max_pkg=None):
+    """One batched-config job: a union of 1-5 circles / spheres (radii U[0.05, 0.25],
+    centres U[r, 1-r]^dim) in the unit box, the fluid inside the union, uniform dx
+    chosen so that the particle count (Get_num_particles) is about n_target."""
+    k = int(rng.integers(1, 6))
+    radii = rng.uniform(0.05, 0.25, size=k)
+    centres = np.stack([rng.uniform(r, 1 - r, size=dim) for r in radii])
+    sdf = sdf_sphere_union(centres, radii, dim)
+    cfg = default_config(dim)
+    cfg.seed = seed
+    # volume and boundary measure of the union by sampling, then N(dx) = V/dx^D + A/(2 dx^(D-1))
+    pts = rng.uniform(0, 1, size=(200000, 3))
+    phi = sdf(pts[:, 0], pts[:, 1], pts[:, 2])
+    vol = max(float(np.mean(phi > 0)), 1e-4)
+    eps = 0.01
+    area = float(np.mean(np.abs(phi) < eps)) / (2 * eps)
+    lo, hi = 1e-5, 1.0
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        if vol / mid ** dim + 0.5 * area / mid ** (dim - 1) > n_target:
+            lo = mid
+        else:
+            hi = mid
+    dx = 0.5 * (lo + hi)
+    if max_pkg is None:
+        max_pkg = 64 if dim == 2 else 16
+    npk = int(np.clip(np.ceil(1.0 / (6.0 * dx)), 6, max_pkg))
+    for a in range(dim):
+        cfg.num_pkg[a] = npk
+    cfg.min_dl = cfg.max_dl = dx
+    ls = make_levelset(cfg, sdf)
+    return cfg, ls, int(ls["stats"].num_particle)
+
+
+def batched_jobs(njobs, n_min, n_max, seed=20261203, dims=(2, 3), max_pkg=None):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(njobs):
+        dim = dims[j % len(dims)]
+        n_t = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        jobs.append(union_job(dim, n_t, rng, seed + 7919 * (j + 1), max_pkg=max_pkg))
+    return jobs
