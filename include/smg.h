@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define SMG_ABI_VERSION 1
+#define SMG_ABI_VERSION 2
 
 enum {
   SMG_OK = 0,
@@ -61,7 +61,9 @@ typedef struct smg_config {
   double lin_band;           /* linear profile band width in phi              */
   uint64_t seed;             /* counter-based RNG key (replaces rand())       */
   int32_t remap_max_attempts;/* bound on the re-seeding rejection loop        */
-  int32_t max_neighbors;     /* per-particle list capacity, 0 = 256           */
+  int32_t max_neighbors;     /* inline list entries per particle, 0 = 64;     
+                                longer lists spill to a shared pool          */
+  int32_t spill_per_slot;    /* spill pool entries per particle slot, 0 = 32  */
 } smg_config;
 
 /* Level-set statistics from smg_lset_build (Levelset::Calculate_total_volume_mass,
@@ -89,8 +91,10 @@ typedef struct smg_stats {
   int32_t num_level;     /* cell-list levels in use                             */
   double dt1, dt2;       /* glbl_timestep after Set_timestep(1) / (2)           */
   double max_v;          /* glbl_max_v after Set_timestep(2)                    */
-  double residual;       /* max_i |v_i| dt2 / dx_i of the last step             */
+  double residual;       /* mean_i |v_i| dt2 / dx_i of the last step (the      */
+                         /* convergence measure; summed in 2^-40 fixed point) */
   double max_disp;       /* max_i |v_i| dt2 of the last step                    */
+  double res_max;        /* max_i |v_i| dt2 / dx_i of the last step             */
 } smg_stats;
 
 typedef struct smg_job smg_job;
@@ -187,6 +191,65 @@ int smg_batch_run(smg_batch* batch, int64_t launches);
 int smg_batch_sync(smg_batch* batch);
 int smg_batch_get_stats(smg_batch* batch, int32_t job, smg_stats* st);
 int smg_batch_get_state(smg_batch* batch, int32_t job, double* x, double* v, int32_t* type, double* h, float* nu);
+
+/* ---- serving: continuous batching over a device-resident job pool ----
+ * No counterpart in the reference.  A server owns one pool of particle slots, cell
+ * tables and level-set cells on the device (its HBM footprint, fixed at creation).
+ * Jobs are admitted into free ranges of the pool between chunks of steps; every step
+ * launch covers all admitted jobs; after each chunk the jobs whose mean residual over
+ * the chunk fell below tol (or that reached max_steps) are retired, their results
+ * copied to the host and their ranges freed for the next admission.  Admission and
+ * retirement happen only at chunk boundaries, so a job's step count at every check is
+ * a multiple of chunk_steps whatever else is in the pool, and its result is the same
+ * bits alone or in any mix.  All calls on one server come from one host thread. */
+typedef struct smg_server_config {
+  int32_t dim;
+  int32_t max_jobs;          /* jobs in the pool at once                              */
+  int64_t max_slots;         /* particle slots (a job takes n rounded up to 256)      */
+  int64_t max_table;         /* cell-table entries (a job takes its cell-list tables) */
+  int64_t max_lset_cells;    /* level-set cells (a job takes (4 num_pkg + 2)^dim)     */
+  int32_t max_neighbors;     /* inline list entries per slot, 0 = 64                  */
+  int32_t spill_per_slot;    /* spill pool entries per slot, 0 = 32                   */
+  int32_t chunk_steps;       /* steps between admissions / convergence checks (> 0)  */
+  int32_t min_steps;         /* no retirement by convergence before this many steps  */
+  int32_t max_steps;         /* retire unconverged at this many steps (0 = no limit) */
+  int32_t reserved;
+  double tol;                /* converged: mean over the chunk of smg_stats.residual < tol */
+} smg_server_config;
+
+/* finished-job status */
+enum { SMG_JOB_CONVERGED = 0, SMG_JOB_MAX_STEPS = 1 };   /* < 0: an SMG_ERR_* code */
+
+typedef struct smg_server_job {
+  int32_t ticket;            /* id returned by smg_server_admit                       */
+  int32_t status;            /* SMG_JOB_* or a negative SMG_ERR_* code                */
+  int64_t steps;             /* steps taken                                           */
+  int64_t n_particle;
+  double window_residual;    /* mean residual over the last chunk                      */
+} smg_server_job;
+
+typedef struct smg_server smg_server;
+/* device bytes a server of this configuration allocates */
+int smg_server_footprint_of(const smg_server_config* cfg, int64_t* bytes);
+/* pool ranges one job takes: slots, table entries, level-set cells */
+int smg_server_job_size(const smg_config* cfg, int64_t n_particles, int64_t* slots, int64_t* table,
+                        int64_t* lset_cells);
+int smg_server_create(const smg_server_config* cfg, smg_server** server);
+int smg_server_destroy(smg_server* server);
+/* Seeds the job (Random_particle_distribution + Pre_run) into free ranges of the pool.
+ * Returns SMG_ERR_CAPACITY, with the pool unchanged, when the job does not fit now. */
+int smg_server_admit(smg_server* server, const smg_config* cfg, const double* phi, const float* normal,
+                     const int8_t* tag, int64_t n_particles, int32_t* ticket);
+/* One chunk of steps for every job in the pool, then convergence checks and
+ * retirement.  n_active: jobs still in the pool afterwards (may be NULL). */
+int smg_server_advance(smg_server* server, int32_t* n_active);
+/* Jobs finished since the last call (at most cap entries). */
+int smg_server_finished(smg_server* server, smg_server_job* out, int32_t cap, int32_t* n);
+/* Final state of a finished job by particle id (any pointer may be NULL), then
+ * releases it.  x: n*3 doubles. */
+int smg_server_result(smg_server* server, int32_t ticket, double* x, int32_t* type, double* h, smg_stats* st);
+/* pool occupancy */
+int smg_server_usage(smg_server* server, int64_t* used_slots, int64_t* total_slots, int32_t* n_active);
 
 /* Diagnostic with no counterpart in the reference: device FP64 sqrt(a), a/b,
  * floor(a/b) and tanh(a) for n inputs (4 outputs per input), to check that the FP64

@@ -165,7 +165,7 @@ def step(cfg, lset, s, step_index, want_terms=False, want_nb=False):
     sm = np.zeros(n) if want_terms else None
     a1 = np.zeros((n, 3)) if want_terms else None
     a2 = np.zeros((n, 3)) if want_terms else None
-    diag = np.zeros(5)
+    diag = np.zeros(6)
     nrm = np.zeros(1, np.int64)
     offs = np.zeros(n + 1, np.int64) if want_nb else None
     cap = 0
@@ -184,7 +184,7 @@ def step(cfg, lset, s, step_index, want_terms=False, want_nb=False):
     s._back()
     if rc != 0:
         raise RuntimeError(f"or_step failed {rc}")
-    out.update(dt1=diag[0], dt2=diag[1], max_v=diag[2], residual=diag[3], max_disp=diag[4], n_remapped=int(nrm[0]))
+    out.update(dt1=diag[0], dt2=diag[1], max_v=diag[2], residual=diag[3], max_disp=diag[4], res_max=diag[5], n_remapped=int(nrm[0]))
     if want_terms:
         out.update(sum=sm, a1=a1, a2=a2)
     if want_nb:

@@ -669,7 +669,7 @@ int or_prerun(const smg_config* cfg, const double* lphi, const float* lnrm, cons
 
 // One SPH_run_mesh_generation (sph_mesh_generation.cpp:1371-1463).  step_index is the
 // re-seeding RNG stream (1 for the first step after Pre_run).  Optional outputs
-// (may be NULL): sum[n], a1[n*3], a2[n*3], diag[5] = {dt1, dt2, max_v, residual, max_disp},
+// (may be NULL): sum[n], a1[n*3], a2[n*3], diag[6] = {dt1, dt2, max_v, residual, max_disp, res_max},
 // nb_offs[n+1] + nb_ids/nb_codes (capacity nb_cap) for the neighbour sets.
 int or_step(const smg_config* cfg, const double* lphi, const float* lnrm, const int8_t* ltag,
             or_state* s, int64_t step_index, double* sum_out, double* a1_out, double* a2_out,
@@ -701,13 +701,20 @@ int or_step(const smg_config* cfg, const double* lphi, const float* lnrm, const 
   if (a2_out) for (int64_t i = 0; i < s->n; i++) for (int a = 0; a < 3; a++) a2_out[3 * i + a] = ps[i].a[a];
   for (P& p : ps) for (int a = 0; a < D; a++) p.v[a] = p.v[a] + p.a[a] * (0.5 * dt2);
   s->dt = dt2;
-  double res = 0., md = 0.;
+  // convergence residual: mean over particles of |v| dt2 / dx_i (the reference has
+  // none; its closest quantity is the global max |v| of Set_timestep,
+  // sph_mesh_generation.cpp:983-998); rmax = the max of the same ratio
+  double rsum = 0., rmax = 0., md = 0.;
   for (P& p : ps) {
     double U = 0.; for (int a = 0; a < D; a++) U += p.v[a] * p.v[a]; U = std::sqrt(U);
     double dx = p.h * cfg->cut_off / cfg->cell_ratio;
-    res = std::max(res, U * dt2 / dx); md = std::max(md, U * dt2);
+    const double r = U * dt2 / dx;
+    rsum += r; rmax = std::max(rmax, r); md = std::max(md, U * dt2);
   }
-  if (diag) { diag[0] = dt1; diag[1] = dt2; diag[2] = mv2; diag[3] = res; diag[4] = md; }
+  if (diag) {
+    diag[0] = dt1; diag[1] = dt2; diag[2] = mv2; diag[3] = ps.empty() ? 0. : rsum / double(ps.size());
+    diag[4] = md; diag[5] = rmax;
+  }
   if (n_remapped) *n_remapped = o.n_remapped;
   if (nb_offs) {
     int64_t k = 0; nb_offs[0] = 0;

@@ -71,7 +71,7 @@ int build_engine(smg_job* job, int64_t n) {
   in.tag = job->tag.data();
   job->eng.reset(new smg::Engine());
   job->eng->profile_enable(job->profile);
-  const int rc = job->eng->setup({in}, job->cfg.max_neighbors, nullptr);
+  const int rc = job->eng->setup({in}, job->cfg.max_neighbors, job->cfg.spill_per_slot, nullptr);
   if (rc != SMG_OK) job->eng.reset();
   return rc;
 }
@@ -111,6 +111,7 @@ int smg_default_config(int dim, smg_config* c) {
   c->seed = 20261203ull;
   c->remap_max_attempts = 100000;
   c->max_neighbors = 0;
+  c->spill_per_slot = 0;
   return SMG_OK;
 }
 
@@ -342,12 +343,15 @@ int smg_batch_create(int32_t njobs, const smg_config* cfgs, const int64_t* n_par
     b->local_of[j] = int(in[e].size());
     in[e].push_back(ji);
   }
-  int maxnb = 0;
-  for (int j = 0; j < njobs; j++) maxnb = std::max(maxnb, int(cfgs[j].max_neighbors));
+  int maxnb = 0, sps = 0;
+  for (int j = 0; j < njobs; j++) {
+    maxnb = std::max(maxnb, int(cfgs[j].max_neighbors));
+    sps = std::max(sps, int(cfgs[j].spill_per_slot));
+  }
   for (int e = 0; e < 2; e++) {
     if (in[e].empty()) continue;
     b->eng[e].reset(new smg::Engine());
-    const int rc = b->eng[e]->setup(in[e], maxnb, nullptr);
+    const int rc = b->eng[e]->setup(in[e], maxnb, sps, nullptr);
     if (rc != SMG_OK) return rc;
   }
   *batch = b.release();

@@ -45,9 +45,20 @@ struct JobParams {
   int64_t p0, n, cap;                         // slot range
 };
 
-// per-job reductions; doubles stored as their bit patterns (all values >= 0)
+// fixed-point scale of the residual sum: each particle's |v| dt2 / dx is rounded to a
+// multiple of 2^-40 before it is added, so the integer sum does not depend on the
+// order of the additions (same value alone, in a batch, or after a re-sort)
+constexpr double kResScale = 1099511627776.0;   // 2^40
+
+// per-job reductions, double buffered by step parity; doubles stored as their bit
+// patterns (all values >= 0) so that atomicMin / atomicMax on the integer order work
 struct JobRed {
-  unsigned long long dt1[2], dt2[2], maxv[2], res[2], maxdisp[2];
+  unsigned long long dt1[2], dt2[2], maxv[2];
+  unsigned long long res[2];       // max_i |v_i| dt2 / dx_i
+  unsigned long long maxdisp[2];   // max_i |v_i| dt2
+  unsigned long long rsum[2];      // sum_i |v_i| dt2 / dx_i in units of 2^-40
+  unsigned long long rcum;         // rsum summed over all steps so far (windowed means)
+  int64_t step;                    // steps taken by this job (re-seeding RNG stream)
   int32_t nremap[2];
   int32_t fail, overflow, maxnb, nsurf;
 };
@@ -84,12 +95,20 @@ struct Work {
   float4* rec0;
   float4* rec1;
   float4* rec2;
-  int32_t* nbl;   // neighbour entries, [k * cap_total + slot]
-  int32_t* nbc;   // neighbour count per slot
-  int64_t* step_ctr;   // index of the step in flight (re-seeding RNG stream)
+  // neighbour lists: the first K entries of a slot inline at [k * cap_total + slot];
+  // entries K.. of a slot with more neighbours in the spill pool at spill[nbo[slot] + k - K]
+  int32_t* nbl;
+  int32_t* nbc;     // neighbour count per slot (all entries, inline + spilled)
+  int32_t* nbo;     // spill offset per slot (valid when nbc > K)
+  int32_t* spill;   // spill pool, spill_cap entries
+  int32_t* ovq;     // slots whose lists spill, appended by the list pass
+  unsigned long long* ctr;   // [0] = ovq length, [1] = spill entries handed out
+  int64_t spill_cap;
+  int64_t tile0;   // first tile of a launch (launches over one job's tiles start there)
   int64_t cap_total;
   int64_t n_tbl;
   int32_t K;
 };
+
 
 }  // namespace smg

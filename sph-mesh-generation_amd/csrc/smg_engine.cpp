@@ -34,7 +34,7 @@ Engine::~Engine() {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
-  for (auto e : pool_) (void)hipEventDestroy(e);
+  for (auto e : ev_pool_) (void)hipEventDestroy(e);
   for (void* p : allocs_) (void)hipFree(p);
   if (own_stream_ && st_) (void)hipStreamDestroy(st_);
 }
@@ -43,7 +43,10 @@ template <typename T>
 T* Engine::alloc_(int64_t count) {
   void* p = nullptr;
   if (count <= 0) count = 1;
-  if (hipMalloc(&p, size_t(count) * sizeof(T)) != hipSuccess) return nullptr;
+  if (hipMalloc(&p, size_t(count) * sizeof(T)) != hipSuccess) {
+    alloc_failed_ = true;
+    return nullptr;
+  }
   allocs_.push_back(p);
   bytes_ += int64_t(count) * int64_t(sizeof(T));
   return static_cast<T*>(p);
@@ -61,117 +64,99 @@ static int ceil_log2(int64_t v) {
   return b;
 }
 
-int Engine::setup(const std::vector<JobInput>& jobs, int max_neighbors, hipStream_t stream) {
-  if (jobs.empty()) { set_error("setup: no jobs"); return SMG_ERR_ARG; }
-  D_ = jobs[0].cfg.dim;
-  if (D_ != 2 && D_ != 3) { set_error("setup: dim must be 2 or 3"); return SMG_ERR_ARG; }
-  K_ = max_neighbors > 0 ? max_neighbors : 256;
-  if (stream) st_ = stream;
-  else {
-    HIP_TRY(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-    own_stream_ = true;
+// One job's parameters: everything except its slot, table and level-set ranges.
+int make_job_params(const JobInput& in, int D, JobParams& J) {
+  const smg_config& c = in.cfg;
+  std::memset(&J, 0, sizeof(J));
+  if (c.dim != D) { set_error("setup: all jobs of one engine need the same dim"); return SMG_ERR_ARG; }
+  if (in.n <= 0) { set_error("setup: job without particles"); return SMG_ERR_ARG; }
+  if (!in.phi || !in.normal || !in.tag) { set_error("setup: level set missing"); return SMG_ERR_STATE; }
+  J.D = D;
+  J.profile = c.profile;
+  J.remap_max = c.remap_max_attempts > 0 ? c.remap_max_attempts : 100000;
+  double dl = 1.e20, dmin = 1.e300;
+  for (int a = 0; a < 3; a++) {
+    const bool on = a < D;
+    J.sym[a] = on ? c.sym[a] : 0;
+    J.num_pkg[a] = on ? c.num_pkg[a] : 1;
+    if (on && c.num_pkg[a] <= 0) { set_error("setup: num_pkg must be positive"); return SMG_ERR_ARG; }
+    J.ng[a] = on ? 4 * c.num_pkg[a] + 2 : 1;
+    J.box_l[a] = on ? c.box_l[a] : 0.;
+    J.domain[a] = on ? c.domain[a] : 1.;
+    J.box_r[a] = J.box_l[a] + J.domain[a];
+    J.pbox_l[a] = on ? c.particle_box_l[a] : 0.;
+    J.pbox_r[a] = on ? c.particle_box_r[a] : 1.;
+    J.dpkg[a] = on ? c.domain[a] / double(c.num_pkg[a]) : 1.;
+    J.dcell[a] = on ? J.dpkg[a] / 4.0 : 1.;
+    if (on) { dl = std::min(dl, J.dcell[a]); dmin = std::min(dmin, J.domain[a]); }
+    const float lh = float(J.box_l[a]), rh = float(J.box_r[a]);
+    J.wl_h[a] = lh; J.wl_l[a] = float(J.box_l[a] - double(lh));
+    J.wh_h[a] = rh; J.wh_l[a] = float(J.box_r[a] - double(rh));
   }
-  jp_.assign(jobs.size(), JobParams{});
-  int64_t p0 = 0, tbl = 0, ls = 0;
-  for (size_t j = 0; j < jobs.size(); j++) {
-    const smg_config& c = jobs[j].cfg;
-    JobParams& J = jp_[j];
-    std::memset(&J, 0, sizeof(J));
-    if (c.dim != D_) { set_error("setup: all jobs of one engine need the same dim"); return SMG_ERR_ARG; }
-    if (jobs[j].n <= 0) { set_error("setup: job without particles"); return SMG_ERR_ARG; }
-    J.D = D_;
-    J.profile = c.profile;
-    J.remap_max = c.remap_max_attempts > 0 ? c.remap_max_attempts : 100000;
-    double dl = 1.e20, dmin = 1.e300;
-    for (int a = 0; a < 3; a++) {
-      const bool on = a < D_;
-      J.sym[a] = on ? c.sym[a] : 0;
-      J.num_pkg[a] = on ? c.num_pkg[a] : 1;
-      if (on && c.num_pkg[a] <= 0) { set_error("setup: num_pkg must be positive"); return SMG_ERR_ARG; }
-      J.ng[a] = on ? 4 * c.num_pkg[a] + 2 : 1;
-      J.box_l[a] = on ? c.box_l[a] : 0.;
-      J.domain[a] = on ? c.domain[a] : 1.;
-      J.box_r[a] = J.box_l[a] + J.domain[a];
-      J.pbox_l[a] = on ? c.particle_box_l[a] : 0.;
-      J.pbox_r[a] = on ? c.particle_box_r[a] : 1.;
-      J.dpkg[a] = on ? c.domain[a] / double(c.num_pkg[a]) : 1.;
-      J.dcell[a] = on ? J.dpkg[a] / 4.0 : 1.;
-      if (on) { dl = std::min(dl, J.dcell[a]); dmin = std::min(dmin, J.domain[a]); }
-      const float lh = float(J.box_l[a]), rh = float(J.box_r[a]);
-      J.wl_h[a] = lh; J.wl_l[a] = float(J.box_l[a] - double(lh));
-      J.wh_h[a] = rh; J.wh_l[a] = float(J.box_r[a] - double(rh));
-    }
-    J.dl = dl;
-    J.cell_ratio = c.cell_ratio;
-    J.cut_off = c.cut_off;
-    J.nu_coeff = c.nu_coeff;
-    J.min_dl = c.min_dl; J.max_dl = c.max_dl;
-    J.max_phi = c.max_phi > 0. ? c.max_phi : jobs[j].max_phi;
-    J.tanh25 = std::tanh(2.5);
-    J.lin_dx0 = c.lin_dx0; J.lin_dx1 = c.lin_dx1; J.lin_band = c.lin_band;
-    J.cos_thres = float(std::cos(c.angle_thres));
-    J.kdx = float(c.cut_off / c.cell_ratio);
-    {
-      // bound on the FP32 error of a relative position formed from the hi parts
-      double X = 0.;
-      for (int a = 0; a < D_; a++) X = std::max(X, std::max(std::fabs(J.box_l[a]), std::fabs(J.box_r[a])));
-      J.pf_tol = float(32.0 * std::ldexp(1.0, -23) * (X + 1.0));
-    }
-    J.seed = c.seed;
-    if (c.profile == SMG_PROFILE_TANH && !(J.max_phi > 0.) && c.max_dl != c.min_dl) {
-      set_error("setup: tanh profile needs max_phi > 0");
-      return SMG_ERR_ARG;
-    }
-    // cell-list levels: c0 >= 2 h_max (+ margin), c_L = c0 / 2^L >= 2 h_min (+ margin)
-    SpacingProfile prof(c, dl, J.max_phi > 0. ? J.max_phi : 1.0);
-    const double hmax = prof.dx_max() * c.cell_ratio / c.cut_off;
-    const double hmin = prof.dx_min() * c.cell_ratio / c.cut_off;
-    if (!(hmin > 0.)) { set_error("setup: spacing must be positive"); return SMG_ERR_ARG; }
-    const double R0 = hmax * c.cut_off * (1. + 1.e-9) + 1.e-9;
-    int64_t nbox = int64_t(std::floor(dmin / R0));
-    if (nbox < 1) nbox = 1;
-    J.c0 = dmin / double(nbox);
-    const int max_bits = D_ == 2 ? 15 : 10;
-    int nlev = 1;
-    while (nlev < kMaxLevels && std::ldexp(J.c0, -nlev) >= hmin * c.cut_off * (1. + 1.e-9) + 2.e-10) nlev++;
-    int bits0 = 0;
-    for (int a = 0; a < D_; a++) bits0 = std::max(bits0, ceil_log2(int64_t(std::ceil(J.domain[a] / J.c0))));
-    while (nlev > 1 && bits0 + nlev - 1 > max_bits) nlev--;
-    if (bits0 > max_bits) { set_error("setup: domain has too many cells per axis"); return SMG_ERR_CAPACITY; }
-    J.nlev = nlev;
-    int64_t off = 0;
-    for (int L = 0; L < nlev; L++) {
-      J.clen[L] = std::ldexp(J.c0, -L);
-      J.cinv[L] = 1.0 / J.clen[L];
-      for (int a = 0; a < 3; a++) J.lev_n[L][a] = a < D_ ? int32_t(std::ceil(J.domain[a] / J.clen[L])) : 1;
-      J.lev_bits[L] = bits0 + L;
-      J.lev_off[L] = off;
-      off += int64_t(1) << (D_ * J.lev_bits[L]);
-    }
-    J.tbl_size = off + 1;   // + pad cell for dead slots
-    J.tbl_base = tbl;
-    tbl += J.tbl_size;
-    J.lset_off = ls;
-    ls += int64_t(J.ng[0]) * J.ng[1] * J.ng[2];
-    J.n = jobs[j].n;
-    J.cap = (J.n + kTile - 1) / kTile * kTile;
-    J.p0 = p0;
-    p0 += J.cap;
+  J.dl = dl;
+  J.cell_ratio = c.cell_ratio;
+  J.cut_off = c.cut_off;
+  J.nu_coeff = c.nu_coeff;
+  J.min_dl = c.min_dl; J.max_dl = c.max_dl;
+  J.max_phi = c.max_phi > 0. ? c.max_phi : in.max_phi;
+  J.tanh25 = std::tanh(2.5);
+  J.lin_dx0 = c.lin_dx0; J.lin_dx1 = c.lin_dx1; J.lin_band = c.lin_band;
+  J.cos_thres = float(std::cos(c.angle_thres));
+  J.kdx = float(c.cut_off / c.cell_ratio);
+  {
+    // bound on the FP32 error of a relative position formed from the hi parts
+    double X = 0.;
+    for (int a = 0; a < D; a++) X = std::max(X, std::max(std::fabs(J.box_l[a]), std::fabs(J.box_r[a])));
+    J.pf_tol = float(32.0 * std::ldexp(1.0, -23) * (X + 1.0));
   }
+  J.seed = c.seed;
+  if (c.profile == SMG_PROFILE_TANH && !(J.max_phi > 0.) && c.max_dl != c.min_dl) {
+    set_error("setup: tanh profile needs max_phi > 0");
+    return SMG_ERR_ARG;
+  }
+  // cell-list levels: c0 >= 2 h_max (+ margin), c_L = c0 / 2^L >= 2 h_min (+ margin)
+  SpacingProfile prof(c, dl, J.max_phi > 0. ? J.max_phi : 1.0);
+  const double hmax = prof.dx_max() * c.cell_ratio / c.cut_off;
+  const double hmin = prof.dx_min() * c.cell_ratio / c.cut_off;
+  if (!(hmin > 0.)) { set_error("setup: spacing must be positive"); return SMG_ERR_ARG; }
+  const double R0 = hmax * c.cut_off * (1. + 1.e-9) + 1.e-9;
+  int64_t nbox = int64_t(std::floor(dmin / R0));
+  if (nbox < 1) nbox = 1;
+  J.c0 = dmin / double(nbox);
+  const int max_bits = D == 2 ? 15 : 10;
+  int nlev = 1;
+  while (nlev < kMaxLevels && std::ldexp(J.c0, -nlev) >= hmin * c.cut_off * (1. + 1.e-9) + 2.e-10) nlev++;
+  int bits0 = 0;
+  for (int a = 0; a < D; a++) bits0 = std::max(bits0, ceil_log2(int64_t(std::ceil(J.domain[a] / J.c0))));
+  while (nlev > 1 && bits0 + nlev - 1 > max_bits) nlev--;
+  if (bits0 > max_bits) { set_error("setup: domain has too many cells per axis"); return SMG_ERR_CAPACITY; }
+  J.nlev = nlev;
+  int64_t off = 0;
+  for (int L = 0; L < nlev; L++) {
+    J.clen[L] = std::ldexp(J.c0, -L);
+    J.cinv[L] = 1.0 / J.clen[L];
+    for (int a = 0; a < 3; a++) J.lev_n[L][a] = a < D ? int32_t(std::ceil(J.domain[a] / J.clen[L])) : 1;
+    J.lev_bits[L] = bits0 + L;
+    J.lev_off[L] = off;
+    off += int64_t(1) << (D * J.lev_bits[L]);
+  }
+  J.tbl_size = off + 1;   // + pad cell for dead slots
+  J.n = in.n;
+  J.cap = (J.n + kTile - 1) / kTile * kTile;
+  return SMG_OK;
+}
+
+int64_t job_lset_cells(const JobParams& J) { return int64_t(J.ng[0]) * J.ng[1] * J.ng[2]; }
+
+int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, int spill_per_slot) {
   if (tbl >= (int64_t(1) << 31) - 2) { set_error("setup: cell tables exceed 2^31 entries"); return SMG_ERR_CAPACITY; }
-  if (p0 >= (int64_t(1) << kJBits)) { set_error("setup: more than 2^26 particle slots"); return SMG_ERR_CAPACITY; }
-  cap_total_ = p0;
-  ntiles_ = p0 / kTile;
+  if (slots >= (int64_t(1) << kJBits)) { set_error("setup: more than 2^26 particle slots"); return SMG_ERR_CAPACITY; }
+  cap_total_ = slots;
   n_tbl_ = tbl;
   n_ls_ = ls;
-  tile_job_h_.assign(ntiles_, 0);
-  for (size_t j = 0; j < jobs.size(); j++)
-    for (int64_t t = jp_[j].p0 / kTile; t < (jp_[j].p0 + jp_[j].cap) / kTile; t++) tile_job_h_[t] = int32_t(j);
-
-  // device buffers
-  d_jp_ = alloc_<JobParams>(int64_t(jp_.size()));
-  d_tile_job_ = alloc_<int32_t>(ntiles_);
-  d_red_ = alloc_<JobRed>(int64_t(jp_.size()));
+  d_jp_ = alloc_<JobParams>(njobs);
+  d_tile_job_ = alloc_<int32_t>(slots / kTile);
+  d_red_ = alloc_<JobRed>(njobs);
   d_ls_ = alloc_<LsCell>(n_ls_);
   PState* S2[2] = {&A_, &B_};
   for (PState* S : S2) {
@@ -203,13 +188,15 @@ int Engine::setup(const std::vector<JobInput>& jobs, int max_neighbors, hipStrea
   W_.bsum = alloc_<int32_t>(nb + 1);
   W_.nbl = alloc_<int32_t>(int64_t(K_) * cap_total_);
   W_.nbc = alloc_<int32_t>(cap_total_);
-  W_.step_ctr = alloc_<int64_t>(1);
-  for (void* p : allocs_)
-    if (!p) { set_error("setup: hipMalloc failed"); return SMG_ERR_HIP; }
-  if (std::find(allocs_.begin(), allocs_.end(), nullptr) != allocs_.end() || !W_.nbc) {
-    set_error("setup: device allocation failed");
-    return SMG_ERR_HIP;
-  }
+  W_.nbo = alloc_<int32_t>(cap_total_);
+  W_.ovq = alloc_<int32_t>(cap_total_);
+  // spill pool: by default 32 entries per slot on average beyond the inline K (the
+  // unrelaxed random start and strong h gradients are where lists run long)
+  const int64_t sps = spill_per_slot > 0 ? spill_per_slot : 32;
+  W_.spill_cap = std::min<int64_t>(std::max<int64_t>(int64_t(1) << 10, sps * cap_total_), (int64_t(1) << 31) - 1);
+  W_.spill = alloc_<int32_t>(W_.spill_cap);
+  W_.ctr = alloc_<unsigned long long>(2);
+  if (alloc_failed_) { set_error("setup: device allocation failed (hipMalloc)"); return SMG_ERR_HIP; }
   W_.jp = d_jp_;
   W_.tile_job = d_tile_job_;
   W_.red = d_red_;
@@ -217,29 +204,14 @@ int Engine::setup(const std::vector<JobInput>& jobs, int max_neighbors, hipStrea
   W_.cap_total = cap_total_;
   W_.n_tbl = n_tbl_;
   W_.K = K_;
-
-  // level sets packed to 24-byte cells
-  std::vector<LsCell> hls(n_ls_);
-  for (size_t j = 0; j < jobs.size(); j++) {
-    const JobParams& J = jp_[j];
-    const int64_t m = int64_t(J.ng[0]) * J.ng[1] * J.ng[2];
-    if (!jobs[j].phi || !jobs[j].normal || !jobs[j].tag) { set_error("setup: level set missing"); return SMG_ERR_STATE; }
-    for (int64_t q = 0; q < m; q++) {
-      LsCell& cc = hls[J.lset_off + q];
-      cc.phi = jobs[j].phi[q];
-      cc.n[0] = jobs[j].normal[3 * q + 0];
-      cc.n[1] = jobs[j].normal[3 * q + 1];
-      cc.n[2] = jobs[j].normal[3 * q + 2];
-      cc.tag = jobs[j].tag[q];
-    }
-  }
-  HIP_TRY(hipMemcpyAsync(d_ls_, hls.data(), sizeof(LsCell) * n_ls_, hipMemcpyHostToDevice, st_));
-  HIP_TRY(hipMemcpyAsync(d_jp_, jp_.data(), sizeof(JobParams) * jp_.size(), hipMemcpyHostToDevice, st_));
-  HIP_TRY(hipMemcpyAsync(d_tile_job_, tile_job_h_.data(), sizeof(int32_t) * ntiles_, hipMemcpyHostToDevice, st_));
-  HIP_TRY(hipMemsetAsync(d_red_, 0, sizeof(JobRed) * jp_.size(), st_));
+  W_.tile0 = 0;
+  HIP_TRY(hipMemsetAsync(d_red_, 0, sizeof(JobRed) * njobs, st_));
+  HIP_TRY(hipMemsetAsync(d_tile_job_, 0xFF, sizeof(int32_t) * (slots / kTile), st_));   // -1: no job
   HIP_TRY(hipMemsetAsync(W_.counts, 0, sizeof(int32_t) * n_tbl_, st_));
+  HIP_TRY(hipMemsetAsync(W_.starts, 0, sizeof(int32_t) * (n_tbl_ + 1), st_));
   HIP_TRY(hipMemsetAsync(W_.nbc, 0, sizeof(int32_t) * cap_total_, st_));
-  HIP_TRY(hipMemsetAsync(W_.step_ctr, 0, sizeof(int64_t), st_));
+  HIP_TRY(hipMemsetAsync(W_.ctr, 0, 2 * sizeof(unsigned long long), st_));
+  HIP_TRY(hipMemsetAsync(W_.nbo, 0, sizeof(int32_t) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.nu_a, 0, sizeof(float) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.nu_b, 0, sizeof(float) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.sum, 0, sizeof(float) * cap_total_, st_));
@@ -248,29 +220,269 @@ int Engine::setup(const std::vector<JobInput>& jobs, int max_neighbors, hipStrea
     HIP_TRY(hipMemsetAsync(W_.a1[a], 0, sizeof(float) * cap_total_, st_));
     HIP_TRY(hipMemsetAsync(W_.a2[a], 0, sizeof(float) * cap_total_, st_));
   }
-  HIP_TRY(hipStreamSynchronize(st_));
-  red_h_.assign(jp_.size(), JobRed{});
+  red_h_.assign(njobs, JobRed{});
   return SMG_OK;
 }
 
-void Engine::begin_(const char* name) {
-  if (!profiling_) return;
-  if (pool_.size() < 2) {
-    hipEvent_t e;
-    for (int q = 0; q < 32; q++) {
-      if (hipEventCreate(&e) == hipSuccess) pool_.push_back(e);
+// level-set cells, parameters and tile ownership of job j (ranges already set in jp_[j])
+int Engine::upload_job_(int j, const JobInput& in) {
+  const JobParams& J = jp_[j];
+  const int64_t m = job_lset_cells(J);
+  std::vector<LsCell> hls(m);
+  for (int64_t q = 0; q < m; q++) {
+    LsCell& cc = hls[q];
+    cc.phi = in.phi[q];
+    cc.n[0] = in.normal[3 * q + 0];
+    cc.n[1] = in.normal[3 * q + 1];
+    cc.n[2] = in.normal[3 * q + 2];
+    cc.tag = in.tag[q];
+  }
+  HIP_TRY(hipMemcpyAsync(d_ls_ + J.lset_off, hls.data(), sizeof(LsCell) * m, hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipMemcpyAsync(d_jp_ + j, &J, sizeof(JobParams), hipMemcpyHostToDevice, st_));
+  const int64_t t0 = J.p0 / kTile, nt = J.cap / kTile;
+  for (int64_t t = t0; t < t0 + nt; t++) tile_job_h_[t] = int32_t(j);
+  HIP_TRY(hipMemcpyAsync(d_tile_job_ + t0, tile_job_h_.data() + t0, sizeof(int32_t) * nt, hipMemcpyHostToDevice, st_));
+  // the host staging buffer must outlive the copy
+  HIP_TRY(hipStreamSynchronize(st_));
+  return SMG_OK;
+}
+
+int Engine::begin_setup_(int D, int max_neighbors, hipStream_t stream) {
+  D_ = D;
+  if (D_ != 2 && D_ != 3) { set_error("setup: dim must be 2 or 3"); return SMG_ERR_ARG; }
+  K_ = max_neighbors > 0 ? max_neighbors : 64;
+  if (stream) st_ = stream;
+  else {
+    HIP_TRY(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    own_stream_ = true;
+  }
+  return SMG_OK;
+}
+
+int Engine::setup(const std::vector<JobInput>& jobs, int max_neighbors, int spill_per_slot, hipStream_t stream) {
+  if (jobs.empty()) { set_error("setup: no jobs"); return SMG_ERR_ARG; }
+  SMG_TRY(begin_setup_(jobs[0].cfg.dim, max_neighbors, stream));
+  jp_.assign(jobs.size(), JobParams{});
+  active_.assign(jobs.size(), 1);
+  int64_t p0 = 0, tbl = 0, ls = 0;
+  for (size_t j = 0; j < jobs.size(); j++) {
+    JobParams& J = jp_[j];
+    SMG_TRY(make_job_params(jobs[j], D_, J));
+    J.tbl_base = tbl;
+    tbl += J.tbl_size;
+    J.lset_off = ls;
+    ls += job_lset_cells(J);
+    J.p0 = p0;
+    p0 += J.cap;
+  }
+  tile_job_h_.assign(p0 / kTile, -1);
+  SMG_TRY(alloc_buffers_(p0, tbl, ls, int(jobs.size()), spill_per_slot));
+  for (size_t j = 0; j < jobs.size(); j++) SMG_TRY(upload_job_(int(j), jobs[j]));
+  ntiles_ = p0 / kTile;
+  HIP_TRY(hipStreamSynchronize(st_));
+  return SMG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// pool mode: capacity allocated once, jobs admitted into free ranges and retired
+
+int64_t RangeSet::take(int64_t size) {
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    if (it->second >= size) {
+      const int64_t at = it->first, len = it->second;
+      free_.erase(it);
+      if (len > size) free_[at + size] = len - size;
+      return at;
     }
   }
-  if (pool_.size() < 2) return;
-  cur_a_ = pool_.back();
-  pool_.pop_back();
+  return -1;
+}
+void RangeSet::give(int64_t at, int64_t size) {
+  auto nx = free_.lower_bound(at);
+  if (nx != free_.end() && at + size == nx->first) {   // merge with the next range
+    size += nx->second;
+    nx = free_.erase(nx);
+  }
+  if (nx != free_.begin()) {
+    auto pv = std::prev(nx);
+    if (pv->first + pv->second == at) {   // merge with the previous range
+      pv->second += size;
+      return;
+    }
+  }
+  free_[at] = size;
+}
+int64_t RangeSet::high_water() const {
+  // end of the last range in use = start of a trailing free range that reaches the end
+  if (!free_.empty()) {
+    auto last = std::prev(free_.end());
+    if (last->first + last->second == total_) return last->first;
+  }
+  return total_;
+}
+
+int Engine::setup_pool(int D, int64_t max_slots, int64_t max_table, int64_t max_lset_cells, int max_jobs,
+                       int max_neighbors, int spill_per_slot, hipStream_t stream) {
+  if (max_jobs <= 0 || max_slots < kTile || max_table <= 0 || max_lset_cells <= 0) {
+    set_error("setup_pool: capacities must be positive");
+    return SMG_ERR_ARG;
+  }
+  SMG_TRY(begin_setup_(D, max_neighbors, stream));
+  const int64_t slots = max_slots / kTile * kTile;
+  jp_.assign(max_jobs, JobParams{});
+  active_.assign(max_jobs, 0);
+  tile_job_h_.assign(slots / kTile, -1);
+  SMG_TRY(alloc_buffers_(slots, max_table, max_lset_cells, max_jobs, spill_per_slot));
+  is_pool_ = true;
+  tiles_free_.reset(slots / kTile);
+  tbl_free_.reset(max_table);
+  ls_free_.reset(max_lset_cells);
+  ntiles_ = 0;
+  W_.n_tbl = 0;
+  prerun_done_ = true;
+  HIP_TRY(hipStreamSynchronize(st_));
+  return SMG_OK;
+}
+
+void Engine::update_extent_() {
+  ntiles_ = tiles_free_.high_water();
+  W_.n_tbl = tbl_free_.high_water();
+}
+
+// Seeds the job into its ranges, then Set_particle_info, sort, Pre_run and sort, all
+// launched over the job's own tiles; the other jobs' state is untouched.
+int Engine::admit(const JobInput& in, int* job) {
+  if (!is_pool_) { set_error("admit: engine is not a pool"); return SMG_ERR_STATE; }
+  int j = -1;
+  for (int q = 0; q < int(active_.size()); q++)
+    if (!active_[q]) { j = q; break; }
+  if (j < 0) { set_error("admit: all job slots in use"); return SMG_ERR_CAPACITY; }
+  JobParams J;
+  SMG_TRY(make_job_params(in, D_, J));
+  const int64_t nt = J.cap / kTile, nls = job_lset_cells(J);
+  const int64_t t0 = tiles_free_.take(nt);
+  if (t0 < 0) { set_error("admit: no free slot range"); return SMG_ERR_CAPACITY; }
+  const int64_t tb = tbl_free_.take(J.tbl_size + 1);   // + the scan's end sentinel
+  if (tb < 0) { tiles_free_.give(t0, nt); set_error("admit: no free cell-table range"); return SMG_ERR_CAPACITY; }
+  const int64_t lo = ls_free_.take(nls);
+  if (lo < 0) {
+    tiles_free_.give(t0, nt);
+    tbl_free_.give(tb, J.tbl_size + 1);
+    set_error("admit: no free level-set range");
+    return SMG_ERR_CAPACITY;
+  }
+  J.p0 = t0 * kTile;
+  J.tbl_base = tb;
+  J.lset_off = lo;
+  jp_[j] = J;
+  active_[j] = 1;
+  SMG_TRY(upload_job_(j, in));
+  update_extent_();
+  const JobRed zero{};
+  HIP_TRY(hipMemcpyAsync(d_red_ + j, &zero, sizeof(JobRed), hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipMemsetAsync(W_.counts + tb, 0, sizeof(int32_t) * (J.tbl_size + 1), st_));
+  Work Ws = W_;
+  Ws.tile0 = t0;
+  launch_seed(st_, D_, nt, Ws, A_);
+  launch_map(st_, D_, nt, Ws, A_, kModeInit, 0);
+  SMG_TRY(sort_job_(j));
+  launch_map(st_, D_, nt, Ws, A_, kModePrerun, 0);
+  SMG_TRY(sort_job_(j));
+  const size_t fb = sizeof(float) * J.cap;
+  HIP_TRY(hipMemsetAsync(W_.nu_a + J.p0, 0, fb, st_));
+  HIP_TRY(hipMemsetAsync(W_.nu_b + J.p0, 0, fb, st_));
+  HIP_TRY(hipMemsetAsync(W_.sum + J.p0, 0, fb, st_));
+  HIP_TRY(hipMemsetAsync(W_.rec2 + J.p0, 0, sizeof(float4) * J.cap, st_));
+  for (int a = 0; a < 3; a++) {
+    HIP_TRY(hipMemsetAsync(W_.a1[a] + J.p0, 0, fb, st_));
+    HIP_TRY(hipMemsetAsync(W_.a2[a] + J.p0, 0, fb, st_));
+  }
+  HIP_TRY(hipGetLastError());
+  JobRed r;
+  SMG_TRY(red_of(j, &r));
+  if (r.fail) {
+    SMG_TRY(retire(j));
+    set_error("admit: no positive-phase position found within remap_max_attempts");
+    return SMG_ERR_REMAP;
+  }
+  // the job starts from clean reductions, as after Pre_run of a job of its own
+  HIP_TRY(hipMemcpyAsync(d_red_ + j, &zero, sizeof(JobRed), hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipStreamSynchronize(st_));
+  *job = j;
+  return SMG_OK;
+}
+
+// counting sort of one job's slots inside its own table segment
+int Engine::sort_job_(int j) {
+  const JobParams& J = jp_[j];
+  const int64_t t0 = J.p0 / kTile, nt = J.cap / kTile;
+  Work Wt = W_;
+  Wt.counts = W_.counts + J.tbl_base;
+  Wt.starts = W_.starts + J.tbl_base;
+  Wt.n_tbl = J.tbl_size;
+  launch_scan(st_, Wt);
+  Work Ws = W_;
+  Ws.tile0 = t0;
+  launch_scatter(st_, nt, Ws);
+  launch_segsort(st_, nt, Ws, A_);
+  launch_gather(st_, D_, nt, Ws, A_, B_);
+  // the job's new order is in B; the other jobs stay current in A, so copy back
+  const int64_t p0 = J.p0, cap = J.cap;
+  for (int a = 0; a < D_; a++) {
+    HIP_TRY(hipMemcpyAsync(A_.xh[a] + p0, B_.xh[a] + p0, sizeof(float) * cap, hipMemcpyDeviceToDevice, st_));
+    HIP_TRY(hipMemcpyAsync(A_.xl[a] + p0, B_.xl[a] + p0, sizeof(float) * cap, hipMemcpyDeviceToDevice, st_));
+    HIP_TRY(hipMemcpyAsync(A_.nrm[a] + p0, B_.nrm[a] + p0, sizeof(float) * cap, hipMemcpyDeviceToDevice, st_));
+  }
+  HIP_TRY(hipMemcpyAsync(A_.h + p0, B_.h + p0, sizeof(double) * cap, hipMemcpyDeviceToDevice, st_));
+  HIP_TRY(hipMemcpyAsync(A_.type + p0, B_.type + p0, sizeof(int32_t) * cap, hipMemcpyDeviceToDevice, st_));
+  HIP_TRY(hipMemcpyAsync(A_.id + p0, B_.id + p0, sizeof(int32_t) * cap, hipMemcpyDeviceToDevice, st_));
+  HIP_TRY(hipGetLastError());
+  return SMG_OK;
+}
+
+int Engine::retire(int j) {
+  if (!is_pool_ || j < 0 || j >= int(active_.size()) || !active_[j]) { set_error("retire: no such job"); return SMG_ERR_ARG; }
+  const JobParams& J = jp_[j];
+  const int64_t t0 = J.p0 / kTile, nt = J.cap / kTile;
+  for (int64_t t = t0; t < t0 + nt; t++) tile_job_h_[t] = -1;
+  HIP_TRY(hipMemcpyAsync(d_tile_job_ + t0, tile_job_h_.data() + t0, sizeof(int32_t) * nt, hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipStreamSynchronize(st_));
+  tiles_free_.give(t0, nt);
+  tbl_free_.give(J.tbl_base, J.tbl_size + 1);
+  ls_free_.give(J.lset_off, job_lset_cells(J));
+  active_[j] = 0;
+  jp_[j].n = 0;
+  update_extent_();
+  return SMG_OK;
+}
+
+int Engine::red_of(int j, JobRed* r) {
+  HIP_TRY(hipMemcpyAsync(&red_h_[j], d_red_ + j, sizeof(JobRed), hipMemcpyDeviceToHost, st_));
+  HIP_TRY(hipStreamSynchronize(st_));
+  *r = red_h_[j];
+  return SMG_OK;
+}
+
+int Engine::download_reds() { return download_red_(); }
+
+void Engine::begin_(const char* name) {
+  if (!profiling_) return;
+  if (ev_pool_.size() < 2) {
+    hipEvent_t e;
+    for (int q = 0; q < 32; q++) {
+      if (hipEventCreate(&e) == hipSuccess) ev_pool_.push_back(e);
+    }
+  }
+  if (ev_pool_.size() < 2) return;
+  cur_a_ = ev_pool_.back();
+  ev_pool_.pop_back();
   cur_name_ = name;
   (void)hipEventRecord(cur_a_, st_);
 }
 void Engine::end_() {
   if (!profiling_ || !cur_a_) return;
-  hipEvent_t b = pool_.back();
-  pool_.pop_back();
+  hipEvent_t b = ev_pool_.back();
+  ev_pool_.pop_back();
   (void)hipEventRecord(b, st_);
   recs_.push_back({cur_name_, cur_a_, b});
   cur_a_ = nullptr;
@@ -360,7 +572,6 @@ int Engine::prerun() {
   SMG_TRY(sort_particles());
   HIP_TRY(hipMemsetAsync(W_.nu_a, 0, sizeof(float) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.nu_b, 0, sizeof(float) * cap_total_, st_));
-  HIP_TRY(hipMemsetAsync(W_.step_ctr, 0, sizeof(int64_t), st_));
   for (int a = 0; a < 3; a++) {
     HIP_TRY(hipMemsetAsync(W_.a1[a], 0, sizeof(float) * cap_total_, st_));
     HIP_TRY(hipMemsetAsync(W_.a2[a], 0, sizeof(float) * cap_total_, st_));
@@ -378,6 +589,7 @@ int Engine::prerun() {
 
 int Engine::step(int64_t nsteps) {
   if (!prerun_done_) { set_error("step: call prerun (or seed) first"); return SMG_ERR_STATE; }
+  if (ntiles_ == 0) return SMG_OK;   // an empty pool
   for (int64_t s = 0; s < nsteps; s++) {
     const int64_t idx = step_ + 1;
     const int par = int(idx & 1);
@@ -388,6 +600,9 @@ int Engine::step(int64_t nsteps) {
     SMG_TRY(sort_particles());
     begin_("k_nb");
     launch_nb(st_, D_, ntiles_, W_, A_);
+    end_();
+    begin_("k_nb_spill");
+    launch_nb_spill(st_, D_, ntiles_, W_, A_);
     end_();
     begin_("k_force1");
     launch_force1(st_, D_, ntiles_, W_, A_, par);
@@ -457,10 +672,12 @@ int Engine::download_red_() {
 int Engine::check_health() {
   SMG_TRY(download_red_());
   for (size_t j = 0; j < red_h_.size(); j++) {
+    if (!active_[j]) continue;
     if (red_h_[j].fail) { set_error("job " + std::to_string(j) + ": re-seeding failed"); return SMG_ERR_REMAP; }
     if (red_h_[j].overflow) {
-      set_error("job " + std::to_string(j) + ": neighbour list overflow (need " + std::to_string(red_h_[j].maxnb) +
-                " > max_neighbors " + std::to_string(K_) + ")");
+      set_error("job " + std::to_string(j) + ": neighbour spill pool exhausted (longest list " +
+                std::to_string(red_h_[j].maxnb) + ", inline " + std::to_string(K_) + ", pool " +
+                std::to_string(W_.spill_cap) + " entries); raise max_neighbors or spill_per_slot");
       return SMG_ERR_CAPACITY;
     }
   }
@@ -557,10 +774,16 @@ int Engine::get_neighbors(int job, int64_t* offs, int32_t* ids, int32_t* codes, 
   if (step_ == 0) { set_error("get_neighbors: no step taken yet"); return SMG_ERR_STATE; }
   const JobParams& J = jp_[job];
   const int64_t cap = J.cap, n = J.n;
-  std::vector<int32_t> id(cap_total_), cnt(cap);
+  std::vector<int32_t> id(cap_total_), cnt(cap), spo(cap);
   HIP_TRY(hipStreamSynchronize(st_));
   HIP_TRY(hipMemcpy(id.data(), A_.id, sizeof(int32_t) * cap_total_, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(cnt.data(), W_.nbc + J.p0, sizeof(int32_t) * cap, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(spo.data(), W_.nbo + J.p0, sizeof(int32_t) * cap, hipMemcpyDeviceToHost));
+  unsigned long long used[2] = {0ull, 0ull};
+  HIP_TRY(hipMemcpy(used, W_.ctr, sizeof(used), hipMemcpyDeviceToHost));
+  const int64_t nsp = std::min<int64_t>(int64_t(used[1]), W_.spill_cap);
+  std::vector<int32_t> spl(std::max<int64_t>(nsp, 1));
+  if (nsp > 0) HIP_TRY(hipMemcpy(spl.data(), W_.spill, sizeof(int32_t) * nsp, hipMemcpyDeviceToHost));
   std::vector<int32_t> ent(int64_t(K_) * cap);
   for (int k = 0; k < K_; k++)
     HIP_TRY(hipMemcpy(ent.data() + int64_t(k) * cap, W_.nbl + int64_t(k) * cap_total_ + J.p0, sizeof(int32_t) * cap,
@@ -580,7 +803,13 @@ int Engine::get_neighbors(int job, int64_t* offs, int32_t* ids, int32_t* codes, 
     const int64_t s = slot_of[i];
     tmp.clear();
     for (int k = 0; k < cnt[s]; k++) {
-      const uint32_t e = uint32_t(ent[int64_t(k) * cap + s]);
+      uint32_t e;
+      if (k < K_) e = uint32_t(ent[int64_t(k) * cap + s]);
+      else {
+        const int64_t at = int64_t(spo[s]) + (k - K_);
+        if (at >= nsp) { set_error("get_neighbors: spill entry out of range"); return SMG_ERR_STATE; }
+        e = uint32_t(spl[at]);
+      }
       const int64_t q = e & kJMask;
       tmp.push_back({id[q], int32_t(e >> kJBits)});
     }
@@ -618,7 +847,8 @@ int Engine::get_stats(int job, smg_stats* st) {
     st->dt1 = bits_to_double(R.dt1[par]);
     st->dt2 = bits_to_double(R.dt2[par]);
     st->max_v = bits_to_double(R.maxv[par]);
-    st->residual = bits_to_double(R.res[par]);
+    st->residual = double(R.rsum[par]) / kResScale / double(J.n);
+    st->res_max = bits_to_double(R.res[par]);
     st->max_disp = bits_to_double(R.maxdisp[par]);
     std::vector<int32_t> cnt(J.cap);
     HIP_TRY(hipMemcpy(cnt.data(), W_.nbc + J.p0, sizeof(int32_t) * J.cap, hipMemcpyDeviceToHost));
@@ -637,8 +867,8 @@ int Engine::profile_get(std::vector<std::string>& names, std::vector<double>& av
       p.second += 1;
       last_ms_[r.name] = ms;
     }
-    pool_.push_back(r.a);
-    pool_.push_back(r.b);
+    ev_pool_.push_back(r.a);
+    ev_pool_.push_back(r.b);
   }
   recs_.clear();
   names.clear();
@@ -657,8 +887,8 @@ void Engine::profile_enable(bool on) {
   if (on) {
     (void)hipStreamSynchronize(st_);
     for (auto& r : recs_) {
-      pool_.push_back(r.a);
-      pool_.push_back(r.b);
+      ev_pool_.push_back(r.a);
+      ev_pool_.push_back(r.b);
     }
     recs_.clear();
     prof_.clear();

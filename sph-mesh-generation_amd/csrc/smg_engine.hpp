@@ -24,6 +24,32 @@ struct JobInput {
   const int8_t* tag = nullptr;
 };
 
+// first-fit allocator of [0, total) ranges with coalescing on release
+class RangeSet {
+ public:
+  void reset(int64_t total) {
+    total_ = total;
+    free_.clear();
+    if (total > 0) free_[0] = total;
+  }
+  int64_t take(int64_t size);            // start of a free range of this size, or -1
+  void give(int64_t at, int64_t size);
+  int64_t high_water() const;            // end of the last range in use
+  int64_t total() const { return total_; }
+  int64_t free_total() const {
+    int64_t s = 0;
+    for (auto& kv : free_) s += kv.second;
+    return s;
+  }
+
+ private:
+  int64_t total_ = 0;
+  std::map<int64_t, int64_t> free_;   // start -> length
+};
+
+int make_job_params(const JobInput& in, int D, JobParams& J);
+int64_t job_lset_cells(const JobParams& J);
+
 class Engine {
  public:
   Engine() = default;
@@ -32,7 +58,18 @@ class Engine {
   Engine& operator=(const Engine&) = delete;
 
   // Allocates and uploads everything for the given jobs (all of dimension D).
-  int setup(const std::vector<JobInput>& jobs, int max_neighbors, hipStream_t stream = nullptr);
+  int setup(const std::vector<JobInput>& jobs, int max_neighbors, int spill_per_slot = 0, hipStream_t stream = nullptr);
+
+  // pool mode: capacities allocated once; jobs enter and leave between steps
+  int setup_pool(int D, int64_t max_slots, int64_t max_table, int64_t max_lset_cells, int max_jobs,
+                 int max_neighbors, int spill_per_slot, hipStream_t stream = nullptr);
+  int admit(const JobInput& in, int* job);   // seed + Pre_run of the new job only
+  int retire(int job);
+  bool active(int job) const { return job >= 0 && job < int(active_.size()) && active_[job]; }
+  int red_of(int job, JobRed* r);
+  int download_reds();
+  const JobRed& red_host(int job) const { return red_h_[job]; }
+  int64_t used_tiles() const { return is_pool_ ? tiles_free_.total() - tiles_free_.free_total() : ntiles_; }
 
   int seed();                                                   // Random_particle_distribution
   int init_positions(int job, const double* x, const int32_t* type);  // upload one job's positions
@@ -69,6 +106,11 @@ class Engine {
 
  private:
   int sort_particles();
+  int sort_job_(int job);
+  int begin_setup_(int D, int max_neighbors, hipStream_t stream);
+  int alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, int spill_per_slot);
+  int upload_job_(int job, const JobInput& in);
+  void update_extent_();
   void begin_(const char* name);
   void end_();
   int download_red_();
@@ -77,6 +119,10 @@ class Engine {
   hipStream_t st_ = nullptr;
   bool own_stream_ = false;
   std::vector<JobParams> jp_;
+  std::vector<uint8_t> active_;
+  bool is_pool_ = false;
+  bool alloc_failed_ = false;
+  RangeSet tiles_free_, tbl_free_, ls_free_;
   std::vector<int32_t> tile_job_h_;
   int64_t ntiles_ = 0, cap_total_ = 0, n_tbl_ = 0, n_ls_ = 0;
   int K_ = 64;
@@ -97,7 +143,7 @@ class Engine {
   bool profiling_ = false;
   struct Rec { std::string name; hipEvent_t a, b; };
   std::vector<Rec> recs_;
-  std::vector<hipEvent_t> pool_;
+  std::vector<hipEvent_t> ev_pool_;
   std::map<std::string, std::pair<double, int64_t>> prof_;
   std::map<std::string, double> last_ms_;
   hipEvent_t cur_a_ = nullptr;

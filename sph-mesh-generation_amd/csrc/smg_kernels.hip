@@ -78,6 +78,28 @@ struct OpMin { __device__ double operator()(double a, double b) const { return a
 struct OpMax { __device__ double operator()(double a, double b) const { return a > b ? a : b; } };
 struct OpAdd { __device__ int operator()(int a, int b) const { return a + b; } };
 struct OpMaxI { __device__ int operator()(int a, int b) const { return a > b ? a : b; } };
+struct OpAddU {
+  __device__ unsigned long long operator()(unsigned long long a, unsigned long long b) const { return a + b; }
+};
+
+// neighbour entry k of a slot (spill offset off is read only when the list spills)
+__device__ __forceinline__ uint32_t nb_entry(const Work& W, int64_t slot, int k, int32_t off) {
+  return k < W.K ? uint32_t(W.nbl[int64_t(k) * W.cap_total + slot]) : uint32_t(W.spill[int64_t(off) + (k - W.K)]);
+}
+
+// |v| dt2 / dx of one particle, its fixed-point form for the order-free sum, and |v| dt2
+struct Disp {
+  double r, md;
+  unsigned long long rq;
+};
+__device__ __forceinline__ Disp disp_of(double U, double dt2, double h, const JobParams& J) {
+  Disp d;
+  d.md = U * dt2;
+  d.r = d.md / (h * J.cut_off / J.cell_ratio);
+  const double q = d.r * kResScale + 0.5;
+  d.rq = (q < 9.0e15) ? (unsigned long long)q : 9000000000000000ull;   // clamp: 2^53 > 9e15
+  return d;
+}
 
 __device__ __forceinline__ int64_t ls_index(const JobParams& J, int D, int gx, int gy, int gz) {
   const int64_t a = gx + 1, b = (D > 1) ? gy + 1 : 0, c = (D > 2) ? gz + 1 : 0;
@@ -212,10 +234,13 @@ template <int D>
 __device__ __forceinline__ void cell_at(const JobParams& J, int L, const double* x, int* c) {
 #pragma unroll
   for (int a = 0; a < D; a++) {
-    int k = int(floor((x[a] - J.box_l[a]) * J.cinv[L]));
-    const int hi = J.lev_n[L][a] - 1;
-    k = k < 0 ? 0 : (k > hi ? hi : k);
-    c[a] = k;
+    // clamp in FP64 before the conversion so that a non-finite position still lands
+    // in a valid cell of the job's table
+    double t = floor((x[a] - J.box_l[a]) * J.cinv[L]);
+    const double hi = double(J.lev_n[L][a] - 1);
+    t = (t >= 0.) ? t : 0.;
+    t = (t <= hi) ? t : hi;
+    c[a] = int(t);
   }
 }
 
@@ -278,16 +303,21 @@ __device__ __forceinline__ void rel_pos(const JobParams& J, const float4& qh, co
 // ---------------------------------------------------------------------------
 // reduction slots of this step
 
-__global__ void k_reset(JobRed* red, int njobs, int par, int64_t* step_ctr) {
+__global__ void k_reset(JobRed* red, int njobs, int par, unsigned long long* ctr) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j == 0) *step_ctr += 1;   // re-seeding RNG stream of this step (graph replays advance it too)
+  if (j == 0) {   // neighbour spill queue and pool of this step
+    ctr[0] = 0ull;
+    ctr[1] = 0ull;
+  }
   if (j >= njobs) return;
   JobRed& R = red[j];
+  R.step += 1;   // re-seeding RNG stream of this step (graph replays advance it too)
   R.dt1[par] = 0x7FF0000000000000ull;
   R.dt2[par] = 0x7FF0000000000000ull;
   R.maxv[par] = 0ull;
   R.res[par ^ 1] = 0ull;
   R.maxdisp[par ^ 1] = 0ull;
+  R.rsum[par ^ 1] = 0ull;
   R.nremap[par] = 0;
 }
 
@@ -296,9 +326,10 @@ __global__ void k_reset(JobRed* red, int njobs, int par, int64_t* step_ctr) {
 
 template <int D>
 __global__ __launch_bounds__(kTile) void k_seed(Work W, PState S) {
-  const int j = W.tile_job[blockIdx.x];
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0) return;   // tile not owned by a job
   const JobParams& J = W.jp[j];
-  const int64_t slot = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const int64_t loc = slot - J.p0;
   int bad = 0;
   if (loc < J.n) {
@@ -359,11 +390,14 @@ template <int D>
 __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int par) {
   __shared__ double lds_d[4];
   __shared__ int lds_i[4];
-  const int j = W.tile_job[blockIdx.x];
+  __shared__ unsigned long long lds_u[4];
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const int64_t loc = slot - J.p0;
   double res = 0., md = 0.;
+  unsigned long long rq = 0ull;
   int nre = 0, bad = 0, nsurf = 0;
   uint32_t key;
   if (loc < J.n) {
@@ -383,8 +417,10 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
         U += v[a] * v[a];
       }
       U = sqrt(U);
-      md = U * dt2;
-      res = U * dt2 / (h * J.cut_off / J.cell_ratio);
+      const Disp dd = disp_of(U, dt2, h, J);
+      res = dd.r;
+      md = dd.md;
+      rq = dd.rq;
       if (type != SMG_SINGULARITY) {
 #pragma unroll
         for (int a = 0; a < D; a++) x[a] = x[a] + v[a] * dt2;
@@ -429,7 +465,7 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
       if (phi < 0. && type == SMG_REAL) {
         // Remap_this_particle_to_positive_phase (sph_mesh_generation.cpp:1885-1928)
         nre = 1;
-        const uint64_t stream = mode == kModeStep ? uint64_t(*W.step_ctr) : 0ull;
+        const uint64_t stream = mode == kModeStep ? uint64_t(W.red[j].step) : 0ull;
         const uint64_t key_r = rng_key(J.seed, stream, uint64_t(S.id[slot]));
         bool ok = false;
         for (int att = 0; att < J.remap_max && !ok; att++) {
@@ -475,14 +511,19 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
   atomicAdd(&W.counts[key], 1);
   res = block_reduce(res, OpMax(), lds_d);
   md = block_reduce(md, OpMax(), lds_d);
+  rq = block_reduce(rq, OpAddU(), lds_u);
   nre = block_reduce(nre, OpAdd(), lds_i);
   bad = block_reduce(bad, OpMaxI(), lds_i);
   nsurf = block_reduce(nsurf, OpAdd(), lds_i);
   if (threadIdx.x == 0) {
     JobRed& R = W.red[j];
-    if (mode == kModeStep) {
+    if (mode == kModeStep) {   // residual of the previous step, whose motion this pass applied
       atomicMax(&R.res[par ^ 1], dbits(res));
       atomicMax(&R.maxdisp[par ^ 1], dbits(md));
+      if (rq) {
+        atomicAdd(&R.rsum[par ^ 1], rq);
+        atomicAdd(&R.rcum, rq);
+      }
     }
     if (nre) atomicAdd(&R.nremap[par], nre);
     if (bad) atomicOr(&R.fail, 1);
@@ -556,20 +597,25 @@ __global__ __launch_bounds__(kTile) void k_scan_down(const int32_t* __restrict__
 // ---------------------------------------------------------------------------
 // counting sort by cell key
 
-__global__ __launch_bounds__(kTile) void k_scatter(Work W, int64_t nslots) {
-  const int64_t p = int64_t(blockIdx.x) * kTile + threadIdx.x;
-  if (p >= nslots) return;
+// A job's cell starts are taken relative to the start of its table segment, so the
+// slot ranges of the jobs may sit anywhere (gaps included) and still sort in place.
+__global__ __launch_bounds__(kTile) void k_scatter(Work W) {
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0) return;
+  const JobParams& J = W.jp[j];
+  const int64_t p = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const uint32_t k = W.key[p];
   const int r = atomicSub(&W.counts[k], 1) - 1;
-  const int64_t d = int64_t(W.starts[k]) + r;
+  const int64_t d = J.p0 + int64_t(W.starts[k] - W.starts[J.tbl_base]) + r;
   W.perm[d] = int32_t(p);
   W.skey[d] = k;
 }
 
 __global__ __launch_bounds__(kTile) void k_segsort(Work W, PState S) {
-  const int j = W.tile_job[blockIdx.x];
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0) return;
   const JobParams& J = W.jp[j];
-  const int64_t d = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const int64_t d = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const uint32_t k = W.skey[d];
   if (k == uint32_t(J.tbl_base + J.tbl_size - 1)) return;   // dead slots: no order needed
   if (d > J.p0 && W.skey[d - 1] == k) return;                 // not the head of its cell
@@ -591,7 +637,8 @@ __global__ __launch_bounds__(kTile) void k_segsort(Work W, PState S) {
 
 template <int D>
 __global__ __launch_bounds__(kTile) void k_gather(Work W, PState A, PState B) {
-  const int64_t d = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  if (W.tile_job[W.tile0 + blockIdx.x] < 0) return;
+  const int64_t d = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const int64_t p = W.perm[d];
 #pragma unroll
   for (int a = 0; a < D; a++) {
@@ -621,114 +668,155 @@ __device__ __forceinline__ void image_of(const JobParams& J, uint32_t code, cons
   }
 }
 
+// Enumerates the neighbours of one particle in a fixed order (mirror images, then
+// levels, then cells, then slots) and hands each entry (k, slot | code << 26) to emit.
+// Returns the number of neighbours.
+template <int D, class Emit>
+__device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, const JobParams& J, int64_t slot,
+                                            Emit emit) {
+  int cnt = 0;
+  const float4 r0 = W.rec0[slot], r1 = W.rec1[slot];
+  const float xh[3] = {r0.x, r0.y, r0.z}, xl[3] = {r1.x, r1.y, r1.z};
+  double xi[3] = {0., 0., 0.};
+#pragma unroll
+  for (int a = 0; a < D; a++) xi[a] = double(xh[a]) + double(xl[a]);
+  const double hi = S.h[slot];
+  const double ri = hi * J.cut_off + 1.e-10;
+  const float rfi = float(hi * J.cut_off);
+  const int li = level_of(J, hi);
+  const int32_t soff = int32_t(J.p0) - W.starts[J.tbl_base];   // table-relative starts -> slots
+  // mirror options per axis: none, low wall, high wall
+  int opt[3][3], nopt[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    nopt[a] = 1;
+    opt[a][0] = 0;
+    if (a < D && J.sym[a]) {
+      if (xi[a] - J.box_l[a] < J.c0) opt[a][nopt[a]++] = 1;
+      if (J.box_r[a] - xi[a] < J.c0) opt[a][nopt[a]++] = 2;
+    }
+  }
+  const float cutf = float(J.cut_off);
+  for (int o0 = 0; o0 < nopt[0]; o0++)
+    for (int o1 = 0; o1 < nopt[1]; o1++)
+      for (int o2 = 0; o2 < nopt[2]; o2++) {
+        const uint32_t code = uint32_t(opt[0][o0] | (opt[1][o1] << 2) | (opt[2][o2] << 4));
+        double y[3] = {0., 0., 0.};
+        image_of<D>(J, code, xi, y);
+        // FP32 relative position: dr_a = A_a + s_a * xh_q
+        float A[3] = {0.f, 0.f, 0.f}, sg[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+          const uint32_t m = (code >> (2 * a)) & 3u;
+          A[a] = m == 0 ? xh[a] : (m == 1 ? xh[a] - 2.f * J.wl_h[a] : xh[a] - 2.f * J.wh_h[a]);
+          sg[a] = m == 0 ? -1.f : 1.f;
+        }
+        for (int L = 0; L < J.nlev; L++) {
+          const int s = L < li ? L : li;
+          const int shift = D * (L - s);
+          int cy[3] = {0, 0, 0};
+          cell_at<D>(J, s, y, cy);
+          const int64_t base = J.tbl_base + J.lev_off[L];
+          for (int dz = (D > 2 ? -1 : 0); dz <= (D > 2 ? 1 : 0); dz++)
+            for (int dy = -1; dy <= 1; dy++)
+              for (int dx = -1; dx <= 1; dx++) {
+                int c[3] = {cy[0] + dx, cy[1] + dy, cy[2] + dz};
+                if (c[0] < 0 || c[0] >= J.lev_n[s][0] || c[1] < 0 || c[1] >= J.lev_n[s][1]) continue;
+                if (D > 2 && (c[2] < 0 || c[2] >= J.lev_n[s][2])) continue;
+                const uint64_t m = morton<D>(c);
+                const int32_t lo = W.starts[base + int64_t(m << shift)] + soff;
+                const int32_t hi2 = W.starts[base + int64_t((m + 1) << shift)] + soff;
+                for (int32_t q = lo; q < hi2; q++) {
+                  const float4 qr = W.rec0[q];
+                  const float qv[3] = {qr.x, qr.y, qr.z};
+                  float d2 = 0.f;
+#pragma unroll
+                  for (int a = 0; a < D; a++) {
+                    const float t = A[a] + sg[a] * qv[a];
+                    d2 += t * t;
+                  }
+                  const float r = fmaxf(rfi, qr.w * cutf);
+                  const float band = J.pf_tol + 1.e-6f * r;
+                  const float rlo = r - band, rhi = r + band;
+                  bool in;
+                  if (d2 < rlo * rlo) in = true;
+                  else if (d2 > rhi * rhi) in = false;
+                  else {
+                    // near the cutoff: the reference's FP64 predicate on the stored positions
+                    const float4 ql = W.rec1[q];
+                    const double xq[3] = {double(qr.x) + double(ql.x), double(qr.y) + double(ql.y),
+                                          double(qr.z) + double(ql.z)};
+                    double g[3];
+                    image_of<D>(J, code, xq, g);
+                    double dist = 0.0;
+#pragma unroll
+                    for (int a = 0; a < D; a++) {
+                      const double dr = xi[a] - g[a];
+                      dist += dr * dr;
+                    }
+                    dist = sqrt(dist);
+                    in = dist <= ri || dist <= S.h[q] * J.cut_off + 1.e-10;
+                  }
+                  if (in) {
+                    emit(cnt, int32_t(uint32_t(q) | (code << kJBits)));
+                    cnt++;
+                  }
+                }
+              }
+        }
+      }
+  return cnt;
+}
+
+// neighbour lists: the first K entries inline; a slot with more joins the spill queue
 template <int D>
 __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
   __shared__ int lds_i[4];
-  const int j = W.tile_job[blockIdx.x];
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const int64_t loc = slot - J.p0;
   int cnt = 0;
   if (loc < J.n) {
-    const float4 r0 = W.rec0[slot], r1 = W.rec1[slot];
-    const float xh[3] = {r0.x, r0.y, r0.z}, xl[3] = {r1.x, r1.y, r1.z};
-    double xi[3] = {0., 0., 0.};
-#pragma unroll
-    for (int a = 0; a < D; a++) xi[a] = double(xh[a]) + double(xl[a]);
-    const double hi = S.h[slot];
-    const double ri = hi * J.cut_off + 1.e-10;
-    const float rfi = float(hi * J.cut_off);
-    const int li = level_of(J, hi);
-    // mirror options per axis: none, low wall, high wall
-    int opt[3][3], nopt[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      nopt[a] = 1;
-      opt[a][0] = 0;
-      if (a < D && J.sym[a]) {
-        if (xi[a] - J.box_l[a] < J.c0) opt[a][nopt[a]++] = 1;
-        if (J.box_r[a] - xi[a] < J.c0) opt[a][nopt[a]++] = 2;
-      }
-    }
     const int64_t cap = W.cap_total;
     const int K = W.K;
-    const float cutf = float(J.cut_off);
-    for (int o0 = 0; o0 < nopt[0]; o0++)
-      for (int o1 = 0; o1 < nopt[1]; o1++)
-        for (int o2 = 0; o2 < nopt[2]; o2++) {
-          const uint32_t code = uint32_t(opt[0][o0] | (opt[1][o1] << 2) | (opt[2][o2] << 4));
-          double y[3] = {0., 0., 0.};
-          image_of<D>(J, code, xi, y);
-          // FP32 relative position: dr_a = A_a + s_a * xh_q
-          float A[3] = {0.f, 0.f, 0.f}, sg[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-          for (int a = 0; a < D; a++) {
-            const uint32_t m = (code >> (2 * a)) & 3u;
-            A[a] = m == 0 ? xh[a] : (m == 1 ? xh[a] - 2.f * J.wl_h[a] : xh[a] - 2.f * J.wh_h[a]);
-            sg[a] = m == 0 ? -1.f : 1.f;
-          }
-          for (int L = 0; L < J.nlev; L++) {
-            const int s = L < li ? L : li;
-            const int shift = D * (L - s);
-            int cy[3] = {0, 0, 0};
-            cell_at<D>(J, s, y, cy);
-            const int64_t base = J.tbl_base + J.lev_off[L];
-            for (int dz = (D > 2 ? -1 : 0); dz <= (D > 2 ? 1 : 0); dz++)
-              for (int dy = -1; dy <= 1; dy++)
-                for (int dx = -1; dx <= 1; dx++) {
-                  int c[3] = {cy[0] + dx, cy[1] + dy, cy[2] + dz};
-                  if (c[0] < 0 || c[0] >= J.lev_n[s][0] || c[1] < 0 || c[1] >= J.lev_n[s][1]) continue;
-                  if (D > 2 && (c[2] < 0 || c[2] >= J.lev_n[s][2])) continue;
-                  const uint64_t m = morton<D>(c);
-                  const int32_t lo = W.starts[base + int64_t(m << shift)];
-                  const int32_t hi2 = W.starts[base + int64_t((m + 1) << shift)];
-                  for (int32_t q = lo; q < hi2; q++) {
-                    const float4 qr = W.rec0[q];
-                    const float qv[3] = {qr.x, qr.y, qr.z};
-                    float d2 = 0.f;
-#pragma unroll
-                    for (int a = 0; a < D; a++) {
-                      const float t = A[a] + sg[a] * qv[a];
-                      d2 += t * t;
-                    }
-                    const float r = fmaxf(rfi, qr.w * cutf);
-                    const float band = J.pf_tol + 1.e-6f * r;
-                    const float rlo = r - band, rhi = r + band;
-                    bool in;
-                    if (d2 < rlo * rlo) in = true;
-                    else if (d2 > rhi * rhi) in = false;
-                    else {
-                      // near the cutoff: the reference's FP64 predicate on the stored positions
-                      const float4 ql = W.rec1[q];
-                      const double xq[3] = {double(qr.x) + double(ql.x), double(qr.y) + double(ql.y),
-                                            double(qr.z) + double(ql.z)};
-                      double g[3];
-                      image_of<D>(J, code, xq, g);
-                      double dist = 0.0;
-#pragma unroll
-                      for (int a = 0; a < D; a++) {
-                        const double dr = xi[a] - g[a];
-                        dist += dr * dr;
-                      }
-                      dist = sqrt(dist);
-                      in = dist <= ri || dist <= S.h[q] * J.cut_off + 1.e-10;
-                    }
-                    if (in) {
-                      if (cnt < K) W.nbl[int64_t(cnt) * cap + slot] = int32_t(uint32_t(q) | (code << kJBits));
-                      cnt++;
-                    }
-                  }
-                }
-          }
-        }
-    W.nbc[slot] = cnt < K ? cnt : K;
-  } else {
-    W.nbc[slot] = 0;
+    cnt = nb_enumerate<D>(W, S, J, slot, [&](int k, int32_t e) {
+      if (k < K) W.nbl[int64_t(k) * cap + slot] = e;
+    });
+    if (cnt > K) {
+      const unsigned long long qi = atomicAdd(&W.ctr[0], 1ull);
+      W.ovq[qi] = int32_t(slot);   // at most one entry per slot: qi < cap_total
+    }
   }
+  W.nbc[slot] = cnt;
   const int mx = block_reduce(cnt, OpMaxI(), lds_i);
-  if (threadIdx.x == 0 && mx > 0) {
-    atomicMax(&W.red[j].maxnb, mx);
-    if (mx > W.K) atomicOr(&W.red[j].overflow, 1);
+  if (threadIdx.x == 0 && mx > 0) atomicMax(&W.red[j].maxnb, mx);
+}
+
+// entries K.. of the queued slots, written to the spill pool; the enumeration repeats
+// the list pass exactly, so the entries keep their order.  A slot that finds the pool
+// full keeps its first K entries and flags its job.
+template <int D>
+__global__ __launch_bounds__(kTile) void k_nb_spill(Work W, PState S) {
+  const unsigned long long nq = W.ctr[0];
+  const int K = W.K;
+  for (unsigned long long i = uint64_t(blockIdx.x) * kTile + threadIdx.x; i < nq; i += uint64_t(gridDim.x) * kTile) {
+    const int64_t slot = W.ovq[i];
+    const int j = W.tile_job[slot / kTile];
+    const JobParams& J = W.jp[j];
+    const int cnt = W.nbc[slot];
+    const unsigned long long off = atomicAdd(&W.ctr[1], (unsigned long long)(cnt - K));
+    if (off + (unsigned long long)(cnt - K) > (unsigned long long)W.spill_cap) {
+      W.nbc[slot] = K;
+      atomicOr(&W.red[j].overflow, 1);
+      continue;
+    }
+    W.nbo[slot] = int32_t(off);
+    int32_t* dst = W.spill + off;
+    nb_enumerate<D>(W, S, J, slot, [&](int k, int32_t e) {
+      if (k >= K) dst[k - K] = e;
+    });
   }
 }
 
@@ -738,13 +826,13 @@ __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
 template <int D>
 __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
   __shared__ double lds_d[4];
-  const int j = W.tile_job[blockIdx.x];
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const int64_t loc = slot - J.p0;
   double dt = 1.0e300;
   if (loc < J.n) {
-    const int64_t cap = W.cap_total;
     const float4 r0 = W.rec0[slot], r1 = W.rec1[slot];
     const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
     float ni[3] = {0.f, 0.f, 0.f};
@@ -752,6 +840,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
     const float h = r0.w;
     const int type = __float_as_int(r1.w);
     const int cnt = W.nbc[slot];
+    const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
     const bool surf = type == SMG_SURFACE;
     if (surf) {
 #pragma unroll
@@ -761,7 +850,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
     float sum = 0.f;
     int ncount = 0;
     for (int k = 0; k < cnt; k++) {
-      const uint32_t e = uint32_t(W.nbl[int64_t(k) * cap + slot]);
+      const uint32_t e = nb_entry(W, slot, k, off);
       const int64_t q = e & kJMask;
       const uint32_t code = e >> kJBits;
       const float4 qh = W.rec0[q], ql = W.rec1[q];
@@ -800,7 +889,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
     const float vol2i = ipow<D>(dxi) * ipow<D>(dxi);
     float F[3] = {0.f, 0.f, 0.f};
     for (int k = 0; k < cnt; k++) {
-      const uint32_t e = uint32_t(W.nbl[int64_t(k) * cap + slot]);
+      const uint32_t e = nb_entry(W, slot, k, off);
       const int64_t q = e & kJMask;
       const uint32_t code = e >> kJBits;
       if (q == slot && code == 0) continue;
@@ -871,13 +960,13 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
 template <int D>
 __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
   __shared__ double lds_d[4];
-  const int j = W.tile_job[blockIdx.x];
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const int64_t loc = slot - J.p0;
   double dt = 1.0e300, U = 0.;
   if (loc < J.n) {
-    const int64_t cap = W.cap_total;
     const double dt1 = bitsd(W.red[j].dt1[par]);
     const float hdt = float(0.5 * dt1);
     const float4 r0 = W.rec0[slot], r1 = W.rec1[slot], r2 = W.rec2[slot];
@@ -899,6 +988,7 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
       for (int a = 0; a < D; a++) ni[a] = S.nrm[a][slot];
     }
     const int cnt = W.nbc[slot];
+    const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
     const float dxi = h * J.kdx;
     const float voli = ipow<D>(dxi);
     const float rhoi = 1.f / voli;
@@ -906,7 +996,7 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
     float F[3] = {0.f, 0.f, 0.f};
     if (etai != 0.f) {   // eta_ij = 0 for every pair when this particle's nu is 0
       for (int k = 0; k < cnt; k++) {
-        const uint32_t e = uint32_t(W.nbl[int64_t(k) * cap + slot]);
+        const uint32_t e = nb_entry(W, slot, k, off);
         const uint32_t code = e >> kJBits;
         if (code != 0) continue;   // mirror ghosts carry nu = 0, so eta_ij = 0 (Q6)
         const int64_t q = e & kJMask;
@@ -994,14 +1084,18 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
   }
 }
 
-// residual of the last step without moving anything (for statistics)
+// residual of the last step without moving anything (for statistics); the same
+// per-particle arithmetic as the residual that the next k_map accumulates
 template <int D>
 __global__ __launch_bounds__(kTile) void k_residual(Work W, PState S, int par) {
   __shared__ double lds_d[4];
-  const int j = W.tile_job[blockIdx.x];
+  __shared__ unsigned long long lds_u[4];
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = int64_t(blockIdx.x) * kTile + threadIdx.x;
+  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   double res = 0., md = 0.;
+  unsigned long long rq = 0ull;
   if (slot - J.p0 < J.n) {
     const double dt1 = bitsd(W.red[j].dt1[par]), dt2 = bitsd(W.red[j].dt2[par]);
     double U = 0.;
@@ -1012,14 +1106,18 @@ __global__ __launch_bounds__(kTile) void k_residual(Work W, PState S, int par) {
       U += v * v;
     }
     U = sqrt(U);
-    md = U * dt2;
-    res = md / (S.h[slot] * J.cut_off / J.cell_ratio);
+    const Disp dd = disp_of(U, dt2, S.h[slot], J);
+    res = dd.r;
+    md = dd.md;
+    rq = dd.rq;
   }
   res = block_reduce(res, OpMax(), lds_d);
   md = block_reduce(md, OpMax(), lds_d);
+  rq = block_reduce(rq, OpAddU(), lds_u);
   if (threadIdx.x == 0) {
     atomicMax(&W.red[j].res[par], dbits(res));
     atomicMax(&W.red[j].maxdisp[par], dbits(md));
+    if (rq) atomicAdd(&W.red[j].rsum[par], rq);
   }
 }
 
@@ -1028,6 +1126,7 @@ __global__ void k_clear_res(JobRed* red, int njobs, int par) {
   if (j < njobs) {
     red[j].res[par] = 0ull;
     red[j].maxdisp[par] = 0ull;
+    red[j].rsum[par] = 0ull;
   }
 }
 
@@ -1051,7 +1150,7 @@ __global__ void k_fp64_probe(int64_t n, const double* a, const double* b, double
   } while (0)
 
 void launch_reset(hipStream_t st, const Work& W, int njobs, int par) {
-  k_reset<<<(njobs + 255) / 256, 256, 0, st>>>(W.red, njobs, par, W.step_ctr);
+  k_reset<<<(njobs + 255) / 256, 256, 0, st>>>(W.red, njobs, par, W.ctr);
 }
 void launch_seed(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S) {
   if (D == 2) k_seed<2><<<ntiles, kTile, 0, st>>>(W, S);
@@ -1068,7 +1167,7 @@ void launch_scan(hipStream_t st, const Work& W) {
   k_scan_down<<<nb, kTile, 0, st>>>(W.counts, W.n_tbl, W.bsum, nb, W.starts);
 }
 void launch_scatter(hipStream_t st, int64_t ntiles, const Work& W) {
-  k_scatter<<<ntiles, kTile, 0, st>>>(W, ntiles * kTile);
+  k_scatter<<<ntiles, kTile, 0, st>>>(W);
 }
 void launch_segsort(hipStream_t st, int64_t ntiles, const Work& W, const PState& S) {
   k_segsort<<<ntiles, kTile, 0, st>>>(W, S);
@@ -1080,6 +1179,11 @@ void launch_gather(hipStream_t st, int D, int64_t ntiles, const Work& W, const P
 void launch_nb(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S) {
   if (D == 2) k_nb<2><<<ntiles, kTile, 0, st>>>(W, S);
   else k_nb<3><<<ntiles, kTile, 0, st>>>(W, S);
+}
+void launch_nb_spill(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S) {
+  const int64_t g = ntiles < 256 ? ntiles : 256;   // grid-stride over the (usually empty) queue
+  if (D == 2) k_nb_spill<2><<<g, kTile, 0, st>>>(W, S);
+  else k_nb_spill<3><<<g, kTile, 0, st>>>(W, S);
 }
 void launch_force1(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int par) {
   if (D == 2) k_force1<2><<<ntiles, kTile, 0, st>>>(W, S, par);

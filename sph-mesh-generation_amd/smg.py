@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libsmg.so")
 
 SMG_OK = 0
+SMG_ERR_ARG, SMG_ERR_HIP, SMG_ERR_CAPACITY, SMG_ERR_STATE, SMG_ERR_REMAP, SMG_ERR_NONFINITE = -1, -2, -3, -4, -5, -6
 SMG_REAL, SMG_SURFACE = 0, 1
 PROFILE_TANH, PROFILE_LINEAR = 0, 1
 PI_REF = 3.1415926   # glbparam.h:185
@@ -32,6 +33,7 @@ class SmgConfig(C.Structure):
         ("min_dl", C.c_double), ("max_dl", C.c_double), ("max_phi", C.c_double),
         ("lin_dx0", C.c_double), ("lin_dx1", C.c_double), ("lin_band", C.c_double),
         ("seed", C.c_uint64), ("remap_max_attempts", C.c_int32), ("max_neighbors", C.c_int32),
+        ("spill_per_slot", C.c_int32),
     ]
 
 
@@ -50,7 +52,7 @@ class SmgStats(C.Structure):
         ("n_remapped", C.c_int64), ("n_neighbor_entries", C.c_int64),
         ("max_neighbors_seen", C.c_int32), ("num_level", C.c_int32),
         ("dt1", C.c_double), ("dt2", C.c_double), ("max_v", C.c_double),
-        ("residual", C.c_double), ("max_disp", C.c_double),
+        ("residual", C.c_double), ("max_disp", C.c_double), ("res_max", C.c_double),
     ]
 
     def as_dict(self):
@@ -403,6 +405,124 @@ class Batch:
         _check(lib().smg_batch_get_state(self.h, int(j), _ptr(x), _ptr(v), _ptr(t), _ptr(hh), _ptr(nu)),
                "smg_batch_get_state")
         return {"x": x, "v": v, "type": t, "h": hh, "nu": nu}
+
+
+# ---------------------------------------------------------------------------
+# serving: continuous batching over a device-resident pool (smg_server_*)
+
+class SmgServerConfig(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("max_jobs", C.c_int32), ("max_slots", C.c_int64), ("max_table", C.c_int64),
+        ("max_lset_cells", C.c_int64), ("max_neighbors", C.c_int32), ("spill_per_slot", C.c_int32),
+        ("chunk_steps", C.c_int32), ("min_steps", C.c_int32), ("max_steps", C.c_int32), ("reserved", C.c_int32),
+        ("tol", C.c_double),
+    ]
+
+
+class SmgServerJob(C.Structure):
+    _fields_ = [("ticket", C.c_int32), ("status", C.c_int32), ("steps", C.c_int64), ("n_particle", C.c_int64),
+                ("window_residual", C.c_double)]
+
+
+JOB_CONVERGED, JOB_MAX_STEPS = 0, 1
+
+
+def _server_sigs(L):
+    P = C.POINTER
+    L.smg_server_footprint_of.argtypes = [P(SmgServerConfig), P(C.c_int64)]
+    L.smg_server_job_size.argtypes = [P(SmgConfig), C.c_int64, P(C.c_int64), P(C.c_int64), P(C.c_int64)]
+    L.smg_server_create.argtypes = [P(SmgServerConfig), P(C.c_void_p)]
+    L.smg_server_destroy.argtypes = [C.c_void_p]
+    L.smg_server_admit.argtypes = [C.c_void_p, P(SmgConfig), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                   P(C.c_int32)]
+    L.smg_server_advance.argtypes = [C.c_void_p, P(C.c_int32)]
+    L.smg_server_finished.argtypes = [C.c_void_p, P(SmgServerJob), C.c_int32, P(C.c_int32)]
+    L.smg_server_result.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, P(SmgStats)]
+    L.smg_server_usage.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_int64), P(C.c_int32)]
+
+
+def server_config(dim, max_jobs, max_slots, max_table, max_lset_cells, chunk_steps=50, min_steps=100,
+                  max_steps=3000, tol=1e-3, max_neighbors=0, spill_per_slot=0):
+    return SmgServerConfig(dim, max_jobs, int(max_slots), int(max_table), int(max_lset_cells), max_neighbors,
+                           spill_per_slot, chunk_steps, min_steps, max_steps, 0, tol)
+
+
+def server_footprint(scfg):
+    L = lib(); _server_sigs(L)
+    b = C.c_int64()
+    _check(L.smg_server_footprint_of(C.byref(scfg), C.byref(b)), "smg_server_footprint_of")
+    return b.value
+
+
+def job_size(cfg, n):
+    """(slots, table entries, level-set cells) one job takes in a server pool."""
+    L = lib(); _server_sigs(L)
+    a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+    _check(L.smg_server_job_size(C.byref(cfg), int(n), C.byref(a), C.byref(b), C.byref(c)), "smg_server_job_size")
+    return a.value, b.value, c.value
+
+
+class Server:
+    """Continuous batching: admit jobs into a device pool between chunks of steps,
+    step all of them together, retire the converged ones (smg_server_* of include/smg.h)."""
+
+    def __init__(self, scfg):
+        L = lib(); _server_sigs(L)
+        h = C.c_void_p()
+        _check(L.smg_server_create(C.byref(scfg), C.byref(h)), "smg_server_create")
+        self.h = h
+        self.cfg = scfg
+        self.n_of = {}
+
+    def close(self):
+        if self.h:
+            lib().smg_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, cfg, lset, n):
+        """Ticket of the admitted job, or None when it does not fit the pool now."""
+        t = C.c_int32()
+        rc = lib().smg_server_admit(self.h, C.byref(cfg), _ptr(lset["phi"]), _ptr(lset["normal"]),
+                                    _ptr(lset["tag"]), int(n), C.byref(t))
+        if rc == SMG_ERR_CAPACITY:
+            return None
+        _check(rc, "smg_server_admit")
+        self.n_of[t.value] = int(n)
+        return t.value
+
+    def advance(self):
+        na = C.c_int32()
+        _check(lib().smg_server_advance(self.h, C.byref(na)), "smg_server_advance")
+        return na.value
+
+    def finished(self):
+        buf = (SmgServerJob * 256)()
+        out = []
+        while True:
+            k = C.c_int32()
+            _check(lib().smg_server_finished(self.h, buf, 256, C.byref(k)), "smg_server_finished")
+            out += [{f: getattr(buf[i], f) for f, _ in SmgServerJob._fields_} for i in range(k.value)]
+            if k.value < 256:
+                return out
+
+    def result(self, ticket):
+        n = self.n_of.pop(ticket)
+        x = np.zeros((n, 3)); t = np.zeros(n, np.int32); hh = np.zeros(n)
+        st = SmgStats()
+        _check(lib().smg_server_result(self.h, int(ticket), _ptr(x), _ptr(t), _ptr(hh), C.byref(st)),
+               "smg_server_result")
+        return {"x": x, "type": t, "h": hh, "stats": st.as_dict()}
+
+    def usage(self):
+        u, tot, na = C.c_int64(), C.c_int64(), C.c_int32()
+        _check(lib().smg_server_usage(self.h, C.byref(u), C.byref(tot), C.byref(na)), "smg_server_usage")
+        return {"used_slots": u.value, "total_slots": tot.value, "n_active": na.value}
 
 
 def union_job(dim, n_target, rng, seed, max_pkg=None):

@@ -29,7 +29,7 @@ def test_host_entry_points(smg_mod):
     assert abs(cfg.angle_thres - 0.4 * 3.1415926) < 1e-15
     cfg2 = smg_mod.default_config(2)
     assert cfg2.cell_ratio == 2.0 and cfg2.sym[2] == 0
-    assert smg_mod.lib().smg_abi_version() == 1
+    assert smg_mod.lib().smg_abi_version() == 2
 
 
 def test_bad_arguments_are_rejected(smg_mod):
@@ -58,6 +58,37 @@ def test_no_cpu_fallback_without_gpu(smg_mod, c1):
         "import sys; sys.path.insert(0, %r); import smg; cfg, sdf = smg.config_c1(num_pkg=8, dx=0.05); "
         "L = smg.make_levelset(cfg, sdf); j = smg.Job(cfg, L)\n"
         "try:\n    j.seed(50)\nexcept smg.SmgError as e:\n    print('RC', e.rc)\nelse:\n    print('RC', 0)\n"
+    ) % os.path.join(ROOT, "sph-mesh-generation_amd")
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1", CUDA_VISIBLE_DEVICES="-1")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert "RC -2" in out.stdout, out.stdout + out.stderr
+
+
+def test_server_sizing_is_host_only(smg_mod):
+    """Pool sizing needs no GPU: a job's slots are n rounded up to the 256-slot tile,
+    its level-set cells (4 num_pkg + 2)^dim, its table at least one entry per finest
+    cell plus the pad cell and the scan sentinel; the footprint grows with every
+    capacity."""
+    cfg, _ = smg_mod.config_c1(num_pkg=8, dx=0.05)
+    slots, table, cells = smg_mod.job_size(cfg, 1000)
+    assert slots == 1024
+    assert cells == (4 * 8 + 2) ** 2
+    assert table >= 2 + 1
+    base = smg_mod.server_config(3, 8, 1 << 16, 1 << 20, 1 << 18)
+    b0 = smg_mod.server_footprint(base)
+    for field in ("max_slots", "max_table", "max_lset_cells", "max_jobs", "max_neighbors"):
+        c = smg_mod.SmgServerConfig.from_buffer_copy(base)
+        setattr(c, field, 2 * getattr(c, field) if getattr(c, field) else 128)
+        assert smg_mod.server_footprint(c) > b0, field
+
+
+def test_server_without_gpu_fails_loudly(smg_mod):
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path.insert(0, %r); import smg\n"
+        "try:\n    smg.Server(smg.server_config(2, 4, 4096, 1 << 16, 1 << 16))\n"
+        "except smg.SmgError as e:\n    print('RC', e.rc)\nelse:\n    print('RC', 0)\n"
     ) % os.path.join(ROOT, "sph-mesh-generation_amd")
     env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1", CUDA_VISIBLE_DEVICES="-1")
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)

@@ -79,6 +79,10 @@ def test_single_step_parity(smg_mod, orc_stored, c1, case):
         dg = g1["v"] * st["dt2"]
         do = ref.v * ref.dt
         assert np.abs(dg - do).max() <= 1e-5 * np.abs(do).max()
+        # FP64 reductions over the FP32 velocities: mean / max of |v| dt2 / dx, max |v| dt2
+        assert st["residual"] == pytest.approx(d["residual"], rel=1e-4)
+        assert st["res_max"] == pytest.approx(d["res_max"], rel=1e-4)
+        assert st["max_disp"] == pytest.approx(d["max_disp"], rel=1e-4)
 
 
 def test_neighbors_match_brute_force(smg_mod, orc):
@@ -125,3 +129,45 @@ def test_spacing_statistics_c1_500_steps(smg_mod, orc, c1):
     assert sg["min"] == pytest.approx(so["min"], rel=5e-2)
     # both runs relaxed: the spacing is far tighter than the seeded random state
     assert so["std"] < 0.2 * so["mean"]
+
+
+@pytest.mark.parametrize("case", ["2d", "3d"])
+def test_spilled_lists_equal_inline_lists(smg_mod, orc, c1, case):
+    """Lists longer than the inline capacity continue in the spill pool; a run with
+    almost everything spilled equals a run with everything inline, bit for bit."""
+    cfg, lset = c1 if case == "2d" else _c3d(smg_mod)
+    n = int(lset["stats"].num_particle)
+    outs = []
+    # seeded 3D lists run ~30 long (up to ~100): keep the mean spill within the pool
+    K = 3 if case == "2d" else 32
+    for K in (0, K):
+        c = smg_mod.SmgConfig.from_buffer_copy(cfg)
+        c.max_neighbors = K
+        job = smg_mod.Job(c, lset)
+        job.seed(n)
+        job.step(6)
+        outs.append((job.state(), job.neighbors(), job.stats()))
+        job.close()
+    (g0, nb0, s0), (g1, nb1, s1) = outs
+    for a, b in zip(nb0, nb1):
+        assert np.array_equal(a, b)
+    for key in ("x", "v", "h", "type"):
+        assert np.array_equal(g0[key], g1[key]), key
+    assert s0["residual"] == s1["residual"] and s0["dt2"] == s1["dt2"]
+    assert s1["max_neighbors_seen"] > K   # the spill path did run
+    bo, bi, bc = orc.neighbors(cfg, g1["x"], g1["h"], brute=True)
+    assert np.array_equal(nb1[0], bo) and np.array_equal(nb1[1], bi) and np.array_equal(nb1[2], bc)
+
+
+def test_spill_pool_exhaustion_is_an_error(smg_mod):
+    """One inline entry and a pool of 4 per slot cannot hold 3D lists of ~30: the step
+    reports SMG_ERR_CAPACITY instead of silently truncating."""
+    cfg, lset = _c3d(smg_mod)
+    c = smg_mod.SmgConfig.from_buffer_copy(cfg)
+    c.max_neighbors = 1
+    c.spill_per_slot = 4
+    job = smg_mod.Job(c, lset)
+    job.seed(int(lset["stats"].num_particle))
+    with pytest.raises(smg_mod.SmgError) as e:
+        job.step(2)
+    assert e.value.rc == -3 and "spill pool" in str(e.value)
