@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define SMG_ABI_VERSION 2
+#define SMG_ABI_VERSION 3
 
 enum {
   SMG_OK = 0,
@@ -61,9 +61,15 @@ typedef struct smg_config {
   double lin_band;           /* linear profile band width in phi              */
   uint64_t seed;             /* counter-based RNG key (replaces rand())       */
   int32_t remap_max_attempts;/* bound on the re-seeding rejection loop        */
-  int32_t max_neighbors;     /* inline list entries per particle, 0 = 64;     
+  int32_t max_neighbors;     /* inline list entries per particle, 0 = 128;    
                                 longer lists spill to a shared pool          */
   int32_t spill_per_slot;    /* spill pool entries per particle slot, 0 = 32  */
+  int32_t reserved0;
+  double skin;               /* neighbour lists cover (1 + skin) * cut_off * h
+                                and are rebuilt (with the cell sort) only when
+                                some particle's drift since the last build,
+                                |dx| + cut_off |dh|, reaches skin*cut_off*h/2;
+                                0 = 0.15, < 0 = rebuild every step            */
 } smg_config;
 
 /* Level-set statistics from smg_lset_build (Levelset::Calculate_total_volume_mass,
@@ -95,6 +101,7 @@ typedef struct smg_stats {
                          /* convergence measure; summed in 2^-40 fixed point) */
   double max_disp;       /* max_i |v_i| dt2 of the last step                    */
   double res_max;        /* max_i |v_i| dt2 / dx_i of the last step             */
+  int64_t n_rebuild;     /* neighbour-list (and cell-sort) builds so far        */
 } smg_stats;
 
 typedef struct smg_job smg_job;
@@ -208,7 +215,7 @@ typedef struct smg_server_config {
   int64_t max_slots;         /* particle slots (a job takes n rounded up to 256)      */
   int64_t max_table;         /* cell-table entries (a job takes its cell-list tables) */
   int64_t max_lset_cells;    /* level-set cells (a job takes (4 num_pkg + 2)^dim)     */
-  int32_t max_neighbors;     /* inline list entries per slot, 0 = 64                  */
+  int32_t max_neighbors;     /* inline list entries per slot, 0 = 128                 */
   int32_t spill_per_slot;    /* spill pool entries per slot, 0 = 32                   */
   int32_t chunk_steps;       /* steps between admissions / convergence checks (> 0)  */
   int32_t min_steps;         /* no retirement by convergence before this many steps  */

@@ -15,6 +15,7 @@ constexpr int kMaxLevels = 8;    // cell-list levels (cell size c0 / 2^L)
 constexpr int kJBits = 26;       // neighbour entry = slot | (mirror code << 26)
 constexpr uint32_t kJMask = (1u << kJBits) - 1u;
 constexpr int kScanItems = 4096; // table entries per scan workgroup (256 threads x 16)
+constexpr int kMaskWords = 2;    // stage-2 entry masks cover the first 128 list entries
 
 // one level-set cell: phi (FP64, used for projection), raw normal, interface tag
 struct alignas(8) LsCell {
@@ -38,6 +39,9 @@ struct JobParams {
   float wl_h[3], wl_l[3], wh_h[3], wh_l[3];   // walls as float pairs
   float cos_thres, kdx;                       // cos(angle_thres), cut_off/cell_ratio
   float pf_tol;                               // FP32 prefilter bound on |dr| error
+  float skin_lim;                             // rebuild when max drift / h_b reaches this
+  double cut_v;                               // list radius factor (1 + skin) * cut_off
+  int64_t spill_base, spill_cap;              // this job's part of the spill pool
   uint64_t seed;
   int64_t lset_off;                           // first cell of this job's halo grid
   int64_t tbl_base, tbl_size;                 // cell-table segment (last entry = pad cell)
@@ -59,8 +63,13 @@ struct JobRed {
   unsigned long long rsum[2];      // sum_i |v_i| dt2 / dx_i in units of 2^-40
   unsigned long long rcum;         // rsum summed over all steps so far (windowed means)
   int64_t step;                    // steps taken by this job (re-seeding RNG stream)
+  unsigned long long drift;        // max_i (|x_i - x_b| + cut_off |h_i - h_b|) / h_b this step
+  unsigned long long spill_used;   // entries of the job's spill part handed out
+  int64_t nrebuild;                // list builds so far
   int32_t nremap[2];
   int32_t fail, overflow, maxnb, nsurf;
+  int32_t rebuild;                 // this step re-sorts and rebuilds the lists
+  int32_t built;                   // the lists match the current slot order
 };
 
 // one buffer set of the fields that follow the particles through the sort
@@ -90,19 +99,22 @@ struct Work {
   int32_t* counts;
   int32_t* starts;
   int32_t* bsum;
-  // neighbour-read records, one 16-byte load each in the pair passes:
-  // rec0 = (xh.x, xh.y, xh.z, h), rec1 = (xl.x, xl.y, xl.z, type bits), rec2 = (a1.xyz, nu)
-  float4* rec0;
-  float4* rec1;
-  float4* rec2;
-  // neighbour lists: the first K entries of a slot inline at [k * cap_total + slot];
-  // entries K.. of a slot with more neighbours in the spill pool at spill[nbo[slot] + k - K]
+  // neighbour-read record of a slot, 64 bytes (one half cache line) at rec[4 slot]:
+  // [0] = (xh.x, xh.y, xh.z, h), [1] = (xl.x, xl.y, xl.z, type bits),
+  // [2] = (a1.xyz, nu of the previous step), [3] = (normal.xyz, 0)
+  float4* rec;
+  // candidate lists (all pairs within (1 + skin) cut_off h at the last build): the
+  // first K entries of a slot inline at [k * cap_total + slot]; entries K.. of a
+  // longer list in the job's part of the spill pool at spill[nbo[slot] + k - K]
   int32_t* nbl;
-  int32_t* nbc;     // neighbour count per slot (all entries, inline + spilled)
+  int32_t* nbc;     // candidate count per slot (inline + spilled)
   int32_t* nbo;     // spill offset per slot (valid when nbc > K)
   int32_t* spill;   // spill pool, spill_cap entries
   int32_t* ovq;     // slots whose lists spill, appended by the list pass
-  unsigned long long* ctr;   // [0] = ovq length, [1] = spill entries handed out
+  int32_t* ntrue;   // neighbours within the reference's cutoff (the force passes' count)
+  uint64_t* nbm;    // stage-2 entry masks from the stage-1 pass, [w * cap_total + slot]
+  float4* recb;     // position (hi parts) and h at the last build: (x, y, z, h)
+  unsigned long long* ctr;   // [0] = ovq length
   int64_t spill_cap;
   int64_t tile0;   // first tile of a launch (launches over one job's tiles start there)
   int64_t cap_total;

@@ -103,6 +103,10 @@ int make_job_params(const JobInput& in, int D, JobParams& J) {
   J.lin_dx0 = c.lin_dx0; J.lin_dx1 = c.lin_dx1; J.lin_band = c.lin_band;
   J.cos_thres = float(std::cos(c.angle_thres));
   J.kdx = float(c.cut_off / c.cell_ratio);
+  // candidate lists cover (1 + skin) cut_off h; skin < 0 rebuilds them every step
+  const double skin = c.skin > 0. ? c.skin : (c.skin < 0. ? 0. : 0.15);
+  J.cut_v = (1.0 + skin) * c.cut_off;
+  J.skin_lim = c.skin < 0. ? -1.f : float(0.5 * skin * c.cut_off);
   {
     // bound on the FP32 error of a relative position formed from the hi parts
     double X = 0.;
@@ -119,13 +123,13 @@ int make_job_params(const JobInput& in, int D, JobParams& J) {
   const double hmax = prof.dx_max() * c.cell_ratio / c.cut_off;
   const double hmin = prof.dx_min() * c.cell_ratio / c.cut_off;
   if (!(hmin > 0.)) { set_error("setup: spacing must be positive"); return SMG_ERR_ARG; }
-  const double R0 = hmax * c.cut_off * (1. + 1.e-9) + 1.e-9;
+  const double R0 = hmax * J.cut_v * (1. + 1.e-9) + 1.e-9;
   int64_t nbox = int64_t(std::floor(dmin / R0));
   if (nbox < 1) nbox = 1;
   J.c0 = dmin / double(nbox);
   const int max_bits = D == 2 ? 15 : 10;
   int nlev = 1;
-  while (nlev < kMaxLevels && std::ldexp(J.c0, -nlev) >= hmin * c.cut_off * (1. + 1.e-9) + 2.e-10) nlev++;
+  while (nlev < kMaxLevels && std::ldexp(J.c0, -nlev) >= hmin * J.cut_v * (1. + 1.e-9) + 2.e-10) nlev++;
   int bits0 = 0;
   for (int a = 0; a < D; a++) bits0 = std::max(bits0, ceil_log2(int64_t(std::ceil(J.domain[a] / J.c0))));
   while (nlev > 1 && bits0 + nlev - 1 > max_bits) nlev--;
@@ -176,9 +180,7 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
     W_.a2[a] = alloc_<float>(cap_total_);
   }
   W_.sum = alloc_<float>(cap_total_);
-  W_.rec0 = alloc_<float4>(cap_total_);
-  W_.rec1 = alloc_<float4>(cap_total_);
-  W_.rec2 = alloc_<float4>(cap_total_);
+  W_.rec = alloc_<float4>(4 * cap_total_);
   W_.key = alloc_<uint32_t>(cap_total_);
   W_.skey = alloc_<uint32_t>(cap_total_);
   W_.perm = alloc_<int32_t>(cap_total_);
@@ -190,10 +192,16 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   W_.nbc = alloc_<int32_t>(cap_total_);
   W_.nbo = alloc_<int32_t>(cap_total_);
   W_.ovq = alloc_<int32_t>(cap_total_);
-  // spill pool: by default 32 entries per slot on average beyond the inline K (the
-  // unrelaxed random start and strong h gradients are where lists run long)
-  const int64_t sps = spill_per_slot > 0 ? spill_per_slot : 32;
-  W_.spill_cap = std::min<int64_t>(std::max<int64_t>(int64_t(1) << 10, sps * cap_total_), (int64_t(1) << 31) - 1);
+  W_.ntrue = alloc_<int32_t>(cap_total_);
+  W_.nbm = alloc_<uint64_t>(int64_t(kMaskWords) * cap_total_);
+  W_.recb = alloc_<float4>(cap_total_);
+  // spill pool: by default 32 entries per slot beyond the inline K (the unrelaxed
+  // random start and strong h gradients are where lists run long); a job owns the
+  // part [p0 * sps, (p0 + cap) * sps) of it
+  sps_ = spill_per_slot > 0 ? spill_per_slot : 32;
+  sps_ = int(std::min<int64_t>(sps_, ((int64_t(1) << 31) - 1) / std::max<int64_t>(cap_total_, 1)));
+  if (sps_ < 1) { set_error("setup: too many slots for the spill pool"); return SMG_ERR_CAPACITY; }
+  W_.spill_cap = int64_t(sps_) * cap_total_;
   W_.spill = alloc_<int32_t>(W_.spill_cap);
   W_.ctr = alloc_<unsigned long long>(2);
   if (alloc_failed_) { set_error("setup: device allocation failed (hipMalloc)"); return SMG_ERR_HIP; }
@@ -206,16 +214,19 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   W_.K = K_;
   W_.tile0 = 0;
   HIP_TRY(hipMemsetAsync(d_red_, 0, sizeof(JobRed) * njobs, st_));
+  HIP_TRY(hipMemsetAsync(d_jp_, 0, sizeof(JobParams) * njobs, st_));   // n = 0: no job
   HIP_TRY(hipMemsetAsync(d_tile_job_, 0xFF, sizeof(int32_t) * (slots / kTile), st_));   // -1: no job
   HIP_TRY(hipMemsetAsync(W_.counts, 0, sizeof(int32_t) * n_tbl_, st_));
   HIP_TRY(hipMemsetAsync(W_.starts, 0, sizeof(int32_t) * (n_tbl_ + 1), st_));
   HIP_TRY(hipMemsetAsync(W_.nbc, 0, sizeof(int32_t) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.ctr, 0, 2 * sizeof(unsigned long long), st_));
   HIP_TRY(hipMemsetAsync(W_.nbo, 0, sizeof(int32_t) * cap_total_, st_));
+  HIP_TRY(hipMemsetAsync(W_.ntrue, 0, sizeof(int32_t) * cap_total_, st_));
+  HIP_TRY(hipMemsetAsync(W_.recb, 0, sizeof(float4) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.nu_a, 0, sizeof(float) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.nu_b, 0, sizeof(float) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.sum, 0, sizeof(float) * cap_total_, st_));
-  HIP_TRY(hipMemsetAsync(W_.rec2, 0, sizeof(float4) * cap_total_, st_));
+  HIP_TRY(hipMemsetAsync(W_.rec, 0, 4 * sizeof(float4) * cap_total_, st_));
   for (int a = 0; a < 3; a++) {
     HIP_TRY(hipMemsetAsync(W_.a1[a], 0, sizeof(float) * cap_total_, st_));
     HIP_TRY(hipMemsetAsync(W_.a2[a], 0, sizeof(float) * cap_total_, st_));
@@ -226,6 +237,8 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
 
 // level-set cells, parameters and tile ownership of job j (ranges already set in jp_[j])
 int Engine::upload_job_(int j, const JobInput& in) {
+  jp_[j].spill_base = jp_[j].p0 * sps_;
+  jp_[j].spill_cap = jp_[j].cap * sps_;
   const JobParams& J = jp_[j];
   const int64_t m = job_lset_cells(J);
   std::vector<LsCell> hls(m);
@@ -250,7 +263,7 @@ int Engine::upload_job_(int j, const JobInput& in) {
 int Engine::begin_setup_(int D, int max_neighbors, hipStream_t stream) {
   D_ = D;
   if (D_ != 2 && D_ != 3) { set_error("setup: dim must be 2 or 3"); return SMG_ERR_ARG; }
-  K_ = max_neighbors > 0 ? max_neighbors : 64;
+  K_ = max_neighbors > 0 ? max_neighbors : 128;
   if (stream) st_ = stream;
   else {
     HIP_TRY(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
@@ -379,7 +392,9 @@ int Engine::admit(const JobInput& in, int* job) {
   SMG_TRY(upload_job_(j, in));
   update_extent_();
   const JobRed zero{};
-  HIP_TRY(hipMemcpyAsync(d_red_ + j, &zero, sizeof(JobRed), hipMemcpyHostToDevice, st_));
+  JobRed sorting{};
+  sorting.rebuild = 1;
+  HIP_TRY(hipMemcpyAsync(d_red_ + j, &sorting, sizeof(JobRed), hipMemcpyHostToDevice, st_));
   HIP_TRY(hipMemsetAsync(W_.counts + tb, 0, sizeof(int32_t) * (J.tbl_size + 1), st_));
   Work Ws = W_;
   Ws.tile0 = t0;
@@ -389,10 +404,10 @@ int Engine::admit(const JobInput& in, int* job) {
   launch_map(st_, D_, nt, Ws, A_, kModePrerun, 0);
   SMG_TRY(sort_job_(j));
   const size_t fb = sizeof(float) * J.cap;
+  HIP_TRY(hipMemsetAsync(W_.recb + J.p0, 0, sizeof(float4) * J.cap, st_));
   HIP_TRY(hipMemsetAsync(W_.nu_a + J.p0, 0, fb, st_));
   HIP_TRY(hipMemsetAsync(W_.nu_b + J.p0, 0, fb, st_));
   HIP_TRY(hipMemsetAsync(W_.sum + J.p0, 0, fb, st_));
-  HIP_TRY(hipMemsetAsync(W_.rec2 + J.p0, 0, sizeof(float4) * J.cap, st_));
   for (int a = 0; a < 3; a++) {
     HIP_TRY(hipMemsetAsync(W_.a1[a] + J.p0, 0, fb, st_));
     HIP_TRY(hipMemsetAsync(W_.a2[a] + J.p0, 0, fb, st_));
@@ -412,30 +427,23 @@ int Engine::admit(const JobInput& in, int* job) {
   return SMG_OK;
 }
 
-// counting sort of one job's slots inside its own table segment
+// counting sort of one job's slots inside its own table segment (its reductions
+// carry rebuild = 1)
 int Engine::sort_job_(int j) {
   const JobParams& J = jp_[j];
   const int64_t t0 = J.p0 / kTile, nt = J.cap / kTile;
+  Work Ws = W_;
+  Ws.tile0 = t0;
+  launch_key(st_, D_, nt, Ws, A_);
   Work Wt = W_;
   Wt.counts = W_.counts + J.tbl_base;
   Wt.starts = W_.starts + J.tbl_base;
   Wt.n_tbl = J.tbl_size;
-  launch_scan(st_, Wt);
-  Work Ws = W_;
-  Ws.tile0 = t0;
+  launch_scan(st_, Wt, false);
   launch_scatter(st_, nt, Ws);
   launch_segsort(st_, nt, Ws, A_);
   launch_gather(st_, D_, nt, Ws, A_, B_);
-  // the job's new order is in B; the other jobs stay current in A, so copy back
-  const int64_t p0 = J.p0, cap = J.cap;
-  for (int a = 0; a < D_; a++) {
-    HIP_TRY(hipMemcpyAsync(A_.xh[a] + p0, B_.xh[a] + p0, sizeof(float) * cap, hipMemcpyDeviceToDevice, st_));
-    HIP_TRY(hipMemcpyAsync(A_.xl[a] + p0, B_.xl[a] + p0, sizeof(float) * cap, hipMemcpyDeviceToDevice, st_));
-    HIP_TRY(hipMemcpyAsync(A_.nrm[a] + p0, B_.nrm[a] + p0, sizeof(float) * cap, hipMemcpyDeviceToDevice, st_));
-  }
-  HIP_TRY(hipMemcpyAsync(A_.h + p0, B_.h + p0, sizeof(double) * cap, hipMemcpyDeviceToDevice, st_));
-  HIP_TRY(hipMemcpyAsync(A_.type + p0, B_.type + p0, sizeof(int32_t) * cap, hipMemcpyDeviceToDevice, st_));
-  HIP_TRY(hipMemcpyAsync(A_.id + p0, B_.id + p0, sizeof(int32_t) * cap, hipMemcpyDeviceToDevice, st_));
+  launch_copyback(st_, D_, nt, Ws, A_, B_);
   HIP_TRY(hipGetLastError());
   return SMG_OK;
 }
@@ -452,6 +460,8 @@ int Engine::retire(int j) {
   ls_free_.give(J.lset_off, job_lset_cells(J));
   active_[j] = 0;
   jp_[j].n = 0;
+  HIP_TRY(hipMemcpyAsync(d_jp_ + j, &jp_[j], sizeof(JobParams), hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipStreamSynchronize(st_));
   update_extent_();
   return SMG_OK;
 }
@@ -488,20 +498,16 @@ void Engine::end_() {
   cur_a_ = nullptr;
 }
 
+// re-sort every job (set-up passes): forced rebuild flags, keys, scan, counting
+// sort into B and back into A
 int Engine::sort_particles() {
-  begin_("k_scan");
-  launch_scan(st_, W_);
-  end_();
-  begin_("k_scatter");
+  launch_decide(st_, W_, int(jp_.size()), 1);
+  launch_key(st_, D_, ntiles_, W_, A_);
+  launch_scan(st_, W_, false);
   launch_scatter(st_, ntiles_, W_);
-  end_();
-  begin_("k_segsort");
   launch_segsort(st_, ntiles_, W_, A_);
-  end_();
-  begin_("k_gather");
   launch_gather(st_, D_, ntiles_, W_, A_, B_);
-  end_();
-  std::swap(A_, B_);
+  launch_copyback(st_, D_, ntiles_, W_, A_, B_);
   HIP_TRY(hipGetLastError());
   return SMG_OK;
 }
@@ -576,7 +582,6 @@ int Engine::prerun() {
     HIP_TRY(hipMemsetAsync(W_.a1[a], 0, sizeof(float) * cap_total_, st_));
     HIP_TRY(hipMemsetAsync(W_.a2[a], 0, sizeof(float) * cap_total_, st_));
   }
-  HIP_TRY(hipMemsetAsync(W_.rec2, 0, sizeof(float4) * cap_total_, st_));
   HIP_TRY(hipStreamSynchronize(st_));
   SMG_TRY(download_red_());
   // the dt slots stay 0 so the first step moves nothing (v = 0 after Pre_run)
@@ -597,7 +602,29 @@ int Engine::step(int64_t nsteps) {
     begin_("k_map");
     launch_map(st_, D_, ntiles_, W_, A_, kModeStep, par);
     end_();
-    SMG_TRY(sort_particles());
+    // jobs whose drift reached the skin re-sort and rebuild their lists; the other
+    // jobs' tiles return at once from the passes up to k_nb_spill
+    begin_("k_decide");
+    launch_decide(st_, W_, int(jp_.size()), 0);
+    end_();
+    begin_("k_key");
+    launch_key(st_, D_, ntiles_, W_, A_);
+    end_();
+    begin_("k_scan");
+    launch_scan(st_, W_, true);
+    end_();
+    begin_("k_scatter");
+    launch_scatter(st_, ntiles_, W_);
+    end_();
+    begin_("k_segsort");
+    launch_segsort(st_, ntiles_, W_, A_);
+    end_();
+    begin_("k_gather");
+    launch_gather(st_, D_, ntiles_, W_, A_, B_);
+    end_();
+    begin_("k_copyback");
+    launch_copyback(st_, D_, ntiles_, W_, A_, B_);
+    end_();
     begin_("k_nb");
     launch_nb(st_, D_, ntiles_, W_, A_);
     end_();
@@ -770,6 +797,8 @@ int Engine::get_pair_terms(int job, float* sum, float* a1, float* a2) {
   return SMG_OK;
 }
 
+// The neighbour sets of the last step: the job's candidate lists with the device's
+// own per-entry predicate applied (k_nb_flags), mapped to particle ids.
 int Engine::get_neighbors(int job, int64_t* offs, int32_t* ids, int32_t* codes, int64_t capacity) {
   if (step_ == 0) { set_error("get_neighbors: no step taken yet"); return SMG_ERR_STATE; }
   const JobParams& J = jp_[job];
@@ -779,20 +808,47 @@ int Engine::get_neighbors(int job, int64_t* offs, int32_t* ids, int32_t* codes, 
   HIP_TRY(hipMemcpy(id.data(), A_.id, sizeof(int32_t) * cap_total_, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(cnt.data(), W_.nbc + J.p0, sizeof(int32_t) * cap, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(spo.data(), W_.nbo + J.p0, sizeof(int32_t) * cap, hipMemcpyDeviceToHost));
-  unsigned long long used[2] = {0ull, 0ull};
-  HIP_TRY(hipMemcpy(used, W_.ctr, sizeof(used), hipMemcpyDeviceToHost));
-  const int64_t nsp = std::min<int64_t>(int64_t(used[1]), W_.spill_cap);
-  std::vector<int32_t> spl(std::max<int64_t>(nsp, 1));
-  if (nsp > 0) HIP_TRY(hipMemcpy(spl.data(), W_.spill, sizeof(int32_t) * nsp, hipMemcpyDeviceToHost));
+  // candidate entries in slot order (CSR over the job's slots)
+  std::vector<int64_t> co(cap + 1, 0);
+  for (int64_t s = 0; s < cap; s++) co[s + 1] = co[s] + (s < n ? cnt[s] : 0);
+  const int64_t m = co[cap];
+  std::vector<uint8_t> fl(std::max<int64_t>(m, 1));
+  {
+    int64_t* d_off = nullptr;
+    uint8_t* d_fl = nullptr;
+    HIP_TRY(hipMalloc(&d_off, sizeof(int64_t) * (cap + 1)));
+    if (hipMalloc(&d_fl, std::max<int64_t>(m, 1)) != hipSuccess) {
+      (void)hipFree(d_off);
+      set_error("get_neighbors: hipMalloc failed");
+      return SMG_ERR_HIP;
+    }
+    hipError_t e = hipMemcpy(d_off, co.data(), sizeof(int64_t) * (cap + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      Work Ws = W_;
+      Ws.tile0 = J.p0 / kTile;
+      launch_nb_flags(st_, D_, cap / kTile, Ws, A_, d_off, d_fl);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st_);
+    if (e == hipSuccess && m > 0) e = hipMemcpy(fl.data(), d_fl, m, hipMemcpyDeviceToHost);
+    (void)hipFree(d_off);
+    (void)hipFree(d_fl);
+    if (e != hipSuccess) { set_error(std::string("get_neighbors: ") + hipGetErrorString(e)); return SMG_ERR_HIP; }
+  }
+  std::vector<int32_t> spl(std::max<int64_t>(J.spill_cap, 1));
+  if (J.spill_cap > 0)
+    HIP_TRY(hipMemcpy(spl.data(), W_.spill + J.spill_base, sizeof(int32_t) * J.spill_cap, hipMemcpyDeviceToHost));
   std::vector<int32_t> ent(int64_t(K_) * cap);
   for (int k = 0; k < K_; k++)
     HIP_TRY(hipMemcpy(ent.data() + int64_t(k) * cap, W_.nbl + int64_t(k) * cap_total_ + J.p0, sizeof(int32_t) * cap,
                       hipMemcpyDeviceToHost));
-  std::vector<int64_t> count_by_id(n, 0);
   std::vector<int64_t> slot_of(n, -1);
+  for (int64_t s = 0; s < n; s++) slot_of[id[J.p0 + s]] = s;
+  std::vector<int64_t> count_by_id(n, 0);
   for (int64_t s = 0; s < n; s++) {
-    count_by_id[id[J.p0 + s]] = cnt[s];
-    slot_of[id[J.p0 + s]] = s;
+    int64_t c = 0;
+    for (int64_t k = co[s]; k < co[s + 1]; k++) c += fl[k];
+    count_by_id[id[J.p0 + s]] = c;
   }
   offs[0] = 0;
   for (int64_t i = 0; i < n; i++) offs[i + 1] = offs[i] + count_by_id[i];
@@ -803,11 +859,12 @@ int Engine::get_neighbors(int job, int64_t* offs, int32_t* ids, int32_t* codes, 
     const int64_t s = slot_of[i];
     tmp.clear();
     for (int k = 0; k < cnt[s]; k++) {
+      if (!fl[co[s] + k]) continue;
       uint32_t e;
       if (k < K_) e = uint32_t(ent[int64_t(k) * cap + s]);
       else {
-        const int64_t at = int64_t(spo[s]) + (k - K_);
-        if (at >= nsp) { set_error("get_neighbors: spill entry out of range"); return SMG_ERR_STATE; }
+        const int64_t at = int64_t(spo[s]) - J.spill_base + (k - K_);
+        if (at < 0 || at >= J.spill_cap) { set_error("get_neighbors: spill entry out of range"); return SMG_ERR_STATE; }
         e = uint32_t(spl[at]);
       }
       const int64_t q = e & kJMask;
@@ -851,8 +908,9 @@ int Engine::get_stats(int job, smg_stats* st) {
     st->res_max = bits_to_double(R.res[par]);
     st->max_disp = bits_to_double(R.maxdisp[par]);
     std::vector<int32_t> cnt(J.cap);
-    HIP_TRY(hipMemcpy(cnt.data(), W_.nbc + J.p0, sizeof(int32_t) * J.cap, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cnt.data(), W_.ntrue + J.p0, sizeof(int32_t) * J.cap, hipMemcpyDeviceToHost));
     for (int64_t s = 0; s < J.n; s++) st->n_neighbor_entries += cnt[s];
+    st->n_rebuild = R.nrebuild;
   }
   return SMG_OK;
 }

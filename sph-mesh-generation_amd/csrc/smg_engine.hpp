@@ -126,6 +126,7 @@ class Engine {
   std::vector<int32_t> tile_job_h_;
   int64_t ntiles_ = 0, cap_total_ = 0, n_tbl_ = 0, n_ls_ = 0;
   int K_ = 64;
+  int sps_ = 32;   // spill entries per slot
   int64_t step_ = 0;   // last completed step index (0 = pre-run)
   bool prerun_done_ = false;
 

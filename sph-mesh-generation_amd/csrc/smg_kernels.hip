@@ -201,8 +201,9 @@ __device__ __forceinline__ double h_of_phi(const JobParams& J, double phi) {
   return get_scale(J, phi) * J.cell_ratio / J.cut_off;
 }
 
+// cell-list level of a particle: the finest level whose cells hold its list radius
 __device__ __forceinline__ int level_of(const JobParams& J, double h) {
-  const double r = h * J.cut_off + 2.e-10;
+  const double r = h * J.cut_v + 2.e-10;
   int l = 0;
   while (l + 1 < J.nlev && J.clen[l + 1] >= r) l++;
   return l;
@@ -305,13 +306,15 @@ __device__ __forceinline__ void rel_pos(const JobParams& J, const float4& qh, co
 
 __global__ void k_reset(JobRed* red, int njobs, int par, unsigned long long* ctr) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j == 0) {   // neighbour spill queue and pool of this step
+  if (j == 0) {   // spill queue and the any-job-rebuilds flag of this step
     ctr[0] = 0ull;
     ctr[1] = 0ull;
   }
   if (j >= njobs) return;
   JobRed& R = red[j];
   R.step += 1;   // re-seeding RNG stream of this step (graph replays advance it too)
+  R.drift = 0ull;
+  R.rebuild = 0;
   R.dt1[par] = 0x7FF0000000000000ull;
   R.dt2[par] = 0x7FF0000000000000ull;
   R.maxv[par] = 0ull;
@@ -396,10 +399,9 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
   const JobParams& J = W.jp[j];
   const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const int64_t loc = slot - J.p0;
-  double res = 0., md = 0.;
+  double res = 0., md = 0., drift = 0.;
   unsigned long long rq = 0ull;
   int nre = 0, bad = 0, nsurf = 0;
-  uint32_t key;
   if (loc < J.n) {
     double x[3] = {0., 0., 0.};
 #pragma unroll
@@ -461,7 +463,8 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
           x[a] = x[a] + nd * dx;
         }
       }
-      ls_sample<D>(J, W.ls, x, phi, n);
+      // sample again where the particle moved (a REAL particle keeps x, phi and n)
+      if (type == SMG_SURFACE) ls_sample<D>(J, W.ls, x, phi, n);
       if (phi < 0. && type == SMG_REAL) {
         // Remap_this_particle_to_positive_phase (sph_mesh_generation.cpp:1885-1928)
         nre = 1;
@@ -500,15 +503,29 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
     S.h[slot] = h;
     S.type[slot] = type;
     nsurf = type == SMG_SURFACE;
-    const int L = level_of(J, h);
-    int c[3] = {0, 0, 0};
-    cell_at<D>(J, L, x, c);
-    key = uint32_t(J.tbl_base + J.lev_off[L]) + morton<D>(c);
-  } else {
-    key = uint32_t(J.tbl_base + J.tbl_size - 1);
+    float hp[3] = {0.f, 0.f, 0.f}, lp[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      hp[a] = float(x[a]);
+      lp[a] = float(x[a] - double(hp[a]));
+    }
+    const float4 r0 = make_float4(hp[0], hp[1], hp[2], float(h));
+    const float4 r1 = make_float4(lp[0], lp[1], lp[2], __int_as_float(type));
+    W.rec[4 * int64_t(slot) + 0] = r0;
+    W.rec[4 * int64_t(slot) + 1] = r1;
+    W.rec[4 * int64_t(slot) + 3] = make_float4(float(n[0]), D > 1 ? float(n[1]) : 0.f, D > 2 ? float(n[2]) : 0.f, 0.f);
+    if (mode == kModeStep) {
+      // drift since the last list build, as a fraction of h_b (rebuild test)
+      const float4 b = W.recb[slot];
+      const double dx0 = double(r0.x) - double(b.x), dx1 = double(r0.y) - double(b.y);
+      const double dx2 = D > 2 ? double(r0.z) - double(b.z) : 0.;
+      const double e = sqrt(dx0 * dx0 + dx1 * dx1 + dx2 * dx2) + 4.0 * double(J.pf_tol) +
+                       J.cut_off * fabs(h - double(b.w));
+      double rho = e / double(b.w) * (1.0 + 1.e-5) + 1.e-9;
+      drift = (rho <= 1.e300) ? rho : 1.e300;   // also for b.w = 0 or NaN
+    }
   }
-  W.key[slot] = key;
-  atomicAdd(&W.counts[key], 1);
+  drift = block_reduce(drift, OpMax(), lds_d);
   res = block_reduce(res, OpMax(), lds_d);
   md = block_reduce(md, OpMax(), lds_d);
   rq = block_reduce(rq, OpAddU(), lds_u);
@@ -518,6 +535,7 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
   if (threadIdx.x == 0) {
     JobRed& R = W.red[j];
     if (mode == kModeStep) {   // residual of the previous step, whose motion this pass applied
+      atomicMax(&R.drift, dbits(drift));
       atomicMax(&R.res[par ^ 1], dbits(res));
       atomicMax(&R.maxdisp[par ^ 1], dbits(md));
       if (rq) {
@@ -532,9 +550,57 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
 }
 
 // ---------------------------------------------------------------------------
+// list rebuild decision per job: no lists yet, forced, or some particle's drift
+// reached skin * cut_off / 2 of its h at the last build
+
+__global__ void k_decide(Work W, int njobs, int force) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= njobs) return;
+  JobRed& R = W.red[j];
+  if (W.jp[j].n == 0) {   // an empty job index of a pool
+    R.rebuild = 0;
+    return;
+  }
+  const int rb = force || !R.built || bitsd(R.drift) >= double(W.jp[j].skin_lim);
+  R.rebuild = rb;
+  if (rb) {
+    R.spill_used = 0ull;
+    if (!force) R.nrebuild += 1;
+    W.ctr[1] = 1ull;   // the table scan runs this step
+  }
+}
+
+// cell key and count of every slot of a rebuilding job
+template <int D>
+__global__ __launch_bounds__(kTile) void k_key(Work W, PState S) {
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0 || !W.red[j].rebuild) return;
+  const JobParams& J = W.jp[j];
+  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  uint32_t key;
+  if (slot - J.p0 < J.n) {
+    const double h = S.h[slot];
+    double x[3] = {0., 0., 0.};
+#pragma unroll
+    for (int a = 0; a < D; a++) x[a] = double(S.xh[a][slot]) + double(S.xl[a][slot]);
+    const int L = level_of(J, h);
+    int c[3] = {0, 0, 0};
+    cell_at<D>(J, L, x, c);
+    key = uint32_t(J.tbl_base + J.lev_off[L]) + morton<D>(c);
+  } else {
+    key = uint32_t(J.tbl_base + J.tbl_size - 1);
+  }
+  W.key[slot] = key;
+  atomicAdd(&W.counts[key], 1);
+}
+
+// ---------------------------------------------------------------------------
 // cell starts: exclusive scan of counts over all job tables
 
-__global__ __launch_bounds__(kTile) void k_scan_reduce(const int32_t* __restrict__ counts, int64_t n, int32_t* bsum) {
+// gate (may be NULL): skip the scan on a step where no job rebuilds
+__global__ __launch_bounds__(kTile) void k_scan_reduce(const int32_t* __restrict__ counts, int64_t n, int32_t* bsum,
+                                                       const unsigned long long* gate) {
+  if (gate && !*gate) return;
   __shared__ int lds[4];
   const int64_t base = int64_t(blockIdx.x) * kScanItems + int64_t(threadIdx.x) * 16;
   int s = 0;
@@ -544,7 +610,8 @@ __global__ __launch_bounds__(kTile) void k_scan_reduce(const int32_t* __restrict
   if (threadIdx.x == 0) bsum[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_top(int32_t* bsum, int64_t nb) {
+__global__ __launch_bounds__(1024) void k_scan_top(int32_t* bsum, int64_t nb, const unsigned long long* gate) {
+  if (gate && !*gate) return;
   __shared__ int lds[1024];
   int carry = 0;
   for (int64_t c0 = 0; c0 < nb; c0 += 1024) {
@@ -567,7 +634,9 @@ __global__ __launch_bounds__(1024) void k_scan_top(int32_t* bsum, int64_t nb) {
 }
 
 __global__ __launch_bounds__(kTile) void k_scan_down(const int32_t* __restrict__ counts, int64_t n,
-                                                     const int32_t* __restrict__ bsum, int64_t nb, int32_t* starts) {
+                                                     const int32_t* __restrict__ bsum, int64_t nb, int32_t* starts,
+                                                     const unsigned long long* gate) {
+  if (gate && !*gate) return;
   __shared__ int lds[kTile];
   const int64_t base = int64_t(blockIdx.x) * kScanItems + int64_t(threadIdx.x) * 16;
   int v[16];
@@ -601,7 +670,7 @@ __global__ __launch_bounds__(kTile) void k_scan_down(const int32_t* __restrict__
 // slot ranges of the jobs may sit anywhere (gaps included) and still sort in place.
 __global__ __launch_bounds__(kTile) void k_scatter(Work W) {
   const int j = W.tile_job[W.tile0 + blockIdx.x];
-  if (j < 0) return;
+  if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
   const int64_t p = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const uint32_t k = W.key[p];
@@ -613,7 +682,7 @@ __global__ __launch_bounds__(kTile) void k_scatter(Work W) {
 
 __global__ __launch_bounds__(kTile) void k_segsort(Work W, PState S) {
   const int j = W.tile_job[W.tile0 + blockIdx.x];
-  if (j < 0) return;
+  if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
   const int64_t d = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const uint32_t k = W.skey[d];
@@ -635,9 +704,12 @@ __global__ __launch_bounds__(kTile) void k_segsort(Work W, PState S) {
   }
 }
 
+// new order into B (the scratch set); k_copyback then moves it into A, so A is the
+// current state at every step whichever jobs re-sorted
 template <int D>
 __global__ __launch_bounds__(kTile) void k_gather(Work W, PState A, PState B) {
-  if (W.tile_job[W.tile0 + blockIdx.x] < 0) return;
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0 || !W.red[j].rebuild) return;
   const int64_t d = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const int64_t p = W.perm[d];
 #pragma unroll
@@ -646,14 +718,36 @@ __global__ __launch_bounds__(kTile) void k_gather(Work W, PState A, PState B) {
     B.xl[a][d] = A.xl[a][p];
     B.nrm[a][d] = A.nrm[a][p];
   }
-  const double h = A.h[p];
-  const int32_t ty = A.type[p];
-  B.h[d] = h;
-  B.type[d] = ty;
+  B.h[d] = A.h[p];
+  B.type[d] = A.type[p];
   B.id[d] = A.id[p];
   W.nu_a[d] = W.nu_b[p];
-  W.rec0[d] = make_float4(A.xh[0][p], A.xh[1][p], D > 2 ? A.xh[2][p] : 0.f, float(h));
-  W.rec1[d] = make_float4(A.xl[0][p], A.xl[1][p], D > 2 ? A.xl[2][p] : 0.f, __int_as_float(ty));
+}
+
+template <int D>
+__global__ __launch_bounds__(kTile) void k_copyback(Work W, PState A, PState B) {
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0 || !W.red[j].rebuild) return;
+  const int64_t d = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  float xh[3] = {0.f, 0.f, 0.f}, xl[3] = {0.f, 0.f, 0.f}, nn[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    xh[a] = B.xh[a][d];
+    xl[a] = B.xl[a][d];
+    nn[a] = B.nrm[a][d];
+    A.xh[a][d] = xh[a];
+    A.xl[a][d] = xl[a];
+    A.nrm[a][d] = nn[a];
+  }
+  const double h = B.h[d];
+  const int32_t ty = B.type[d];
+  A.h[d] = h;
+  A.type[d] = ty;
+  A.id[d] = B.id[d];
+  W.nu_b[d] = W.nu_a[d];
+  W.rec[4 * int64_t(d) + 0] = make_float4(xh[0], xh[1], xh[2], float(h));
+  W.rec[4 * int64_t(d) + 1] = make_float4(xl[0], xl[1], xl[2], __int_as_float(ty));
+  W.rec[4 * int64_t(d) + 3] = make_float4(nn[0], nn[1], nn[2], 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -668,21 +762,22 @@ __device__ __forceinline__ void image_of(const JobParams& J, uint32_t code, cons
   }
 }
 
-// Enumerates the neighbours of one particle in a fixed order (mirror images, then
-// levels, then cells, then slots) and hands each entry (k, slot | code << 26) to emit.
-// Returns the number of neighbours.
+// Enumerates the list candidates of one particle in a fixed order (mirror images,
+// then levels, then cells, then slots): every slot (or mirror image) within
+// cut_v * max(h_i, h_q) by the FP32 distance of the hi parts, widened by that
+// distance's error bound -- a superset, so no FP64 decision is needed here.  Hands
+// each entry (k, slot | code << 26) to emit and returns the count.
 template <int D, class Emit>
 __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, const JobParams& J, int64_t slot,
                                             Emit emit) {
   int cnt = 0;
-  const float4 r0 = W.rec0[slot], r1 = W.rec1[slot];
+  const float4 r0 = W.rec[4 * int64_t(slot) + 0], r1 = W.rec[4 * int64_t(slot) + 1];
   const float xh[3] = {r0.x, r0.y, r0.z}, xl[3] = {r1.x, r1.y, r1.z};
   double xi[3] = {0., 0., 0.};
 #pragma unroll
   for (int a = 0; a < D; a++) xi[a] = double(xh[a]) + double(xl[a]);
   const double hi = S.h[slot];
-  const double ri = hi * J.cut_off + 1.e-10;
-  const float rfi = float(hi * J.cut_off);
+  const float rfi = float(hi * J.cut_v);
   const int li = level_of(J, hi);
   const int32_t soff = int32_t(J.p0) - W.starts[J.tbl_base];   // table-relative starts -> slots
   // mirror options per axis: none, low wall, high wall
@@ -696,7 +791,7 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
       if (J.box_r[a] - xi[a] < J.c0) opt[a][nopt[a]++] = 2;
     }
   }
-  const float cutf = float(J.cut_off);
+  const float cutf = float(J.cut_v);
   for (int o0 = 0; o0 < nopt[0]; o0++)
     for (int o1 = 0; o1 < nopt[1]; o1++)
       for (int o2 = 0; o2 < nopt[2]; o2++) {
@@ -727,7 +822,7 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
                 const int32_t lo = W.starts[base + int64_t(m << shift)] + soff;
                 const int32_t hi2 = W.starts[base + int64_t((m + 1) << shift)] + soff;
                 for (int32_t q = lo; q < hi2; q++) {
-                  const float4 qr = W.rec0[q];
+                  const float4 qr = W.rec[4 * int64_t(q) + 0];
                   const float qv[3] = {qr.x, qr.y, qr.z};
                   float d2 = 0.f;
 #pragma unroll
@@ -736,28 +831,8 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
                     d2 += t * t;
                   }
                   const float r = fmaxf(rfi, qr.w * cutf);
-                  const float band = J.pf_tol + 1.e-6f * r;
-                  const float rlo = r - band, rhi = r + band;
-                  bool in;
-                  if (d2 < rlo * rlo) in = true;
-                  else if (d2 > rhi * rhi) in = false;
-                  else {
-                    // near the cutoff: the reference's FP64 predicate on the stored positions
-                    const float4 ql = W.rec1[q];
-                    const double xq[3] = {double(qr.x) + double(ql.x), double(qr.y) + double(ql.y),
-                                          double(qr.z) + double(ql.z)};
-                    double g[3];
-                    image_of<D>(J, code, xq, g);
-                    double dist = 0.0;
-#pragma unroll
-                    for (int a = 0; a < D; a++) {
-                      const double dr = xi[a] - g[a];
-                      dist += dr * dr;
-                    }
-                    dist = sqrt(dist);
-                    in = dist <= ri || dist <= S.h[q] * J.cut_off + 1.e-10;
-                  }
-                  if (in) {
+                  const float rhi = r + (J.pf_tol + 1.e-6f * r);
+                  if (d2 <= rhi * rhi) {
                     emit(cnt, int32_t(uint32_t(q) | (code << kJBits)));
                     cnt++;
                   }
@@ -768,12 +843,12 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
   return cnt;
 }
 
-// neighbour lists: the first K entries inline; a slot with more joins the spill queue
+// candidate lists of the rebuilding jobs: the first K entries inline, a slot with
+// more joins the spill queue; the build position and h are kept for the drift test
 template <int D>
 __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
-  __shared__ int lds_i[4];
   const int j = W.tile_job[W.tile0 + blockIdx.x];
-  if (j < 0) return;
+  if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
   const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const int64_t loc = slot - J.p0;
@@ -788,15 +863,16 @@ __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
       const unsigned long long qi = atomicAdd(&W.ctr[0], 1ull);
       W.ovq[qi] = int32_t(slot);   // at most one entry per slot: qi < cap_total
     }
+    const float4 r0 = W.rec[4 * int64_t(slot) + 0];
+    W.recb[slot] = make_float4(r0.x, r0.y, r0.z, float(S.h[slot]));
   }
   W.nbc[slot] = cnt;
-  const int mx = block_reduce(cnt, OpMaxI(), lds_i);
-  if (threadIdx.x == 0 && mx > 0) atomicMax(&W.red[j].maxnb, mx);
+  if (threadIdx.x == 0) W.red[j].built = 1;
 }
 
-// entries K.. of the queued slots, written to the spill pool; the enumeration repeats
-// the list pass exactly, so the entries keep their order.  A slot that finds the pool
-// full keeps its first K entries and flags its job.
+// entries K.. of the queued slots, written to the job's part of the spill pool; the
+// enumeration repeats the list pass exactly, so the entries keep their order.  A
+// slot that finds its job's part full keeps its first K entries and flags the job.
 template <int D>
 __global__ __launch_bounds__(kTile) void k_nb_spill(Work W, PState S) {
   const unsigned long long nq = W.ctr[0];
@@ -806,38 +882,99 @@ __global__ __launch_bounds__(kTile) void k_nb_spill(Work W, PState S) {
     const int j = W.tile_job[slot / kTile];
     const JobParams& J = W.jp[j];
     const int cnt = W.nbc[slot];
-    const unsigned long long off = atomicAdd(&W.ctr[1], (unsigned long long)(cnt - K));
-    if (off + (unsigned long long)(cnt - K) > (unsigned long long)W.spill_cap) {
+    const unsigned long long need = (unsigned long long)(cnt - K);
+    const unsigned long long off = atomicAdd(&W.red[j].spill_used, need);
+    if (off + need > (unsigned long long)J.spill_cap) {
       W.nbc[slot] = K;
       atomicOr(&W.red[j].overflow, 1);
       continue;
     }
-    W.nbo[slot] = int32_t(off);
-    int32_t* dst = W.spill + off;
+    const int64_t at = J.spill_base + int64_t(off);
+    W.nbo[slot] = int32_t(at);
+    int32_t* dst = W.spill + at;
     nb_enumerate<D>(W, S, J, slot, [&](int k, int32_t e) {
       if (k >= K) dst[k - K] = e;
     });
   }
 }
 
+// The reference's neighbour predicate (Level_info::Refresh_neighbor_info with
+// CUT_OFF * h of either particle, level_infor.cpp:469-506) on one list entry: the
+// FP32 distance d2 of the compensated relative position decides outside its error
+// band; inside it the FP64 distance of the stored positions decides.
+template <int D>
+__device__ __forceinline__ bool is_neighbor(const JobParams& J, const Work& W, const PState& S, int64_t q,
+                                            uint32_t code, float d2, float rfi, float hq, const float* xhi,
+                                            const float* xli, double hid) {
+  const float r = fmaxf(rfi, hq * float(J.cut_off));
+  const float band = J.pf_tol + 1.e-6f * r;
+  const float rlo = r - band, rhi = r + band;
+  if (d2 < rlo * rlo) return true;
+  if (d2 > rhi * rhi) return false;
+  const float4 qr = W.rec[4 * int64_t(q) + 0], ql = W.rec[4 * int64_t(q) + 1];
+  const double xq[3] = {double(qr.x) + double(ql.x), double(qr.y) + double(ql.y), double(qr.z) + double(ql.z)};
+  double g[3];
+  image_of<D>(J, code, xq, g);
+  double dist = 0.0;
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    const double dr = (double(xhi[a]) + double(xli[a])) - g[a];
+    dist += dr * dr;
+  }
+  dist = sqrt(dist);
+  return dist <= hid * J.cut_off + 1.e-10 || dist <= S.h[q] * J.cut_off + 1.e-10;
+}
+
+// per-entry neighbour flags of one job's lists (diagnostic download of the sets)
+template <int D>
+__global__ __launch_bounds__(kTile) void k_nb_flags(Work W, PState S, const int64_t* offs, uint8_t* flags) {
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0) return;
+  const JobParams& J = W.jp[j];
+  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  if (slot - J.p0 >= J.n) return;
+  const float4 r0 = W.rec[4 * int64_t(slot) + 0], r1 = W.rec[4 * int64_t(slot) + 1];
+  const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
+  const double hd = S.h[slot];
+  const float rfi = float(hd * J.cut_off);
+  const int cnt = W.nbc[slot];
+  const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
+  const int64_t o = offs[slot - J.p0];
+  for (int k = 0; k < cnt; k++) {
+    const uint32_t e = nb_entry(W, slot, k, off);
+    const int64_t q = e & kJMask;
+    const uint32_t code = e >> kJBits;
+    const float4 qh = W.rec[4 * int64_t(q) + 0], ql = W.rec[4 * int64_t(q) + 1];
+    float dr[3] = {0.f, 0.f, 0.f};
+    rel_pos<D>(J, qh, ql, code, xhi, xli, dr);
+    float d2 = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; a++) d2 += dr[a] * dr[a];
+    flags[o + k] = is_neighbor<D>(J, W, S, q, code, d2, rfi, qh.w, xhi, xli, hd) ? 1 : 0;
+  }
+}
+
 // ---------------------------------------------------------------------------
-// P3: kernel summation + stage-1 force + stage-1 dt
+// P3: kernel summation + stage-1 force + stage-1 dt, one pass over the list
 
 template <int D>
 __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
   __shared__ double lds_d[4];
+  __shared__ int lds_i[4];
   const int j = W.tile_job[W.tile0 + blockIdx.x];
   if (j < 0) return;
   const JobParams& J = W.jp[j];
   const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const int64_t loc = slot - J.p0;
   double dt = 1.0e300;
+  int ntr = 0;
   if (loc < J.n) {
-    const float4 r0 = W.rec0[slot], r1 = W.rec1[slot];
+    const float4 r0 = W.rec[4 * int64_t(slot) + 0], r1 = W.rec[4 * int64_t(slot) + 1];
     const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
     float ni[3] = {0.f, 0.f, 0.f};
     const double hd = S.h[slot];
     const float h = r0.w;
+    const float rfi = float(hd * J.cut_off);
     const int type = __float_as_int(r1.w);
     const int cnt = W.nbc[slot];
     const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
@@ -846,90 +983,92 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
 #pragma unroll
       for (int a = 0; a < D; a++) ni[a] = S.nrm[a][slot];
     }
-    // Get_kernel_summation (particle_mesh_generation.cpp:690-765)
+    const float dxi = h * J.kdx;
+    const float vol2i = ipow<D>(dxi) * ipow<D>(dxi);
+    // Get_kernel_summation (particle_mesh_generation.cpp:690-765) and the stage-1
+    // Accumulate_particle_force_real / _surf in one pass: the force is
+    // -sum_i * sum_j (vol_i^2 + vol_j^2) dW dr with sum_i known only at the end
     float sum = 0.f;
     int ncount = 0;
+    float G[3] = {0.f, 0.f, 0.f};
+    uint64_t mask[kMaskWords];
+#pragma unroll
+    for (int w = 0; w < kMaskWords; w++) mask[w] = 0ull;
     for (int k = 0; k < cnt; k++) {
       const uint32_t e = nb_entry(W, slot, k, off);
       const int64_t q = e & kJMask;
       const uint32_t code = e >> kJBits;
-      const float4 qh = W.rec0[q], ql = W.rec1[q];
+      const float4 qh = W.rec[4 * int64_t(q) + 0], ql = W.rec[4 * int64_t(q) + 1];
       float dr[3] = {0.f, 0.f, 0.f};
+      rel_pos<D>(J, qh, ql, code, xhi, xli, dr);
+      float d2 = 0.f;
+#pragma unroll
+      for (int a = 0; a < D; a++) d2 += dr[a] * dr[a];
+      if (!is_neighbor<D>(J, W, S, q, code, d2, rfi, qh.w, xhi, xli, hd)) continue;
+      ntr++;
+      const int tq = __float_as_int(ql.w);
+      const bool self = q == slot && code == 0;
+      // entries the stage-2 pass visits: real neighbours other than itself, and for a
+      // surface particle not the REAL ones
+      if (k < 64 * kMaskWords && code == 0 && !self && !(surf && tq == SMG_REAL)) {
+#pragma unroll
+        for (int w = 0; w < kMaskWords; w++)
+          if ((k >> 6) == w) mask[w] |= 1ull << (k & 63);
+      }
       if (!surf) {
-        rel_pos<D>(J, qh, ql, code, xhi, xli, dr);
-        float r2 = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; a++) r2 += dr[a] * dr[a];
-        sum += Wk<D>(sqrtf(r2), h);
+        const float rr = sqrtf(d2);
+        sum += Wk<D>(rr, h);
         ncount++;
-      } else if (__float_as_int(ql.w) == SMG_SURFACE) {
-        float ca = 0.f;
+        if (self) continue;
+        const float hq = qh.w;
+        const float dxq = hq * J.kdx;
+        const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
+        const float aw = dWk<D>(rr, 0.5f * (h + hq)) * (vol2i + vol2q);
 #pragma unroll
-        for (int a = 0; a < D; a++) ca += ni[a] * S.nrm[a][q];
-        if (fabsf(ca) > J.cos_thres) {
-          rel_pos<D>(J, qh, ql, code, xhi, xli, dr);
-          float dn = 0.f;
+        for (int a = 0; a < D; a++) G[a] += dr[a] * aw;
+      } else {
+        if (tq == SMG_REAL) continue;   // a surface particle sees surface neighbours only
+        bool aligned = true;
+        if (tq == SMG_SURFACE) {
+          const float4 qn = W.rec[4 * q + 3];
+          const float nq[3] = {qn.x, qn.y, qn.z};
+          float ca = 0.f;
 #pragma unroll
-          for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
-          float r2 = 0.f;
+          for (int a = 0; a < D; a++) ca += ni[a] * nq[a];
+          aligned = fabsf(ca) > J.cos_thres;
+        }
+        if (!aligned) continue;
+        float dn = 0.f;
 #pragma unroll
-          for (int a = 0; a < D; a++) {
-            const float t = dr[a] - ni[a] * dn;
-            r2 += t * t;
-          }
-          sum += Wk<D - 1>(sqrtf(r2), h);
+        for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
+        float rt = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+          dr[a] = dr[a] - ni[a] * dn;
+          rt += dr[a] * dr[a];
+        }
+        rt = sqrtf(rt);
+        if (tq == SMG_SURFACE) {
+          sum += Wk<D - 1>(rt, h);
           ncount++;
         }
+        if (self) continue;
+        const float hq = qh.w;
+        const float dxq = hq * J.kdx;
+        const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
+        const float aw = dWk<D - 1>(rt, 0.5f * (h + hq)) * (vol2i + vol2q);
+#pragma unroll
+        for (int a = 0; a < D; a++) G[a] += dr[a] * aw;
       }
     }
     if (ncount > 1) sum = 1.f / (sum + 1.e-15f);
     W.sum[slot] = sum;
-    // Accumulate_particle_force_real / _surf, stage 1
-    const float dxi = h * J.kdx;
-    const float vol2i = ipow<D>(dxi) * ipow<D>(dxi);
+    W.ntrue[slot] = ntr;
+#pragma unroll
+    for (int w = 0; w < kMaskWords; w++) W.nbm[int64_t(w) * W.cap_total + slot] = mask[w];
     float F[3] = {0.f, 0.f, 0.f};
-    for (int k = 0; k < cnt; k++) {
-      const uint32_t e = nb_entry(W, slot, k, off);
-      const int64_t q = e & kJMask;
-      const uint32_t code = e >> kJBits;
-      if (q == slot && code == 0) continue;
-      const float4 qh = W.rec0[q], ql = W.rec1[q];
-      const int tq = __float_as_int(ql.w);
-      if (surf && tq == SMG_REAL) continue;
-      float dr[3] = {0.f, 0.f, 0.f};
-      rel_pos<D>(J, qh, ql, code, xhi, xli, dr);
-      const float hq = qh.w;
-      const float dxq = hq * J.kdx;
-      const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
-      const float hij = 0.5f * (h + hq);
-      float aw;
-      if (!surf) {
-        float r2 = 0.f;
 #pragma unroll
-        for (int a = 0; a < D; a++) r2 += dr[a] * dr[a];
-        aw = dWk<D>(sqrtf(r2), hij);
-      } else {
-        if (tq == SMG_SURFACE) {
-          float ca = 0.f;
-#pragma unroll
-          for (int a = 0; a < D; a++) ca += ni[a] * S.nrm[a][q];
-          if (fabsf(ca) <= J.cos_thres) continue;
-        }
-        float dn = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
-        float r2 = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; a++) {
-          dr[a] = dr[a] - ni[a] * dn;
-          r2 += dr[a] * dr[a];
-        }
-        aw = dWk<D - 1>(sqrtf(r2), hij);
-      }
-      const float Pij = sum * (vol2i + vol2q);
-#pragma unroll
-      for (int a = 0; a < D; a++) F[a] += dr[a] * aw * -Pij;
-    }
+    for (int a = 0; a < D; a++) F[a] = -sum * G[a];
     if (surf) {   // Map_surface_particle_force with the raw normal
       float nf = 0.f;
 #pragma unroll
@@ -943,7 +1082,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
       W.a1[a][slot] = F[a];
       A += double(F[a]) * double(F[a]);
     }
-    W.rec2[slot] = make_float4(F[0], F[1], D > 2 ? F[2] : 0.f, W.nu_a[slot]);
+    W.rec[4 * int64_t(slot) + 2] = make_float4(F[0], F[1], D > 2 ? F[2] : 0.f, W.nu_b[slot]);
     A = sqrt(A);
     // Set_timestep(1): v was zeroed by Reset_particle_force(1), so U = 0
     const double t1 = 0.25 * sqrt(hd / (A + 1.e-15));
@@ -951,7 +1090,11 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
     dt = amin(t1, t2);
   }
   dt = block_reduce(dt, OpMin(), lds_d);
-  if (threadIdx.x == 0 && dt < 1.0e300) atomicMin(&W.red[j].dt1[par], dbits(dt));
+  const int mx = block_reduce(ntr, OpMaxI(), lds_i);
+  if (threadIdx.x == 0) {
+    if (dt < 1.0e300) atomicMin(&W.red[j].dt1[par], dbits(dt));
+    if (mx > 0) atomicMax(&W.red[j].maxnb, mx);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -969,7 +1112,7 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
   if (loc < J.n) {
     const double dt1 = bitsd(W.red[j].dt1[par]);
     const float hdt = float(0.5 * dt1);
-    const float4 r0 = W.rec0[slot], r1 = W.rec1[slot], r2 = W.rec2[slot];
+    const float4 r0 = W.rec[4 * int64_t(slot) + 0], r1 = W.rec[4 * int64_t(slot) + 1], r2 = W.rec[4 * int64_t(slot) + 2];
     const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
     const float a1i[3] = {r2.x, r2.y, r2.z};
     float ni[3] = {0.f, 0.f, 0.f}, vi[3] = {0.f, 0.f, 0.f};
@@ -981,6 +1124,7 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
     }
     const double hd = S.h[slot];
     const float h = r0.w;
+    const float rfi = float(hd * J.cut_off);
     const int type = __float_as_int(r1.w);
     const bool surf = type == SMG_SURFACE;
     if (surf) {
@@ -994,64 +1138,79 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
     const float rhoi = 1.f / voli;
     const float etai = r2.w * rhoi;
     float F[3] = {0.f, 0.f, 0.f};
-    if (etai != 0.f) {   // eta_ij = 0 for every pair when this particle's nu is 0
-      for (int k = 0; k < cnt; k++) {
-        const uint32_t e = nb_entry(W, slot, k, off);
-        const uint32_t code = e >> kJBits;
-        if (code != 0) continue;   // mirror ghosts carry nu = 0, so eta_ij = 0 (Q6)
-        const int64_t q = e & kJMask;
-        if (q == slot) continue;
-        const float4 qh = W.rec0[q], ql = W.rec1[q];
-        const int tq = __float_as_int(ql.w);
-        if (surf && tq == SMG_REAL) continue;
-        const float4 qv = W.rec2[q];
-        const float hq = qh.w;
-        const float dxq = hq * J.kdx;
-        const float volq = ipow<D>(dxq);
-        const float rhoq = 1.f / volq;
-        const float etaq = qv.w * rhoq;
-        if (etaq == 0.f) continue;   // eta_ij = 0
-        float dr[3] = {0.f, 0.f, 0.f};
-        rel_pos<D>(J, qh, ql, 0u, xhi, xli, dr);
-        const float hij = 0.5f * (h + hq);
-        float aw;
-        if (!surf) {
-          float rr = 0.f;
+    // one stage-2 pair term; checked = the entry is a real, non-self neighbour that a
+    // surface particle keeps (from the stage-1 masks)
+    auto pair = [&](int k, bool checked) {
+      const uint32_t e = nb_entry(W, slot, k, off);
+      const uint32_t code = e >> kJBits;
+      const int64_t q = e & kJMask;
+      if (!checked && (code != 0 || q == slot)) return;   // ghosts carry nu = 0 (Q6)
+      const float4 qh = W.rec[4 * int64_t(q) + 0], ql = W.rec[4 * int64_t(q) + 1];
+      const int tq = __float_as_int(ql.w);
+      if (!checked && surf && tq == SMG_REAL) return;
+      const float4 qv = W.rec[4 * int64_t(q) + 2];
+      const float hq = qh.w;
+      const float dxq = hq * J.kdx;
+      const float volq = ipow<D>(dxq);
+      const float rhoq = 1.f / volq;
+      const float etaq = qv.w * rhoq;
+      if (etaq == 0.f) return;   // eta_ij = 0
+      float dr[3] = {0.f, 0.f, 0.f};
+      rel_pos<D>(J, qh, ql, 0u, xhi, xli, dr);
+      float rr = 0.f;
 #pragma unroll
-          for (int a = 0; a < D; a++) rr += dr[a] * dr[a];
-          aw = dWk<D>(sqrtf(rr), hij);
-        } else {
-          if (tq == SMG_SURFACE) {
-            float ca = 0.f;
+      for (int a = 0; a < D; a++) rr += dr[a] * dr[a];
+      if (!checked && !is_neighbor<D>(J, W, S, q, 0u, rr, rfi, hq, xhi, xli, hd)) return;
+      const float hij = 0.5f * (h + hq);
+      float aw;
+      if (!surf) {
+        aw = dWk<D>(sqrtf(rr), hij);
+      } else {
+        if (tq == SMG_SURFACE) {
+          const float4 qn = W.rec[4 * q + 3];
+          const float nq[3] = {qn.x, qn.y, qn.z};
+          float ca = 0.f;
 #pragma unroll
-            for (int a = 0; a < D; a++) ca += ni[a] * S.nrm[a][q];
-            if (fabsf(ca) <= J.cos_thres) continue;
-          }
-          float dn = 0.f;
-#pragma unroll
-          for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
-          float rr = 0.f;
-#pragma unroll
-          for (int a = 0; a < D; a++) {
-            const float t = dr[a] - ni[a] * dn;
-            rr += t * t;
-          }
-          aw = dWk<D - 1>(sqrtf(rr), hij);
+          for (int a = 0; a < D; a++) ca += ni[a] * nq[a];
+          if (fabsf(ca) <= J.cos_thres) return;
         }
-        const float qa[3] = {qv.x, qv.y, qv.z};
-        float vq[3];
-        if (!surf && tq != SMG_REAL) {   // Q8: v_j := -v_i
+        float dn = 0.f;
 #pragma unroll
-          for (int a = 0; a < D; a++) vq[a] = -vi[a];
-        } else {
+        for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
+        float rt = 0.f;
 #pragma unroll
-          for (int a = 0; a < D; a++) vq[a] = qa[a] * hdt;
+        for (int a = 0; a < D; a++) {
+          const float t = dr[a] - ni[a] * dn;
+          rt += t * t;
         }
-        const float etaij = 2.f * etaq * etai / (etai + etaq + 1.e-15f);
-        const float temp0 = aw * etaij * (voli * voli + volq * volq);
-#pragma unroll
-        for (int a = 0; a < D; a++) F[a] += (vi[a] - vq[a]) * temp0;
+        aw = dWk<D - 1>(sqrtf(rt), hij);
       }
+      const float qa[3] = {qv.x, qv.y, qv.z};
+      float vq[3];
+      if (!surf && tq != SMG_REAL) {   // Q8: v_j := -v_i
+#pragma unroll
+        for (int a = 0; a < D; a++) vq[a] = -vi[a];
+      } else {
+#pragma unroll
+        for (int a = 0; a < D; a++) vq[a] = qa[a] * hdt;
+      }
+      const float etaij = 2.f * etaq * etai / (etai + etaq + 1.e-15f);
+      const float temp0 = aw * etaij * (voli * voli + volq * volq);
+#pragma unroll
+      for (int a = 0; a < D; a++) F[a] += (vi[a] - vq[a]) * temp0;
+    };
+    if (etai != 0.f) {   // eta_ij = 0 for every pair when this particle's nu is 0
+#pragma unroll
+      for (int w = 0; w < kMaskWords; w++) {
+        if (64 * w >= cnt) break;
+        uint64_t m = W.nbm[int64_t(w) * W.cap_total + slot];
+        while (m) {
+          const int k = 64 * w + __builtin_ctzll(m);
+          m &= m - 1;
+          pair(k, true);
+        }
+      }
+      for (int k = 64 * kMaskWords; k < cnt; k++) pair(k, false);
     }
     if (surf) {
       float nf = 0.f;
@@ -1149,6 +1308,22 @@ __global__ void k_fp64_probe(int64_t n, const double* a, const double* b, double
     else KERNEL<3><<<__VA_ARGS__>>>;                                      \
   } while (0)
 
+void launch_decide(hipStream_t st, const Work& W, int njobs, int force) {
+  k_decide<<<(njobs + 255) / 256, 256, 0, st>>>(W, njobs, force);
+}
+void launch_key(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S) {
+  if (D == 2) k_key<2><<<ntiles, kTile, 0, st>>>(W, S);
+  else k_key<3><<<ntiles, kTile, 0, st>>>(W, S);
+}
+void launch_copyback(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& A, const PState& B) {
+  if (D == 2) k_copyback<2><<<ntiles, kTile, 0, st>>>(W, A, B);
+  else k_copyback<3><<<ntiles, kTile, 0, st>>>(W, A, B);
+}
+void launch_nb_flags(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, const int64_t* offs,
+                     uint8_t* flags) {
+  if (D == 2) k_nb_flags<2><<<ntiles, kTile, 0, st>>>(W, S, offs, flags);
+  else k_nb_flags<3><<<ntiles, kTile, 0, st>>>(W, S, offs, flags);
+}
 void launch_reset(hipStream_t st, const Work& W, int njobs, int par) {
   k_reset<<<(njobs + 255) / 256, 256, 0, st>>>(W.red, njobs, par, W.ctr);
 }
@@ -1160,11 +1335,13 @@ void launch_map(hipStream_t st, int D, int64_t ntiles, const Work& W, const PSta
   if (D == 2) k_map<2><<<ntiles, kTile, 0, st>>>(W, S, mode, par);
   else k_map<3><<<ntiles, kTile, 0, st>>>(W, S, mode, par);
 }
-void launch_scan(hipStream_t st, const Work& W) {
+void launch_scan(hipStream_t st, const Work& W, bool gated) {
   const int64_t nb = (W.n_tbl + kScanItems - 1) / kScanItems;
-  k_scan_reduce<<<nb, kTile, 0, st>>>(W.counts, W.n_tbl, W.bsum);
-  k_scan_top<<<1, 1024, 0, st>>>(W.bsum, nb);
-  k_scan_down<<<nb, kTile, 0, st>>>(W.counts, W.n_tbl, W.bsum, nb, W.starts);
+  if (nb == 0) return;
+  const unsigned long long* gate = gated ? W.ctr + 1 : nullptr;
+  k_scan_reduce<<<nb, kTile, 0, st>>>(W.counts, W.n_tbl, W.bsum, gate);
+  k_scan_top<<<1, 1024, 0, st>>>(W.bsum, nb, gate);
+  k_scan_down<<<nb, kTile, 0, st>>>(W.counts, W.n_tbl, W.bsum, nb, W.starts, gate);
 }
 void launch_scatter(hipStream_t st, int64_t ntiles, const Work& W) {
   k_scatter<<<ntiles, kTile, 0, st>>>(W);
@@ -1181,7 +1358,7 @@ void launch_nb(hipStream_t st, int D, int64_t ntiles, const Work& W, const PStat
   else k_nb<3><<<ntiles, kTile, 0, st>>>(W, S);
 }
 void launch_nb_spill(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S) {
-  const int64_t g = ntiles < 256 ? ntiles : 256;   // grid-stride over the (usually empty) queue
+  const int64_t g = ntiles < 1024 ? ntiles : 1024;   // grid-stride over the (usually empty) queue
   if (D == 2) k_nb_spill<2><<<g, kTile, 0, st>>>(W, S);
   else k_nb_spill<3><<<g, kTile, 0, st>>>(W, S);
 }

@@ -11,9 +11,14 @@ namespace smg {
 enum { kModeInit = 0, kModePrerun = 1, kModeStep = 2 };
 
 void launch_reset(hipStream_t st, const Work& W, int njobs, int par);
+void launch_decide(hipStream_t st, const Work& W, int njobs, int force);
+void launch_key(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S);
+void launch_copyback(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& A, const PState& B);
+void launch_nb_flags(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, const int64_t* offs,
+                     uint8_t* flags);
 void launch_seed(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S);
 void launch_map(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int mode, int par);
-void launch_scan(hipStream_t st, const Work& W);
+void launch_scan(hipStream_t st, const Work& W, bool gated);
 void launch_scatter(hipStream_t st, int64_t ntiles, const Work& W);
 void launch_segsort(hipStream_t st, int64_t ntiles, const Work& W, const PState& S);
 void launch_gather(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& A, const PState& B);

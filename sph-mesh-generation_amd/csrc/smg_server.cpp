@@ -38,7 +38,7 @@ struct Sizes {
 Sizes sizes_for(int K, int sps) {
   Sizes z;
   const int64_t state = 9 * 4 + 8 + 4 + 4;   // xh, xl, nrm (3 each), h, type, id
-  z.per_slot = 2 * state + 2 * 4 + 6 * 4 + 4 + 3 * 16 + 3 * 4 + int64_t(K) * 4 + 3 * 4 + int64_t(sps) * 4 + 1;
+  z.per_slot = 2 * state + 2 * 4 + 6 * 4 + 4 + 4 * 16 + 3 * 4 + int64_t(K) * 4 + 4 * 4 + 16 + int64_t(sps) * 4 + 1;
   z.per_table = 2 * 4 + 1;
   z.per_cell = int64_t(sizeof(smg::LsCell));
   z.per_job = int64_t(sizeof(smg::JobParams) + sizeof(smg::JobRed));
@@ -60,7 +60,7 @@ extern "C" {
 
 int smg_server_footprint_of(const smg_server_config* c, int64_t* bytes) {
   if (!c || !bytes) { set_error("smg_server_footprint_of: NULL argument"); return SMG_ERR_ARG; }
-  const Sizes z = sizes_for(c->max_neighbors > 0 ? c->max_neighbors : 64, c->spill_per_slot > 0 ? c->spill_per_slot : 32);
+  const Sizes z = sizes_for(c->max_neighbors > 0 ? c->max_neighbors : 128, c->spill_per_slot > 0 ? c->spill_per_slot : 32);
   *bytes = c->max_slots * z.per_slot + c->max_table * z.per_table + c->max_lset_cells * z.per_cell +
            int64_t(c->max_jobs) * z.per_job;
   return SMG_OK;

@@ -33,7 +33,7 @@ class SmgConfig(C.Structure):
         ("min_dl", C.c_double), ("max_dl", C.c_double), ("max_phi", C.c_double),
         ("lin_dx0", C.c_double), ("lin_dx1", C.c_double), ("lin_band", C.c_double),
         ("seed", C.c_uint64), ("remap_max_attempts", C.c_int32), ("max_neighbors", C.c_int32),
-        ("spill_per_slot", C.c_int32),
+        ("spill_per_slot", C.c_int32), ("reserved0", C.c_int32), ("skin", C.c_double),
     ]
 
 
@@ -53,6 +53,7 @@ class SmgStats(C.Structure):
         ("max_neighbors_seen", C.c_int32), ("num_level", C.c_int32),
         ("dt1", C.c_double), ("dt2", C.c_double), ("max_v", C.c_double),
         ("residual", C.c_double), ("max_disp", C.c_double), ("res_max", C.c_double),
+        ("n_rebuild", C.c_int64),
     ]
 
     def as_dict(self):

@@ -29,7 +29,7 @@ def test_host_entry_points(smg_mod):
     assert abs(cfg.angle_thres - 0.4 * 3.1415926) < 1e-15
     cfg2 = smg_mod.default_config(2)
     assert cfg2.cell_ratio == 2.0 and cfg2.sym[2] == 0
-    assert smg_mod.lib().smg_abi_version() == 2
+    assert smg_mod.lib().smg_abi_version() == 3
 
 
 def test_bad_arguments_are_rejected(smg_mod):
@@ -78,7 +78,7 @@ def test_server_sizing_is_host_only(smg_mod):
     b0 = smg_mod.server_footprint(base)
     for field in ("max_slots", "max_table", "max_lset_cells", "max_jobs", "max_neighbors"):
         c = smg_mod.SmgServerConfig.from_buffer_copy(base)
-        setattr(c, field, 2 * getattr(c, field) if getattr(c, field) else 128)
+        setattr(c, field, 2 * getattr(c, field) if getattr(c, field) else 512)
         assert smg_mod.server_footprint(c) > b0, field
 
 

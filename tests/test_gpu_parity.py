@@ -171,3 +171,38 @@ def test_spill_pool_exhaustion_is_an_error(smg_mod):
     with pytest.raises(smg_mod.SmgError) as e:
         job.step(2)
     assert e.value.rc == -3 and "spill pool" in str(e.value)
+
+
+@pytest.mark.parametrize("case", ["2d", "3d"])
+def test_lists_between_rebuilds_match_brute_force(smg_mod, orc, c1, case):
+    """Candidate lists are rebuilt only when some particle drifted by the skin; on the
+    steps in between the force passes' neighbour sets still equal a brute-force search
+    on the current positions."""
+    cfg, lset = c1 if case == "2d" else _c3d(smg_mod)
+    n = int(lset["stats"].num_particle)
+    job = smg_mod.Job(cfg, lset)
+    job.seed(n)
+    job.step(30)
+    r0 = job.stats()["n_rebuild"]
+    for _ in range(10):
+        job.step(1)
+        g = job.state()
+        offs, ids, codes = job.neighbors()
+        # 2D: brute force; 3D: the oracle's cell search (itself pinned to brute force)
+        bo, bi, bc = orc.neighbors(cfg, g["x"], g["h"], brute=case == "2d")
+        assert np.array_equal(offs, bo) and np.array_equal(ids, bi) and np.array_equal(codes, bc)
+    st = job.stats()
+    assert st["n_rebuild"] - r0 < 10   # some steps reused the lists
+    assert st["n_neighbor_entries"] == int(offs[-1])
+    job.close()
+
+
+def test_negative_skin_rebuilds_every_step(smg_mod, c1):
+    cfg, lset = c1
+    c = smg_mod.SmgConfig.from_buffer_copy(cfg)
+    c.skin = -1.0
+    job = smg_mod.Job(c, lset)
+    job.seed(int(lset["stats"].num_particle))
+    job.step(7)
+    assert job.stats()["n_rebuild"] == 7
+    job.close()
