@@ -1,10 +1,17 @@
-"""Benchmark: 3D relaxation throughput (particle-steps/s) of the MI355X engine.
+"""Benchmarks of the MI355X relaxation engine.  Each mode prints one JSON line.
 
-Workload (one GPU): unit cube minus a sphere r = 0.3 at the centre, level set on a
-256^3 grid, target spacing linear in phi from 0.00451 at the surface to 0.01354 at
-phi >= 0.2 (0.564 x the 0.008 -> 0.024 profile, ~1.0M particles), seeded uniform
-random positions, the full relaxation step (mapping, sort, neighbour lists, both
-force stages, both dt reductions) per timed step.  Prints one JSON line.
+relax (default): 3D relaxation throughput in particle-steps/s.  Workload: unit cube
+  minus a sphere r = 0.3 at the centre, level set on a 256^3 grid, target spacing
+  linear in phi from 0.00451 at the surface to 0.01354 at phi >= 0.2 (~1.0M
+  particles), seeded uniform-random positions; each timed step is the full
+  relaxation step (mapping, drift test and -- where due -- sort and list rebuild,
+  both force stages, both dt reductions, residual).
+batch: 512 mixed jobs (half 2D circle unions, half 3D sphere unions, N log-uniform
+  in [1e3, 2e5]) run for a fixed number of steps, batched (one launch per pass for
+  all jobs of a dimension, graph-replayed) against the plain path (one job at a time).
+serve: Poisson arrivals of 3D sphere-union jobs (N log-uniform in [5e3, 5e5]) into a
+  continuous-batching server, retired when converged; jobs/s and latency percentiles
+  against the same arrivals served one job at a time.
 """
 import argparse
 import json
@@ -12,28 +19,40 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sph-mesh-generation_amd"))
 
 METRIC = "particle-steps/s (3D 1M relax); batched jobs/s + p99 latency vs plain GPU"
 HBM_PEAK_GBS = 8000.0
 
-# algorithmic bytes per particle and launch (3D, 2D): each field counted once per
-# dependent pass, neighbour data assumed served from cache
+# Algorithmic bytes per particle and launch (3D, 2D), from the per-pass model: each
+# particle field counted once per dependent pass, neighbour data assumed served from
+# cache (P1 integrate + sample + map + h + key 252/156, P2 sort 156/116, P3 density +
+# force1 44/32, P4 force2 60/44, P5 v/nu/residual 40/32; 552/380 per particle-step).
+# k_map carries P1 and P5 (the velocity update and residual fold into it); the sort
+# passes carry P2 and run only on the steps where a job rebuilds its lists.
+STEP_BYTES = (552, 380)
 BYTES = {
-    "k_map": (292, 188),      # integrate (a1, a2 -> v) + level-set sample + map + h + key
-    "k_nb": (32, 24),         # read x, h
-    "k_force1": (44, 32),     # read x, h, type, n; write a1
-    "k_force2": (60, 44),     # read x, h, type, n, a1, nu; write a2, nu
-    "k_scan": (0, 0),
+    "k_map": (252 + 40, 156 + 32),
+    "k_force1": (44, 32),
+    "k_force2": (60, 44),
+    "k_key": (36, 28),
     "k_scatter": (16, 16),
     "k_segsort": (20, 20),
-    "k_gather": (120, 80),
+    "k_gather": (60, 40),
+    "k_copyback": (60, 40),
+    "k_nb": (32, 24),
 }
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def pct(v, q):
+    return float(np.percentile(np.asarray(v, float), q)) if len(v) else None
 
 
 def cpu_baseline(cfg, lset, n, steps):
@@ -54,21 +73,7 @@ def cpu_baseline(cfg, lset, n, steps):
                       f"(FP64 oracle, OpenMP, {dt:.1f} s)"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--scale", type=float, default=0.564)
-    ap.add_argument("--num-pkg", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU: the relaxation of one job has no multi-GPU path in this build "
-              "(see the sharding note in DESIGN.md); use --gpus 1", file=sys.stderr)
-        sys.exit(2)
-    import smg
+def relax(args, smg):
     t0 = time.perf_counter()
     cfg, sdf = smg.config_c2(scale=args.scale, num_pkg=args.num_pkg)
     lset = smg.make_levelset(cfg, sdf)
@@ -79,10 +84,11 @@ def main():
     job.seed(n)
     job.step(args.warmup)
     log(f"seeded + {args.warmup} warmup steps in {time.perf_counter() - t0:.1f} s")
-    job.sync = None
-    smg.lib().smg_profile_enable(job.h, 1)   # events around each kernel over the timed region
+    rb0 = job.stats()["n_rebuild"]
+    # HIP events around each kernel on the engine's stream over the timed region
+    smg.lib().smg_profile_enable(job.h, 1)
     t0 = time.perf_counter()
-    job.step(args.steps)
+    job.step(args.steps)   # returns after the stream has drained
     elapsed = time.perf_counter() - t0
     prof = job.profile()
     st = job.stats()
@@ -128,10 +134,11 @@ def main():
             "algorithmic_bytes_per_launch": bpl,
         },
         "step_roofline": {
-            "bytes_per_particle_step": 552 if D == 3 else 380,
-            "achieved_GBs": (552 if D == 3 else 380) * value / 1e9,
-            "frac": (552 if D == 3 else 380) * value / 1e9 / HBM_PEAK_GBS,
+            "bytes_per_particle_step": STEP_BYTES[0 if D == 3 else 1],
+            "achieved_GBs": STEP_BYTES[0 if D == 3 else 1] * value / 1e9,
+            "frac": STEP_BYTES[0 if D == 3 else 1] * value / 1e9 / HBM_PEAK_GBS,
         },
+        "list_rebuilds_per_step": (st["n_rebuild"] - rb0) / args.steps,
         "kernels_ms": {k: v[0] for k, v in prof.items()},
     }
     if not args.no_cpu:
@@ -140,6 +147,163 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+
+
+def batch(args, smg):
+    t0 = time.perf_counter()
+    jobs = smg.batched_jobs(args.jobs, 1000, 200000, seed=20261203, dims=(2, 3))
+    n_tot = sum(n for _, _, n in jobs)
+    log(f"{len(jobs)} jobs, {n_tot} particles, level sets in {time.perf_counter() - t0:.1f} s")
+    S, G = args.job_steps, args.graph_steps
+    # batched: all jobs of a dimension in one slot space, graph-replayed step loop
+    t0 = time.perf_counter()
+    b = smg.Batch(jobs)
+    b.seed()
+    b.capture(G)
+    b.run(S // G)
+    b.sync()
+    tb = time.perf_counter() - t0
+    b.close()
+    log(f"batched: {tb:.2f} s")
+    # plain: one job per launch sequence, eager, one after the other
+    t0 = time.perf_counter()
+    lat = []
+    for cfg, ls, n in jobs:
+        j = smg.Job(cfg, ls)
+        j.seed(n)
+        j.step(S)
+        j.close()
+        lat.append(time.perf_counter() - t0)
+    tp = time.perf_counter() - t0
+    log(f"plain: {tp:.2f} s")
+    out = {
+        "metric": "jobs/s (batched mixed 2D/3D relaxation, fixed steps)",
+        "value": len(jobs) / tb, "unit": "jobs/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(jobs)} jobs, half 2D circle unions, half 3D sphere unions, "
+                               f"N log-uniform 1e3..2e5, {S} steps each (seed + pre-run included)",
+                   "particles": n_tot, "graph_steps": G},
+        "plain_jobs_per_s": len(jobs) / tp,
+        "speedup_vs_plain": tp / tb,
+        "latency_s": {"batched_p50": tb, "batched_p99": tb, "plain_p50": pct(lat, 50), "plain_p99": pct(lat, 99)},
+        "particle_steps_per_s": {"batched": n_tot * S / tb, "plain": n_tot * S / tp},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def serve_once(smg, specs, lam, duration, max_jobs, budget_gb, rng_seed, label, max_active=0):
+    """Poisson arrivals at rate lam for `duration` seconds into a server; returns the
+    completions inside the window."""
+    dim = specs[0][0].dim
+    tbl = max(smg.job_size(c, n)[1] for c, _, n in specs)
+    cells = max(smg.job_size(c, n)[2] for c, _, n in specs)
+    slots = (1 << 26) - 256   # 26-bit slot indices of the neighbour entries
+    jobs_cap = max_jobs
+    sc = smg.server_config(dim, jobs_cap, slots, min(jobs_cap * tbl, (1 << 31) - 16), jobs_cap * cells,
+                           chunk_steps=50, min_steps=100, max_steps=2000, tol=1e-3,
+                           max_active_particles=max_active)
+    while smg.server_footprint(sc) > budget_gb * 1e9 and sc.max_slots > (1 << 20):
+        sc.max_slots //= 2
+    srv = smg.Server(sc)
+    rng = np.random.default_rng(rng_seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=int(lam * duration * 3) + 10))
+    arrivals = arrivals[arrivals < duration]
+    queue, arrived_at, done = [], {}, []
+    k = 0
+    t0 = time.perf_counter()
+    while True:
+        now = time.perf_counter() - t0
+        if now >= duration:
+            break
+        while k < len(arrivals) and arrivals[k] <= now:
+            queue.append(k)
+            k += 1
+        while queue:
+            i = queue[0]
+            c, ls, n = specs[i % len(specs)]
+            cc = smg.SmgConfig.from_buffer_copy(c)
+            cc.seed = c.seed + 1000003 * i
+            t = srv.admit(cc, ls, n)
+            if t is None:
+                break
+            arrived_at[t] = (arrivals[i], n)
+            queue.pop(0)
+        if srv.usage()["n_active"] == 0:
+            if k < len(arrivals):
+                time.sleep(max(0.0, min(arrivals[k] - (time.perf_counter() - t0), duration - now)))
+            continue
+        srv.advance()
+        for f in srv.finished():
+            ta, n = arrived_at[f["ticket"]]
+            done.append({"latency": time.perf_counter() - t0 - ta, "n": n, "steps": f["steps"], "status": f["status"]})
+            srv.result(f["ticket"])
+    foot = smg.server_footprint(sc)
+    srv.close()
+    lat = [d["latency"] for d in done]
+    r = {"jobs_per_s": len(done) / duration, "completed": len(done), "arrived": int(len(arrivals)),
+         "queued_at_end": len(queue) + (len(arrivals) - k), "p50_s": pct(lat, 50), "p99_s": pct(lat, 99),
+         "mean_steps": float(np.mean([d["steps"] for d in done])) if done else None,
+         "converged": sum(d["status"] == 0 for d in done),
+         "particle_steps_per_s": sum(d["n"] * d["steps"] for d in done) / duration,
+         "pool_footprint_GB": foot / 1e9, "max_jobs": jobs_cap}
+    log(f"{label} lam={lam}: {json.dumps(r)}")
+    return r
+
+
+def serve(args, smg):
+    t0 = time.perf_counter()
+    specs = smg.batched_jobs(args.specs, 5000, 500000, seed=777, dims=(3,), max_pkg=16)
+    log(f"{len(specs)} job specs in {time.perf_counter() - t0:.1f} s")
+    res = {}
+    for lam in args.lam:
+        res[str(lam)] = {
+            "served": serve_once(smg, specs, lam, args.duration, args.max_jobs, args.budget_gb, 5 + int(lam), "served",
+                                 args.max_active),
+            "plain": serve_once(smg, specs, lam, args.duration, 1, args.budget_gb, 5 + int(lam), "plain"),
+        }
+    top = res[str(args.lam[-1])]
+    out = {
+        "metric": "jobs/s (served 3D sphere unions, Poisson arrivals, retired at residual < 1e-3)",
+        "value": top["served"]["jobs_per_s"], "unit": "jobs/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"3D sphere unions, N log-uniform 5e3..5e5 ({len(specs)} level sets, distinct "
+                               f"seeds), Poisson lambda {args.lam}/s, {args.duration} s per lambda, chunk 50 steps, "
+                               f"at most {args.max_active} particles in the pool, "
+                               f"retire at mean residual < 1e-3 over a chunk (>= 100 steps) or 2000 steps"},
+        "p50_s": top["served"]["p50_s"], "p99_s": top["served"]["p99_s"],
+        "speedup_vs_plain": (top["served"]["jobs_per_s"] / top["plain"]["jobs_per_s"]
+                             if top["plain"]["jobs_per_s"] else None),
+        "by_lambda": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--mode", choices=("relax", "batch", "serve"), default="relax")
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--scale", type=float, default=0.564)
+    ap.add_argument("--num-pkg", type=int, default=64)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--jobs", type=int, default=512)
+    ap.add_argument("--job-steps", type=int, default=200)
+    ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--specs", type=int, default=48)
+    ap.add_argument("--lam", type=float, nargs="+", default=[20.0, 100.0, 200.0])
+    ap.add_argument("--duration", type=float, default=15.0)
+    ap.add_argument("--max-jobs", type=int, default=256)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--max-active", type=int, default=4000000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (one GPU per box); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import smg
+    {"relax": relax, "batch": batch, "serve": serve}[args.mode](args, smg)
 
 
 if __name__ == "__main__":

@@ -63,13 +63,13 @@ typedef struct smg_config {
   int32_t remap_max_attempts;/* bound on the re-seeding rejection loop        */
   int32_t max_neighbors;     /* inline list entries per particle, 0 = 128;    
                                 longer lists spill to a shared pool          */
-  int32_t spill_per_slot;    /* spill pool entries per particle slot, 0 = 32  */
+  int32_t spill_per_slot;    /* spill pool entries per particle slot, 0 = 64  */
   int32_t reserved0;
   double skin;               /* neighbour lists cover (1 + skin) * cut_off * h
                                 and are rebuilt (with the cell sort) only when
                                 some particle's drift since the last build,
                                 |dx| + cut_off |dh|, reaches skin*cut_off*h/2;
-                                0 = 0.15, < 0 = rebuild every step            */
+                                0 = 0.1, < 0 = rebuild every step             */
 } smg_config;
 
 /* Level-set statistics from smg_lset_build (Levelset::Calculate_total_volume_mass,
@@ -102,6 +102,8 @@ typedef struct smg_stats {
   double max_disp;       /* max_i |v_i| dt2 of the last step                    */
   double res_max;        /* max_i |v_i| dt2 / dx_i of the last step             */
   int64_t n_rebuild;     /* neighbour-list (and cell-sort) builds so far        */
+  double drift;          /* max_i (|x_i - x_b| + cut_off |h_i - h_b|) / h_b of   */
+                         /* the last step (x_b, h_b: at the last list build)     */
 } smg_stats;
 
 typedef struct smg_job smg_job;
@@ -216,12 +218,15 @@ typedef struct smg_server_config {
   int64_t max_table;         /* cell-table entries (a job takes its cell-list tables) */
   int64_t max_lset_cells;    /* level-set cells (a job takes (4 num_pkg + 2)^dim)     */
   int32_t max_neighbors;     /* inline list entries per slot, 0 = 128                 */
-  int32_t spill_per_slot;    /* spill pool entries per slot, 0 = 32                   */
+  int32_t spill_per_slot;    /* spill pool entries per slot, 0 = 64                   */
   int32_t chunk_steps;       /* steps between admissions / convergence checks (> 0)  */
   int32_t min_steps;         /* no retirement by convergence before this many steps  */
   int32_t max_steps;         /* retire unconverged at this many steps (0 = no limit) */
   int32_t reserved;
   double tol;                /* converged: mean over the chunk of smg_stats.residual < tol */
+  int64_t max_active_particles; /* admit only while the pool's particles stay within
+                                   this (0 = no limit; an empty pool admits any job):
+                                   bounds the work per chunk, hence latency          */
 } smg_server_config;
 
 /* finished-job status */

@@ -86,6 +86,7 @@ struct PState {
 struct Work {
   const JobParams* jp;
   const int32_t* tile_job;
+  const int32_t* sblk_job;   // job owning each 4096-entry block of the cell tables (-1: none)
   JobRed* red;
   const LsCell* ls;
   float* nu_a;   // nu read by the stage-2 pass (previous step's value)

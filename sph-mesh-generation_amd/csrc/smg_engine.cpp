@@ -104,7 +104,7 @@ int make_job_params(const JobInput& in, int D, JobParams& J) {
   J.cos_thres = float(std::cos(c.angle_thres));
   J.kdx = float(c.cut_off / c.cell_ratio);
   // candidate lists cover (1 + skin) cut_off h; skin < 0 rebuilds them every step
-  const double skin = c.skin > 0. ? c.skin : (c.skin < 0. ? 0. : 0.15);
+  const double skin = c.skin > 0. ? c.skin : (c.skin < 0. ? 0. : 0.1);
   J.cut_v = (1.0 + skin) * c.cut_off;
   J.skin_lim = c.skin < 0. ? -1.f : float(0.5 * skin * c.cut_off);
   {
@@ -152,6 +152,18 @@ int make_job_params(const JobInput& in, int D, JobParams& J) {
 
 int64_t job_lset_cells(const JobParams& J) { return int64_t(J.ng[0]) * J.ng[1] * J.ng[2]; }
 
+// 4096-entry scan blocks of a job's table segment (+ one entry for a sub-scan's end
+// sentinel); segments start on block boundaries so that no scan block spans two jobs
+int64_t table_blocks(const JobParams& J) { return (J.tbl_size + 1 + kScanItems - 1) / kScanItems; }
+
+int Engine::set_table_owner_(const JobParams& J, int32_t owner) {
+  const int64_t b0 = J.tbl_base / kScanItems, nb = table_blocks(J);
+  for (int64_t b = b0; b < b0 + nb; b++) sblk_job_h_[b] = owner;
+  HIP_TRY(hipMemcpyAsync(d_sblk_job_ + b0, sblk_job_h_.data() + b0, sizeof(int32_t) * nb, hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipStreamSynchronize(st_));
+  return SMG_OK;
+}
+
 int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, int spill_per_slot) {
   if (tbl >= (int64_t(1) << 31) - 2) { set_error("setup: cell tables exceed 2^31 entries"); return SMG_ERR_CAPACITY; }
   if (slots >= (int64_t(1) << kJBits)) { set_error("setup: more than 2^26 particle slots"); return SMG_ERR_CAPACITY; }
@@ -160,6 +172,7 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   n_ls_ = ls;
   d_jp_ = alloc_<JobParams>(njobs);
   d_tile_job_ = alloc_<int32_t>(slots / kTile);
+  d_sblk_job_ = alloc_<int32_t>((tbl + kScanItems - 1) / kScanItems + 1);
   d_red_ = alloc_<JobRed>(njobs);
   d_ls_ = alloc_<LsCell>(n_ls_);
   PState* S2[2] = {&A_, &B_};
@@ -195,10 +208,10 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   W_.ntrue = alloc_<int32_t>(cap_total_);
   W_.nbm = alloc_<uint64_t>(int64_t(kMaskWords) * cap_total_);
   W_.recb = alloc_<float4>(cap_total_);
-  // spill pool: by default 32 entries per slot beyond the inline K (the unrelaxed
+  // spill pool: by default 64 entries per slot beyond the inline K (the unrelaxed
   // random start and strong h gradients are where lists run long); a job owns the
   // part [p0 * sps, (p0 + cap) * sps) of it
-  sps_ = spill_per_slot > 0 ? spill_per_slot : 32;
+  sps_ = spill_per_slot > 0 ? spill_per_slot : 64;
   sps_ = int(std::min<int64_t>(sps_, ((int64_t(1) << 31) - 1) / std::max<int64_t>(cap_total_, 1)));
   if (sps_ < 1) { set_error("setup: too many slots for the spill pool"); return SMG_ERR_CAPACITY; }
   W_.spill_cap = int64_t(sps_) * cap_total_;
@@ -207,6 +220,7 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   if (alloc_failed_) { set_error("setup: device allocation failed (hipMalloc)"); return SMG_ERR_HIP; }
   W_.jp = d_jp_;
   W_.tile_job = d_tile_job_;
+  W_.sblk_job = d_sblk_job_;
   W_.red = d_red_;
   W_.ls = d_ls_;
   W_.cap_total = cap_total_;
@@ -216,6 +230,8 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   HIP_TRY(hipMemsetAsync(d_red_, 0, sizeof(JobRed) * njobs, st_));
   HIP_TRY(hipMemsetAsync(d_jp_, 0, sizeof(JobParams) * njobs, st_));   // n = 0: no job
   HIP_TRY(hipMemsetAsync(d_tile_job_, 0xFF, sizeof(int32_t) * (slots / kTile), st_));   // -1: no job
+  sblk_job_h_.assign((tbl + kScanItems - 1) / kScanItems + 1, -1);
+  HIP_TRY(hipMemsetAsync(d_sblk_job_, 0xFF, sizeof(int32_t) * sblk_job_h_.size(), st_));
   HIP_TRY(hipMemsetAsync(W_.counts, 0, sizeof(int32_t) * n_tbl_, st_));
   HIP_TRY(hipMemsetAsync(W_.starts, 0, sizeof(int32_t) * (n_tbl_ + 1), st_));
   HIP_TRY(hipMemsetAsync(W_.nbc, 0, sizeof(int32_t) * cap_total_, st_));
@@ -255,6 +271,7 @@ int Engine::upload_job_(int j, const JobInput& in) {
   const int64_t t0 = J.p0 / kTile, nt = J.cap / kTile;
   for (int64_t t = t0; t < t0 + nt; t++) tile_job_h_[t] = int32_t(j);
   HIP_TRY(hipMemcpyAsync(d_tile_job_ + t0, tile_job_h_.data() + t0, sizeof(int32_t) * nt, hipMemcpyHostToDevice, st_));
+  SMG_TRY(set_table_owner_(J, int32_t(j)));
   // the host staging buffer must outlive the copy
   HIP_TRY(hipStreamSynchronize(st_));
   return SMG_OK;
@@ -282,7 +299,7 @@ int Engine::setup(const std::vector<JobInput>& jobs, int max_neighbors, int spil
     JobParams& J = jp_[j];
     SMG_TRY(make_job_params(jobs[j], D_, J));
     J.tbl_base = tbl;
-    tbl += J.tbl_size;
+    tbl += table_blocks(J) * kScanItems;
     J.lset_off = ls;
     ls += job_lset_cells(J);
     J.p0 = p0;
@@ -348,7 +365,7 @@ int Engine::setup_pool(int D, int64_t max_slots, int64_t max_table, int64_t max_
   SMG_TRY(alloc_buffers_(slots, max_table, max_lset_cells, max_jobs, spill_per_slot));
   is_pool_ = true;
   tiles_free_.reset(slots / kTile);
-  tbl_free_.reset(max_table);
+  tbl_free_.reset(max_table / kScanItems);   // in 4096-entry scan blocks
   ls_free_.reset(max_lset_cells);
   ntiles_ = 0;
   W_.n_tbl = 0;
@@ -359,7 +376,7 @@ int Engine::setup_pool(int D, int64_t max_slots, int64_t max_table, int64_t max_
 
 void Engine::update_extent_() {
   ntiles_ = tiles_free_.high_water();
-  W_.n_tbl = tbl_free_.high_water();
+  W_.n_tbl = tbl_free_.high_water() * kScanItems;
 }
 
 // Seeds the job into its ranges, then Set_particle_info, sort, Pre_run and sort, all
@@ -375,17 +392,18 @@ int Engine::admit(const JobInput& in, int* job) {
   const int64_t nt = J.cap / kTile, nls = job_lset_cells(J);
   const int64_t t0 = tiles_free_.take(nt);
   if (t0 < 0) { set_error("admit: no free slot range"); return SMG_ERR_CAPACITY; }
-  const int64_t tb = tbl_free_.take(J.tbl_size + 1);   // + the scan's end sentinel
+  const int64_t tblocks = table_blocks(J);
+  const int64_t tb = tbl_free_.take(tblocks);
   if (tb < 0) { tiles_free_.give(t0, nt); set_error("admit: no free cell-table range"); return SMG_ERR_CAPACITY; }
   const int64_t lo = ls_free_.take(nls);
   if (lo < 0) {
     tiles_free_.give(t0, nt);
-    tbl_free_.give(tb, J.tbl_size + 1);
+    tbl_free_.give(tb, tblocks);
     set_error("admit: no free level-set range");
     return SMG_ERR_CAPACITY;
   }
   J.p0 = t0 * kTile;
-  J.tbl_base = tb;
+  J.tbl_base = tb * kScanItems;
   J.lset_off = lo;
   jp_[j] = J;
   active_[j] = 1;
@@ -395,7 +413,7 @@ int Engine::admit(const JobInput& in, int* job) {
   JobRed sorting{};
   sorting.rebuild = 1;
   HIP_TRY(hipMemcpyAsync(d_red_ + j, &sorting, sizeof(JobRed), hipMemcpyHostToDevice, st_));
-  HIP_TRY(hipMemsetAsync(W_.counts + tb, 0, sizeof(int32_t) * (J.tbl_size + 1), st_));
+  HIP_TRY(hipMemsetAsync(W_.counts + J.tbl_base, 0, sizeof(int32_t) * tblocks * kScanItems, st_));
   Work Ws = W_;
   Ws.tile0 = t0;
   launch_seed(st_, D_, nt, Ws, A_);
@@ -439,7 +457,7 @@ int Engine::sort_job_(int j) {
   Wt.counts = W_.counts + J.tbl_base;
   Wt.starts = W_.starts + J.tbl_base;
   Wt.n_tbl = J.tbl_size;
-  launch_scan(st_, Wt, false);
+  launch_scan(st_, Wt, false, false);
   launch_scatter(st_, nt, Ws);
   launch_segsort(st_, nt, Ws, A_);
   launch_gather(st_, D_, nt, Ws, A_, B_);
@@ -456,8 +474,9 @@ int Engine::retire(int j) {
   HIP_TRY(hipMemcpyAsync(d_tile_job_ + t0, tile_job_h_.data() + t0, sizeof(int32_t) * nt, hipMemcpyHostToDevice, st_));
   HIP_TRY(hipStreamSynchronize(st_));
   tiles_free_.give(t0, nt);
-  tbl_free_.give(J.tbl_base, J.tbl_size + 1);
+  tbl_free_.give(J.tbl_base / kScanItems, table_blocks(J));
   ls_free_.give(J.lset_off, job_lset_cells(J));
+  SMG_TRY(set_table_owner_(J, -1));
   active_[j] = 0;
   jp_[j].n = 0;
   HIP_TRY(hipMemcpyAsync(d_jp_ + j, &jp_[j], sizeof(JobParams), hipMemcpyHostToDevice, st_));
@@ -503,7 +522,7 @@ void Engine::end_() {
 int Engine::sort_particles() {
   launch_decide(st_, W_, int(jp_.size()), 1);
   launch_key(st_, D_, ntiles_, W_, A_);
-  launch_scan(st_, W_, false);
+  launch_scan(st_, W_, false, true);
   launch_scatter(st_, ntiles_, W_);
   launch_segsort(st_, ntiles_, W_, A_);
   launch_gather(st_, D_, ntiles_, W_, A_, B_);
@@ -611,7 +630,7 @@ int Engine::step(int64_t nsteps) {
     launch_key(st_, D_, ntiles_, W_, A_);
     end_();
     begin_("k_scan");
-    launch_scan(st_, W_, true);
+    launch_scan(st_, W_, true, true);
     end_();
     begin_("k_scatter");
     launch_scatter(st_, ntiles_, W_);
@@ -911,6 +930,7 @@ int Engine::get_stats(int job, smg_stats* st) {
     HIP_TRY(hipMemcpy(cnt.data(), W_.ntrue + J.p0, sizeof(int32_t) * J.cap, hipMemcpyDeviceToHost));
     for (int64_t s = 0; s < J.n; s++) st->n_neighbor_entries += cnt[s];
     st->n_rebuild = R.nrebuild;
+    st->drift = bits_to_double(R.drift);
   }
   return SMG_OK;
 }

@@ -49,6 +49,7 @@ class RangeSet {
 
 int make_job_params(const JobInput& in, int D, JobParams& J);
 int64_t job_lset_cells(const JobParams& J);
+int64_t table_blocks(const JobParams& J);
 
 class Engine {
  public:
@@ -111,6 +112,7 @@ class Engine {
   int alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, int spill_per_slot);
   int upload_job_(int job, const JobInput& in);
   void update_extent_();
+  int set_table_owner_(const JobParams& J, int32_t owner);
   void begin_(const char* name);
   void end_();
   int download_red_();
@@ -132,6 +134,8 @@ class Engine {
 
   JobParams* d_jp_ = nullptr;
   int32_t* d_tile_job_ = nullptr;
+  int32_t* d_sblk_job_ = nullptr;
+  std::vector<int32_t> sblk_job_h_;
   JobRed* d_red_ = nullptr;
   LsCell* d_ls_ = nullptr;
   PState A_{}, B_{};

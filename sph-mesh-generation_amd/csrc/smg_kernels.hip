@@ -597,10 +597,23 @@ __global__ __launch_bounds__(kTile) void k_key(Work W, PState S) {
 // ---------------------------------------------------------------------------
 // cell starts: exclusive scan of counts over all job tables
 
-// gate (may be NULL): skip the scan on a step where no job rebuilds
+// gate (may be NULL): skip the scan on a step where no job rebuilds.  blk_job (may be
+// NULL): the job owning each 4096-entry block (job table segments are block aligned);
+// the blocks of jobs that do not rebuild count as empty and keep their starts, which
+// stay valid because a job's starts are only ever read relative to its first entry.
+__device__ __forceinline__ bool scan_skip(const int32_t* blk_job, const JobRed* red) {
+  if (!blk_job) return false;
+  const int j = blk_job[blockIdx.x];
+  return j < 0 || !red[j].rebuild;
+}
 __global__ __launch_bounds__(kTile) void k_scan_reduce(const int32_t* __restrict__ counts, int64_t n, int32_t* bsum,
-                                                       const unsigned long long* gate) {
+                                                       const unsigned long long* gate, const int32_t* blk_job,
+                                                       const JobRed* red) {
   if (gate && !*gate) return;
+  if (scan_skip(blk_job, red)) {
+    if (threadIdx.x == 0) bsum[blockIdx.x] = 0;
+    return;
+  }
   __shared__ int lds[4];
   const int64_t base = int64_t(blockIdx.x) * kScanItems + int64_t(threadIdx.x) * 16;
   int s = 0;
@@ -635,8 +648,10 @@ __global__ __launch_bounds__(1024) void k_scan_top(int32_t* bsum, int64_t nb, co
 
 __global__ __launch_bounds__(kTile) void k_scan_down(const int32_t* __restrict__ counts, int64_t n,
                                                      const int32_t* __restrict__ bsum, int64_t nb, int32_t* starts,
-                                                     const unsigned long long* gate) {
+                                                     const unsigned long long* gate, const int32_t* blk_job,
+                                                     const JobRed* red) {
   if (gate && !*gate) return;
+  if (scan_skip(blk_job, red)) return;
   __shared__ int lds[kTile];
   const int64_t base = int64_t(blockIdx.x) * kScanItems + int64_t(threadIdx.x) * 16;
   int v[16];
@@ -747,6 +762,7 @@ __global__ __launch_bounds__(kTile) void k_copyback(Work W, PState A, PState B) 
   W.nu_b[d] = W.nu_a[d];
   W.rec[4 * int64_t(d) + 0] = make_float4(xh[0], xh[1], xh[2], float(h));
   W.rec[4 * int64_t(d) + 1] = make_float4(xl[0], xl[1], xl[2], __int_as_float(ty));
+  W.recb[d] = make_float4(xh[0], xh[1], xh[2], float(h));   // build position and h
   W.rec[4 * int64_t(d) + 3] = make_float4(nn[0], nn[1], nn[2], 0.f);
 }
 
@@ -822,7 +838,7 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
                 const int32_t lo = W.starts[base + int64_t(m << shift)] + soff;
                 const int32_t hi2 = W.starts[base + int64_t((m + 1) << shift)] + soff;
                 for (int32_t q = lo; q < hi2; q++) {
-                  const float4 qr = W.rec[4 * int64_t(q) + 0];
+                  const float4 qr = W.recb[q];   // == rec[0] at a build, 16-byte stride
                   const float qv[3] = {qr.x, qr.y, qr.z};
                   float d2 = 0.f;
 #pragma unroll
@@ -844,7 +860,7 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
 }
 
 // candidate lists of the rebuilding jobs: the first K entries inline, a slot with
-// more joins the spill queue; the build position and h are kept for the drift test
+// more joins the spill queue (k_copyback left the build positions and h in recb)
 template <int D>
 __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
   const int j = W.tile_job[W.tile0 + blockIdx.x];
@@ -863,8 +879,6 @@ __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
       const unsigned long long qi = atomicAdd(&W.ctr[0], 1ull);
       W.ovq[qi] = int32_t(slot);   // at most one entry per slot: qi < cap_total
     }
-    const float4 r0 = W.rec[4 * int64_t(slot) + 0];
-    W.recb[slot] = make_float4(r0.x, r0.y, r0.z, float(S.h[slot]));
   }
   W.nbc[slot] = cnt;
   if (threadIdx.x == 0) W.red[j].built = 1;
@@ -1335,13 +1349,14 @@ void launch_map(hipStream_t st, int D, int64_t ntiles, const Work& W, const PSta
   if (D == 2) k_map<2><<<ntiles, kTile, 0, st>>>(W, S, mode, par);
   else k_map<3><<<ntiles, kTile, 0, st>>>(W, S, mode, par);
 }
-void launch_scan(hipStream_t st, const Work& W, bool gated) {
+void launch_scan(hipStream_t st, const Work& W, bool gated, bool per_job) {
   const int64_t nb = (W.n_tbl + kScanItems - 1) / kScanItems;
   if (nb == 0) return;
   const unsigned long long* gate = gated ? W.ctr + 1 : nullptr;
-  k_scan_reduce<<<nb, kTile, 0, st>>>(W.counts, W.n_tbl, W.bsum, gate);
+  const int32_t* bj = per_job ? W.sblk_job : nullptr;
+  k_scan_reduce<<<nb, kTile, 0, st>>>(W.counts, W.n_tbl, W.bsum, gate, bj, W.red);
   k_scan_top<<<1, 1024, 0, st>>>(W.bsum, nb, gate);
-  k_scan_down<<<nb, kTile, 0, st>>>(W.counts, W.n_tbl, W.bsum, nb, W.starts, gate);
+  k_scan_down<<<nb, kTile, 0, st>>>(W.counts, W.n_tbl, W.bsum, nb, W.starts, gate, bj, W.red);
 }
 void launch_scatter(hipStream_t st, int64_t ntiles, const Work& W) {
   k_scatter<<<ntiles, kTile, 0, st>>>(W);

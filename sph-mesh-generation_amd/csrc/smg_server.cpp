@@ -54,13 +54,14 @@ struct smg_server {
   std::map<int32_t, Result> results;     // finished, not yet collected
   std::deque<smg_server_job> finished;   // not yet reported by smg_server_finished
   int32_t next_ticket = 0;
+  int64_t active_particles = 0;
 };
 
 extern "C" {
 
 int smg_server_footprint_of(const smg_server_config* c, int64_t* bytes) {
   if (!c || !bytes) { set_error("smg_server_footprint_of: NULL argument"); return SMG_ERR_ARG; }
-  const Sizes z = sizes_for(c->max_neighbors > 0 ? c->max_neighbors : 128, c->spill_per_slot > 0 ? c->spill_per_slot : 32);
+  const Sizes z = sizes_for(c->max_neighbors > 0 ? c->max_neighbors : 128, c->spill_per_slot > 0 ? c->spill_per_slot : 64);
   *bytes = c->max_slots * z.per_slot + c->max_table * z.per_table + c->max_lset_cells * z.per_cell +
            int64_t(c->max_jobs) * z.per_job;
   return SMG_OK;
@@ -83,7 +84,7 @@ int smg_server_job_size(const smg_config* cfg, int64_t n, int64_t* slots, int64_
   const int rc = smg::make_job_params(in, cfg->dim, J);
   if (rc != SMG_OK) return rc;
   if (slots) *slots = J.cap;
-  if (table) *table = J.tbl_size + 1;
+  if (table) *table = smg::table_blocks(J) * smg::kScanItems;
   if (lset_cells) *lset_cells = smg::job_lset_cells(J);
   return SMG_OK;
 }
@@ -115,6 +116,11 @@ int smg_server_admit(smg_server* s, const smg_config* cfg, const double* phi, co
     return SMG_ERR_ARG;
   }
   if (cfg->dim != s->cfg.dim) { set_error("smg_server_admit: job dim differs from the server's"); return SMG_ERR_ARG; }
+  if (s->cfg.max_active_particles > 0 && !s->by_engine.empty() &&
+      s->active_particles + n > s->cfg.max_active_particles) {
+    set_error("smg_server_admit: the pool's particle limit is reached");
+    return SMG_ERR_CAPACITY;
+  }
   smg::JobInput in;
   in.cfg = *cfg;
   in.n = n;
@@ -133,6 +139,7 @@ int smg_server_admit(smg_server* s, const smg_config* cfg, const double* phi, co
   Active a;
   a.ticket = s->next_ticket++;
   s->by_engine[j] = a;
+  s->active_particles += n;
   *ticket = a.ticket;
   return SMG_OK;
 }
@@ -181,6 +188,7 @@ int smg_server_advance(smg_server* s, int32_t* n_active) {
       done.push_back(j);
     }
     for (int j : done) {
+      s->active_particles -= s->eng.params(j).n;
       rc = s->eng.retire(j);
       if (rc != SMG_OK) return rc;
       s->by_engine.erase(j);

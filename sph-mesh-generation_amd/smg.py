@@ -53,7 +53,7 @@ class SmgStats(C.Structure):
         ("max_neighbors_seen", C.c_int32), ("num_level", C.c_int32),
         ("dt1", C.c_double), ("dt2", C.c_double), ("max_v", C.c_double),
         ("residual", C.c_double), ("max_disp", C.c_double), ("res_max", C.c_double),
-        ("n_rebuild", C.c_int64),
+        ("n_rebuild", C.c_int64), ("drift", C.c_double),
     ]
 
     def as_dict(self):
@@ -416,7 +416,7 @@ class SmgServerConfig(C.Structure):
         ("dim", C.c_int32), ("max_jobs", C.c_int32), ("max_slots", C.c_int64), ("max_table", C.c_int64),
         ("max_lset_cells", C.c_int64), ("max_neighbors", C.c_int32), ("spill_per_slot", C.c_int32),
         ("chunk_steps", C.c_int32), ("min_steps", C.c_int32), ("max_steps", C.c_int32), ("reserved", C.c_int32),
-        ("tol", C.c_double),
+        ("tol", C.c_double), ("max_active_particles", C.c_int64),
     ]
 
 
@@ -443,9 +443,9 @@ def _server_sigs(L):
 
 
 def server_config(dim, max_jobs, max_slots, max_table, max_lset_cells, chunk_steps=50, min_steps=100,
-                  max_steps=3000, tol=1e-3, max_neighbors=0, spill_per_slot=0):
+                  max_steps=3000, tol=1e-3, max_neighbors=0, spill_per_slot=0, max_active_particles=0):
     return SmgServerConfig(dim, max_jobs, int(max_slots), int(max_table), int(max_lset_cells), max_neighbors,
-                           spill_per_slot, chunk_steps, min_steps, max_steps, 0, tol)
+                           spill_per_slot, chunk_steps, min_steps, max_steps, 0, tol, int(max_active_particles))
 
 
 def server_footprint(scfg):

@@ -112,3 +112,19 @@ def test_admission_by_capacity(smg_mod, jobs3):
         r = srv.result(t)
         assert np.isfinite(r["x"]).all()
     srv.close()
+
+
+def test_admission_by_particle_limit(smg_mod, jobs3):
+    """With a particle limit the pool admits a job only while the admitted particles
+    stay within it (an empty pool admits any job)."""
+    n0, n1 = jobs3[0][2], jobs3[1][2]
+    srv = _server(smg_mod, 3, max_active_particles=n0 + n1 - 1, max_steps=2 * CHUNK, min_steps=2 * CHUNK,
+                  tol=0.0)
+    t0 = srv.admit(*jobs3[0])
+    assert t0 is not None
+    assert srv.admit(*jobs3[1]) is None
+    srv.advance()
+    srv.advance()
+    assert [f["ticket"] for f in srv.finished()] == [t0]
+    assert srv.admit(*jobs3[1]) is not None
+    srv.close()
