@@ -40,7 +40,25 @@ def build(verbose=False):
     cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_driver(verbose)
     return LIB
+
+
+DRIVER = os.path.join(HERE, "lib", "smg_run")
+
+
+def build_driver(verbose=False):
+    """The C++ host driver (host/smg_run.cpp over the C ABI), plain g++, linked to
+    lib/libsmg.so with an rpath so that it runs from the tree."""
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host"),
+           os.path.join(HERE, "host", "smg_run.cpp"), "-o", DRIVER + ".tmp",
+           "-L", os.path.dirname(LIB), "-lsmg", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -195,10 +195,7 @@ int smg_step(smg_job* job, int64_t nsteps) {
 
 int smg_get_timestep(smg_job* job, double* dt) {
   if (!job || !job->eng || !dt) { set_error("smg_get_timestep: bad argument"); return SMG_ERR_STATE; }
-  smg_stats st;
-  const int rc = job->eng->get_stats(0, &st);
-  *dt = st.dt2;
-  return rc;
+  return job->eng->timestep(0, dt);
 }
 
 int smg_get_stats(smg_job* job, smg_stats* st) {

@@ -898,6 +898,16 @@ int Engine::get_neighbors(int job, int64_t* offs, int32_t* ids, int32_t* codes, 
   return SMG_OK;
 }
 
+// glbl_timestep after the last step's Set_timestep(2) (0 before the first step)
+int Engine::timestep(int job, double* dt) {
+  *dt = 0.;
+  if (step_ == 0) return SMG_OK;
+  JobRed r;
+  SMG_TRY(red_of(job, &r));
+  *dt = bits_to_double(r.dt2[step_ & 1]);
+  return SMG_OK;
+}
+
 int Engine::get_stats(int job, smg_stats* st) {
   std::memset(st, 0, sizeof(*st));
   const int par = int(step_ & 1);

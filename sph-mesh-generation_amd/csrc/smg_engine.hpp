@@ -87,6 +87,7 @@ class Engine {
   int get_pair_terms(int job, float* sum, float* a1, float* a2);
   int get_neighbors(int job, int64_t* offs, int32_t* ids, int32_t* codes, int64_t cap);
   int get_stats(int job, smg_stats* st);
+  int timestep(int job, double* dt);
   int check_health();   // remap failures, neighbour overflow
 
   void profile_enable(bool on);

@@ -1,0 +1,160 @@
+// smg_solver.hpp -- C++ host mirror of the reference's solver interface over the C ABI
+// of include/smg.h, single rank (the reference's communicator argument is dropped).
+//
+//   Initialization        system.h:9-20, system.cpp:12-48
+//   SPH_mesh_generation   the SOLVER of glbparam.h:23-25: Initialize_case, Set_current_time,
+//                         SPH_run_mesh_generation, Get_timestep, Post_processing,
+//                         Output_simulation_infor (sph_mesh_generation.h)
+//   Solver                solver.h:28-45: Load_solver (solver.cpp:35-78) and
+//                         SPH_Mesh_Generation_Solver (solver.cpp:257-321)
+//
+// Error behaviour follows the reference: a failure prints a message and ends the
+// process with a non-zero status (the reference calls world.abort / exit).
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "smg.h"
+
+namespace smg_host {
+
+typedef double Real;
+
+inline void check(int rc, const char* where) {
+  if (rc != SMG_OK) {
+    std::fprintf(stderr, "ERROR in %s (%d): %s\n", where, rc, smg_last_error());
+    std::exit(1);
+  }
+}
+
+// system parameters for global control (system.h:9-20)
+class Initialization {
+ public:
+  Real start_time = 0.0, end_time = 20000.0;
+  Real output_dt = 100.0;
+  int num_output = 0;
+  Initialization(Real start = 0.0, Real end = 20000.0, Real dt = 100.0)
+      : start_time(start), end_time(end), output_dt(dt) {
+    num_output = int(std::ceil((end_time - start_time) / output_dt));
+  }
+  Real Get_start_time() const { return start_time; }
+  Real Get_dt() const { return output_dt; }
+  int Get_num_timestep() const { return num_output; }
+};
+
+// The level set the case runs on: cell-centred phi with a full margin package per
+// side, from which smg_lset_build derives normals, cut-cell tags and the particle count.
+struct LevelSetCase {
+  smg_config cfg{};
+  std::vector<double> phi_ext;
+  int zero_margin = 1;
+};
+
+// SPH_mesh_generation as the Solver drives it (one relaxation job on the GPU)
+class SPH_mesh_generation {
+ public:
+  int i_iter = 0;
+  Real run_time = 0.0;
+  Real error_global = 0.0;   // the relaxation residual (mean |v| dt2 / dx)
+  int64_t total_num_particle = 0;
+
+  ~SPH_mesh_generation() {
+    if (job_) smg_destroy(job_);
+  }
+
+  // Initialize_case (sph_mesh_generation.cpp:594): level set, particle budget
+  // (Get_num_particles), Random_particle_distribution and Pre_run
+  void Initialize_case(const LevelSetCase& c) {
+    cfg_ = c.cfg;
+    int64_t m = 1;
+    for (int a = 0; a < cfg_.dim; a++) m *= 4 * int64_t(cfg_.num_pkg[a]) + 2;
+    phi_.assign(m, 0.);
+    normal_.assign(3 * m, 0.f);
+    tag_.assign(m, 0);
+    check(smg_lset_build(&cfg_, c.phi_ext.data(), c.zero_margin, phi_.data(), normal_.data(), tag_.data(), &lset_),
+          "smg_lset_build");
+    total_num_particle = lset_.num_particle;
+    check(smg_create(&cfg_, &job_), "smg_create");
+    check(smg_set_levelset(job_, phi_.data(), normal_.data(), tag_.data()), "smg_set_levelset");
+    check(smg_seed_particles(job_, total_num_particle), "smg_seed_particles");
+  }
+  void Set_current_time(Real t) { run_time = t; }
+  // one SPH_run_mesh_generation (sph_mesh_generation.cpp:1371-1463)
+  void SPH_run_mesh_generation() {
+    check(smg_step(job_, 1), "smg_step");
+  }
+  Real Get_timestep() {
+    Real dt = 0.;
+    check(smg_get_timestep(job_, &dt), "smg_get_timestep");
+    return dt;
+  }
+  // Post_processing: the final particle state, by particle id
+  void Post_processing(int n, std::vector<double>& x, std::vector<int32_t>& type) {
+    (void)n;
+    x.assign(3 * total_num_particle, 0.);
+    type.assign(total_num_particle, 0);
+    check(smg_get_state(job_, x.data(), nullptr, type.data(), nullptr, nullptr), "smg_get_state");
+  }
+  // Output_simulation_infor: step count, dt, residual, particle counts
+  smg_stats Output_simulation_infor() {
+    smg_stats st{};
+    check(smg_get_stats(job_, &st), "smg_get_stats");
+    error_global = st.residual;
+    return st;
+  }
+  const smg_lset_stats& lset_stats() const { return lset_; }
+  smg_job* handle() { return job_; }
+
+ private:
+  smg_config cfg_{};
+  smg_lset_stats lset_{};
+  std::vector<double> phi_;
+  std::vector<float> normal_;
+  std::vector<int8_t> tag_;
+  smg_job* job_ = nullptr;
+};
+
+class Solver {
+ public:
+  int total_iterate_num = 0;
+  Real output_dt = 0.0, timestep = 0.0, run_time = 0.0, integral_time = 0.0;
+  SPH_mesh_generation current_solver;
+
+  explicit Solver(const Initialization& Ini) : total_iterate_num(Ini.num_output), output_dt(Ini.output_dt) {}
+
+  void Set_current_time(Real t) { run_time = t; }
+
+  // Load_solver (solver.cpp:35-78): Initialize_case, then num_output rounds of
+  // SPH_Mesh_Generation_Solver; on_output is called after each round
+  template <class OnOutput>
+  void Load_solver(const Initialization& Ini, const LevelSetCase& c, OnOutput on_output) {
+    current_solver.Initialize_case(c);
+    Real Time = Ini.Get_start_time();
+    for (int n = 0; n < total_iterate_num; n++) {
+      Set_current_time(Time);
+      SPH_Mesh_Generation_Solver(Ini, n);
+      on_output(n, current_solver);
+      Time += Ini.Get_dt();
+    }
+  }
+
+  // SPH_Mesh_Generation_Solver (solver.cpp:257-321): output_dt relaxation steps, one
+  // per unit of integral time
+  void SPH_Mesh_Generation_Solver(const Initialization& Ini, int n) {
+    (void)Ini;
+    integral_time = 0.0;
+    current_solver.i_iter = n;
+    while (integral_time < output_dt) {
+      integral_time += 1.;
+      current_solver.Set_current_time(run_time + integral_time);
+      current_solver.SPH_run_mesh_generation();
+      timestep = current_solver.Get_timestep();
+    }
+  }
+};
+
+}  // namespace smg_host
