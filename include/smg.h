@@ -201,6 +201,40 @@ int smg_batch_sync(smg_batch* batch);
 int smg_batch_get_stats(smg_batch* batch, int32_t job, smg_stats* st);
 int smg_batch_get_state(smg_batch* batch, int32_t job, double* x, double* v, int32_t* type, double* h, float* nu);
 
+/* ---- 2D triangulation (Graphmeshcls::Reconstruct_tris, graph_mesh.cpp:37-407) ----
+ * Edges: the Voronoi faces of each particle's cell among its neighbours, inside the box
+ * x +- 1.5 cut_off h (Voronoi_construction::Local_VD_generation,
+ * voronoi_construction.cpp:6-127), on the GPU in FP64; a mirror image stands for its
+ * source particle.  Then the reference's integer stage: all 3-cliques (i > j > k);
+ * a triangle of three non-REAL particles whose centroid phi is <= 1e-4 of their mean
+ * phi loses its longest edge; the graph is rebuilt from the surviving triangles and
+ * its 3-cliques are the mesh.  Quality statistics follow Tri_quality_statistics
+ * (graph_mesh.cpp:411-671). */
+typedef struct smg_tri_stats {
+  int64_t n_tri, n_edge;           /* final triangles, edges of the Voronoi stage        */
+  int64_t n_deleted_edge;          /* edges removed by the centroid test                */
+  int64_t n_cell_overflow;         /* cells that needed more vertices / faces than kept */
+  double min_angle, max_angle;     /* degrees                                            */
+  double avg_min_angle;
+  double min_edge, max_edge;
+  double min_area, max_area, total_area;
+  double max_aspect;               /* longest / shortest edge                            */
+  double G_min, G_avg;             /* 2 sqrt(3) area / (longest edge * half perimeter)   */
+  int64_t n_min_angle_below[4];    /* triangles with min angle < 10, 20, 30, 40 degrees  */
+  int32_t angle_hist[45];          /* angles in 4-degree bins                            */
+  int32_t reserved;
+} smg_tri_stats;
+
+/* Triangulates a 2D job's current particles.  tris: 3 particle ids per triangle
+ * (i > j > k), may be NULL to query the count; capacity in triangles. */
+int smg_triangulate(smg_job* job, int32_t* tris, int64_t capacity, int64_t* n_tri, smg_tri_stats* stats);
+/* The Voronoi-stage edges (particle ids, i > j, sorted) of the last smg_triangulate. */
+int smg_tri_edges(smg_job* job, int32_t* edges, int64_t capacity, int64_t* n_edges);
+/* The integer stage alone (host only): all 3-cliques (i > j > k) of an undirected
+ * graph given as n_edges pairs (self loops and repeats ignored), sorted. */
+int smg_tri_cliques(int64_t n_vertices, const int32_t* edges, int64_t n_edges, int32_t* tris, int64_t capacity,
+                    int64_t* n_tri);
+
 /* ---- serving: continuous batching over a device-resident job pool ----
  * No counterpart in the reference.  A server owns one pool of particle slots, cell
  * tables and level-set cells on the device (its HBM footprint, fixed at creation).

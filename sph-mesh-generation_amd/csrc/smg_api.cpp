@@ -2,6 +2,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -409,6 +410,34 @@ int smg_batch_get_stats(smg_batch* batch, int32_t job, smg_stats* st) {
 int smg_batch_get_state(smg_batch* batch, int32_t job, double* x, double* v, int32_t* type, double* h, float* nu) {
   if (!batch || job < 0 || job >= int32_t(batch->dim_of.size())) { set_error("smg_batch_get_state: bad argument"); return SMG_ERR_ARG; }
   return batch->eng[batch->dim_of[job]]->get_state(batch->local_of[job], x, v, type, h, nu);
+}
+
+int smg_triangulate(smg_job* job, int32_t* tris, int64_t capacity, int64_t* n_tri, smg_tri_stats* stats) {
+  if (!job || !job->eng || !n_tri) { set_error("smg_triangulate: bad argument"); return SMG_ERR_STATE; }
+  std::vector<std::array<int32_t, 3>> t;
+  smg_tri_stats st;
+  const int rc = job->eng->triangulate(0, t, &st);
+  if (rc != SMG_OK) return rc;
+  *n_tri = int64_t(t.size());
+  if (stats) *stats = st;
+  if (!tris) return SMG_OK;
+  if (int64_t(t.size()) > capacity) { set_error("smg_triangulate: capacity too small"); return SMG_ERR_CAPACITY; }
+  for (size_t q = 0; q < t.size(); q++)
+    for (int a = 0; a < 3; a++) tris[3 * q + a] = t[q][a];
+  return SMG_OK;
+}
+
+int smg_tri_edges(smg_job* job, int32_t* edges, int64_t capacity, int64_t* n_edges) {
+  if (!job || !job->eng || !n_edges) { set_error("smg_tri_edges: bad argument"); return SMG_ERR_STATE; }
+  const auto& e = job->eng->last_edges();
+  *n_edges = int64_t(e.size());
+  if (!edges) return SMG_OK;
+  if (int64_t(e.size()) > capacity) { set_error("smg_tri_edges: capacity too small"); return SMG_ERR_CAPACITY; }
+  for (size_t q = 0; q < e.size(); q++) {
+    edges[2 * q] = e[q][0];
+    edges[2 * q + 1] = e[q][1];
+  }
+  return SMG_OK;
 }
 
 int smg_fp64_probe(int64_t n, const double* a, const double* b, double* out) {

@@ -6,6 +6,7 @@
 #pragma once
 #include <hip/hip_runtime_api.h>
 #include <cstdint>
+#include <array>
 #include <map>
 #include <string>
 #include <vector>
@@ -88,6 +89,9 @@ class Engine {
   int get_neighbors(int job, int64_t* offs, int32_t* ids, int32_t* codes, int64_t cap);
   int get_stats(int job, smg_stats* st);
   int timestep(int job, double* dt);
+  int triangulate(int job, std::vector<std::array<int32_t, 3>>& tris, smg_tri_stats* st);   // smg_tri.cpp
+  // Voronoi-stage edges (i > j) of the last triangulate call
+  const std::vector<std::array<int32_t, 2>>& last_edges() const { return last_edges_; }
   int check_health();   // remap failures, neighbour overflow
 
   void profile_enable(bool on);
@@ -122,6 +126,7 @@ class Engine {
   hipStream_t st_ = nullptr;
   bool own_stream_ = false;
   std::vector<JobParams> jp_;
+  std::vector<std::array<int32_t, 2>> last_edges_;
   std::vector<uint8_t> active_;
   bool is_pool_ = false;
   bool alloc_failed_ = false;

@@ -1257,6 +1257,127 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// 2D triangulation, geometric stage: the Voronoi cell of each particle among its
+// neighbours (Voronoi_construction::Local_VD_generation, voronoi_construction.cpp:6-127)
+// inside the box x_i +- 1.5 cut_off h_i that the reference gives voro++, built by
+// clipping the box with the bisector half-planes in FP64.  Each cell edge that a
+// neighbour's bisector made is a Voronoi face: the particle id of that neighbour (a
+// mirror image counts as its source particle, as the reference's ghosts carry their
+// source's local_id) goes to out_nb[loc * KE + c].  out_cnt[loc] = number of faces, or
+// -1 when the cell needs more than kVorVerts vertices or KE faces.
+
+constexpr int kVorVerts = 48;
+
+template <int D>
+__global__ __launch_bounds__(kTile) void k_vor2d(Work W, PState S, int32_t* out_nb, int32_t* out_cnt, int KE) {
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0) return;
+  const JobParams& J = W.jp[j];
+  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t loc = slot - J.p0;
+  if (loc >= J.n) return;
+  const float4 r0 = W.rec[4 * slot + 0], r1 = W.rec[4 * slot + 1];
+  const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
+  const double xi = double(xhi[0]) + double(xli[0]), yi = double(xhi[1]) + double(xli[1]);
+  const double hd = S.h[slot];
+  const float rfi = float(hd * J.cut_off);
+  const double sc = hd * J.cut_off * 1.5;
+  double px[kVorVerts], py[kVorVerts];
+  int lab[kVorVerts];   // label of the edge from vertex v to v+1: -1 box wall, else list entry
+  int nv = 4;
+  px[0] = xi - sc; py[0] = yi - sc; lab[0] = -1;
+  px[1] = xi + sc; py[1] = yi - sc; lab[1] = -1;
+  px[2] = xi + sc; py[2] = yi + sc; lab[2] = -1;
+  px[3] = xi - sc; py[3] = yi + sc; lab[3] = -1;
+  bool bad = false;
+  const int cnt = W.nbc[slot];
+  const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
+  for (int k = 0; k < cnt && !bad; k++) {
+    const uint32_t e = nb_entry(W, slot, k, off);
+    const int64_t q = e & kJMask;
+    const uint32_t code = e >> kJBits;
+    if (q == slot && code == 0) continue;
+    const float4 qh = W.rec[4 * q + 0], ql = W.rec[4 * q + 1];
+    float dr[3] = {0.f, 0.f, 0.f};
+    rel_pos<D>(J, qh, ql, code, xhi, xli, dr);
+    float d2 = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; a++) d2 += dr[a] * dr[a];
+    if (!is_neighbor<D>(J, W, S, q, code, d2, rfi, qh.w, xhi, xli, hd)) continue;
+    const double xq[3] = {double(qh.x) + double(ql.x), double(qh.y) + double(ql.y), 0.};
+    double g[3];
+    image_of<D>(J, code, xq, g);
+    // keep the side closer to x_i: f(p) = (p - x_i).(g - x_i) - |g - x_i|^2 / 2 <= 0
+    const double ux = g[0] - xi, uy = g[1] - yi;
+    const double c = 0.5 * (ux * ux + uy * uy);
+    double f[kVorVerts];
+    bool any_out = false, any_in = false;
+    for (int v = 0; v < nv; v++) {
+      f[v] = (px[v] - xi) * ux + (py[v] - yi) * uy - c;
+      if (f[v] > 0.) any_out = true;
+      else any_in = true;
+    }
+    if (!any_out) continue;        // the bisector misses the cell
+    if (!any_in) { nv = 0; break; }   // cannot happen for a neighbour other than a coincident point
+    double nx[kVorVerts], ny[kVorVerts];
+    int nl[kVorVerts];
+    int m = 0;
+    for (int v = 0; v < nv && !bad; v++) {
+      const int w = v + 1 < nv ? v + 1 : 0;
+      const bool vin = f[v] <= 0., win = f[w] <= 0.;
+      if (vin) {
+        if (m >= kVorVerts) { bad = true; break; }
+        nx[m] = px[v]; ny[m] = py[v];
+        nl[m] = lab[v];   // the edge from v (to w or to the cut point) keeps its label
+        m++;
+      }
+      if (vin != win) {   // the edge crosses the bisector: the cut point starts a new edge
+        const double t = f[v] / (f[v] - f[w]);
+        if (m >= kVorVerts) { bad = true; break; }
+        nx[m] = px[v] + t * (px[w] - px[v]);
+        ny[m] = py[v] + t * (py[w] - py[v]);
+        // leaving the kept side: the new edge runs along the bisector (label k);
+        // entering it: the edge continues the old edge v -> w
+        nl[m] = vin ? k : lab[v];
+        m++;
+      }
+    }
+    for (int v = 0; v < m; v++) {
+      px[v] = nx[v]; py[v] = ny[v]; lab[v] = nl[v];
+    }
+    nv = m;
+  }
+  int nf = 0;
+  if (!bad) {
+    for (int v = 0; v < nv; v++) {
+      if (lab[v] < 0) continue;
+      const uint32_t e = nb_entry(W, slot, lab[v], off);
+      const int32_t src = S.id[e & kJMask];
+      if (src == S.id[slot]) continue;   // the cell face with the particle's own mirror image
+      bool seen = false;
+      for (int u = 0; u < nf; u++) seen = seen || out_nb[loc * KE + u] == src;
+      if (seen) continue;
+      if (nf >= KE) { bad = true; break; }
+      out_nb[loc * KE + nf] = src;
+      nf++;
+    }
+  }
+  out_cnt[loc] = bad ? -1 : nf;
+}
+
+// FP64 level-set phi at arbitrary points of one job (Calculate_particle_infor)
+template <int D>
+__global__ void k_sample_phi(Work W, int job, int64_t n, const double* pts, double* phi) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const JobParams& J = W.jp[job];
+  const double x[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+  double p, nn[3];
+  ls_sample<D>(J, W.ls, x, p, nn);
+  phi[i] = p;
+}
+
 // residual of the last step without moving anything (for statistics); the same
 // per-particle arithmetic as the residual that the next k_map accumulates
 template <int D>
@@ -1389,6 +1510,16 @@ void launch_residual(hipStream_t st, int D, int64_t ntiles, const Work& W, const
   k_clear_res<<<(njobs + 255) / 256, 256, 0, st>>>(W.red, njobs, par);
   if (D == 2) k_residual<2><<<ntiles, kTile, 0, st>>>(W, S, par);
   else k_residual<3><<<ntiles, kTile, 0, st>>>(W, S, par);
+}
+void launch_vor2d(hipStream_t st, int64_t ntiles, const Work& W, const PState& S, int32_t* out_nb, int32_t* out_cnt,
+                  int KE) {
+  k_vor2d<2><<<ntiles, kTile, 0, st>>>(W, S, out_nb, out_cnt, KE);
+}
+void launch_sample_phi(hipStream_t st, int D, const Work& W, int job, int64_t n, const double* pts, double* phi) {
+  if (n <= 0) return;
+  const int64_t g = (n + 255) / 256;
+  if (D == 2) k_sample_phi<2><<<g, 256, 0, st>>>(W, job, n, pts, phi);
+  else k_sample_phi<3><<<g, 256, 0, st>>>(W, job, n, pts, phi);
 }
 void launch_fp64_probe(hipStream_t st, int64_t n, const double* a, const double* b, double* out) {
   k_fp64_probe<<<(n + 255) / 256, 256, 0, st>>>(n, a, b, out);

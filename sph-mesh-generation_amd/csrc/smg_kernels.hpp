@@ -27,6 +27,9 @@ void launch_nb_spill(hipStream_t st, int D, int64_t ntiles, const Work& W, const
 void launch_force1(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int par);
 void launch_force2(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int par);
 void launch_residual(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int njobs, int par);
+void launch_vor2d(hipStream_t st, int64_t ntiles, const Work& W, const PState& S, int32_t* out_nb, int32_t* out_cnt,
+                  int KE);
+void launch_sample_phi(hipStream_t st, int D, const Work& W, int job, int64_t n, const double* pts, double* phi);
 void launch_fp64_probe(hipStream_t st, int64_t n, const double* a, const double* b, double* out);
 
 }  // namespace smg

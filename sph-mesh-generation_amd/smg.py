@@ -411,6 +411,62 @@ class Batch:
 # ---------------------------------------------------------------------------
 # serving: continuous batching over a device-resident pool (smg_server_*)
 
+class SmgTriStats(C.Structure):
+    _fields_ = [
+        ("n_tri", C.c_int64), ("n_edge", C.c_int64), ("n_deleted_edge", C.c_int64), ("n_cell_overflow", C.c_int64),
+        ("min_angle", C.c_double), ("max_angle", C.c_double), ("avg_min_angle", C.c_double),
+        ("min_edge", C.c_double), ("max_edge", C.c_double),
+        ("min_area", C.c_double), ("max_area", C.c_double), ("total_area", C.c_double),
+        ("max_aspect", C.c_double), ("G_min", C.c_double), ("G_avg", C.c_double),
+        ("n_min_angle_below", C.c_int64 * 4), ("angle_hist", C.c_int32 * 45), ("reserved", C.c_int32),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        d["n_min_angle_below"] = list(self.n_min_angle_below)
+        d["angle_hist"] = list(self.angle_hist)
+        return d
+
+
+def _tri_sigs(L):
+    P = C.POINTER
+    L.smg_triangulate.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, P(C.c_int64), P(SmgTriStats)]
+    L.smg_tri_cliques.argtypes = [C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, P(C.c_int64)]
+    L.smg_tri_edges.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, P(C.c_int64)]
+
+
+def tri_edges(job):
+    """Voronoi-stage edges (i > j) of the job's last triangulate() call."""
+    L = lib(); _tri_sigs(L)
+    m = C.c_int64()
+    _check(L.smg_tri_edges(job.h, None, 0, C.byref(m)), "smg_tri_edges")
+    e = np.zeros((max(m.value, 1), 2), np.int32)
+    _check(L.smg_tri_edges(job.h, _ptr(e), m.value, C.byref(m)), "smg_tri_edges")
+    return e[:m.value]
+
+
+def triangulate(job):
+    """(tris (m x 3 particle ids, i > j > k), stats) of a 2D job's current particles."""
+    L = lib(); _tri_sigs(L)
+    m = C.c_int64()
+    st = SmgTriStats()
+    _check(L.smg_triangulate(job.h, None, 0, C.byref(m), C.byref(st)), "smg_triangulate")
+    tris = np.zeros((max(m.value, 1), 3), np.int32)
+    _check(L.smg_triangulate(job.h, _ptr(tris), m.value, C.byref(m), C.byref(st)), "smg_triangulate")
+    return tris[:m.value], st.as_dict()
+
+
+def tri_cliques(n_vertices, edges):
+    """All 3-cliques (i > j > k) of an undirected graph (host-only integer stage)."""
+    L = lib(); _tri_sigs(L)
+    e = np.ascontiguousarray(edges, np.int32).reshape(-1, 2)
+    m = C.c_int64()
+    _check(L.smg_tri_cliques(int(n_vertices), _ptr(e), len(e), None, 0, C.byref(m)), "smg_tri_cliques")
+    out = np.zeros((max(m.value, 1), 3), np.int32)
+    _check(L.smg_tri_cliques(int(n_vertices), _ptr(e), len(e), _ptr(out), m.value, C.byref(m)), "smg_tri_cliques")
+    return out[:m.value]
+
+
 class SmgServerConfig(C.Structure):
     _fields_ = [
         ("dim", C.c_int32), ("max_jobs", C.c_int32), ("max_slots", C.c_int64), ("max_table", C.c_int64),
