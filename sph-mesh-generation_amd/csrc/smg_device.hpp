@@ -16,6 +16,9 @@ constexpr int kJBits = 26;       // neighbour entry = slot | (mirror code << 26)
 constexpr uint32_t kJMask = (1u << kJBits) - 1u;
 constexpr int kScanItems = 4096; // table entries per scan workgroup (256 threads x 16)
 constexpr int kMaskWords = 2;    // stage-2 entry masks cover the first 128 list entries
+constexpr int kJumpMax = 64;     // particles per job that may outrun their lists between builds
+constexpr int kJumpList = 512;   // candidate entries of one jumper's own list
+constexpr int kJumpBit = 1 << 30;   // flag in the type bits of a record: the slot is a jumper
 
 // one level-set cell: phi (FP64, used for projection), raw normal, interface tag
 struct alignas(8) LsCell {
@@ -70,6 +73,8 @@ struct JobRed {
   int32_t fail, overflow, maxnb, nsurf;
   int32_t rebuild;                 // this step re-sorts and rebuilds the lists
   int32_t built;                   // the lists match the current slot order
+  int32_t njump;                   // jumpers since the last build (jset holds the first kJumpMax)
+  int32_t jover;                   // more jumpers than kJumpMax: rebuild
 };
 
 // one buffer set of the fields that follow the particles through the sort
@@ -114,7 +119,10 @@ struct Work {
   int32_t* ovq;     // slots whose lists spill, appended by the list pass
   int32_t* ntrue;   // neighbours within the reference's cutoff (the force passes' count)
   uint64_t* nbm;    // stage-2 entry masks from the stage-1 pass, [w * cap_total + slot]
-  float4* recb;     // position (hi parts) and h at the last build: (x, y, z, h)
+  float4* recb;     // position (hi parts) and h at the last build: (x, y, z, h); h < 0 marks a jumper
+  int32_t* jset;    // per job, the jumper slots (kJumpMax each), sorted by slot at each step
+  int32_t* jl;      // per job and jumper index, that jumper's candidate list (kJumpList entries)
+  int32_t* jcnt;    // its length
   unsigned long long* ctr;   // [0] = ovq length
   int64_t spill_cap;
   int64_t tile0;   // first tile of a launch (launches over one job's tiles start there)

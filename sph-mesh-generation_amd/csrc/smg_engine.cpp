@@ -208,6 +208,9 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   W_.ntrue = alloc_<int32_t>(cap_total_);
   W_.nbm = alloc_<uint64_t>(int64_t(kMaskWords) * cap_total_);
   W_.recb = alloc_<float4>(cap_total_);
+  W_.jset = alloc_<int32_t>(int64_t(njobs) * kJumpMax);
+  W_.jl = alloc_<int32_t>(int64_t(njobs) * kJumpMax * kJumpList);
+  W_.jcnt = alloc_<int32_t>(int64_t(njobs) * kJumpMax);
   // spill pool: by default 64 entries per slot beyond the inline K (the unrelaxed
   // random start and strong h gradients are where lists run long); a job owns the
   // part [p0 * sps, (p0 + cap) * sps) of it
@@ -623,6 +626,9 @@ int Engine::step(int64_t nsteps) {
     end_();
     // jobs whose drift reached the skin re-sort and rebuild their lists; the other
     // jobs' tiles return at once from the passes up to k_nb_spill
+    begin_("k_jump_lists");
+    launch_jump_lists(st_, D_, W_, A_, int(jp_.size()));
+    end_();
     begin_("k_decide");
     launch_decide(st_, W_, int(jp_.size()), 0);
     end_();
@@ -816,78 +822,59 @@ int Engine::get_pair_terms(int job, float* sum, float* a1, float* a2) {
   return SMG_OK;
 }
 
-// The neighbour sets of the last step: the job's candidate lists with the device's
-// own per-entry predicate applied (k_nb_flags), mapped to particle ids.
+// The neighbour sets of the last step, as the force passes visit them (k_nb_out runs
+// the same neighbour iterator), mapped to particle ids.
 int Engine::get_neighbors(int job, int64_t* offs, int32_t* ids, int32_t* codes, int64_t capacity) {
   if (step_ == 0) { set_error("get_neighbors: no step taken yet"); return SMG_ERR_STATE; }
   const JobParams& J = jp_[job];
   const int64_t cap = J.cap, n = J.n;
-  std::vector<int32_t> id(cap_total_), cnt(cap), spo(cap);
-  HIP_TRY(hipStreamSynchronize(st_));
-  HIP_TRY(hipMemcpy(id.data(), A_.id, sizeof(int32_t) * cap_total_, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(cnt.data(), W_.nbc + J.p0, sizeof(int32_t) * cap, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(spo.data(), W_.nbo + J.p0, sizeof(int32_t) * cap, hipMemcpyDeviceToHost));
-  // candidate entries in slot order (CSR over the job's slots)
-  std::vector<int64_t> co(cap + 1, 0);
-  for (int64_t s = 0; s < cap; s++) co[s + 1] = co[s] + (s < n ? cnt[s] : 0);
-  const int64_t m = co[cap];
-  std::vector<uint8_t> fl(std::max<int64_t>(m, 1));
-  {
-    int64_t* d_off = nullptr;
-    uint8_t* d_fl = nullptr;
-    HIP_TRY(hipMalloc(&d_off, sizeof(int64_t) * (cap + 1)));
-    if (hipMalloc(&d_fl, std::max<int64_t>(m, 1)) != hipSuccess) {
-      (void)hipFree(d_off);
-      set_error("get_neighbors: hipMalloc failed");
-      return SMG_ERR_HIP;
-    }
-    hipError_t e = hipMemcpy(d_off, co.data(), sizeof(int64_t) * (cap + 1), hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-      Work Ws = W_;
-      Ws.tile0 = J.p0 / kTile;
-      launch_nb_flags(st_, D_, cap / kTile, Ws, A_, d_off, d_fl);
-      e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(st_);
-    if (e == hipSuccess && m > 0) e = hipMemcpy(fl.data(), d_fl, m, hipMemcpyDeviceToHost);
-    (void)hipFree(d_off);
-    (void)hipFree(d_fl);
-    if (e != hipSuccess) { set_error(std::string("get_neighbors: ") + hipGetErrorString(e)); return SMG_ERR_HIP; }
+  std::vector<int32_t> id(cap_total_), cnt(std::max<int64_t>(n, 1));
+  std::vector<int64_t> co(n + 1, 0);
+  std::vector<int32_t> ent;
+  int32_t* d_cnt = nullptr;
+  int32_t* d_out = nullptr;
+  int64_t* d_off = nullptr;
+  hipError_t e = hipStreamSynchronize(st_);
+  Work Ws = W_;
+  Ws.tile0 = J.p0 / kTile;
+  if (e == hipSuccess) e = hipMemcpy(id.data(), A_.id, sizeof(int32_t) * cap_total_, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMalloc(&d_cnt, sizeof(int32_t) * std::max<int64_t>(n, 1));
+  if (e == hipSuccess) e = hipMalloc(&d_off, sizeof(int64_t) * (n + 1));
+  if (e == hipSuccess) {
+    launch_nb_out(st_, D_, cap / kTile, Ws, A_, 0, d_off, d_cnt, nullptr);
+    e = hipGetLastError();
   }
-  std::vector<int32_t> spl(std::max<int64_t>(J.spill_cap, 1));
-  if (J.spill_cap > 0)
-    HIP_TRY(hipMemcpy(spl.data(), W_.spill + J.spill_base, sizeof(int32_t) * J.spill_cap, hipMemcpyDeviceToHost));
-  std::vector<int32_t> ent(int64_t(K_) * cap);
-  for (int k = 0; k < K_; k++)
-    HIP_TRY(hipMemcpy(ent.data() + int64_t(k) * cap, W_.nbl + int64_t(k) * cap_total_ + J.p0, sizeof(int32_t) * cap,
-                      hipMemcpyDeviceToHost));
+  if (e == hipSuccess) e = hipStreamSynchronize(st_);
+  if (e == hipSuccess) e = hipMemcpy(cnt.data(), d_cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) {
+    for (int64_t s = 0; s < n; s++) co[s + 1] = co[s] + cnt[s];
+    ent.resize(std::max<int64_t>(co[n], 1));
+    e = hipMemcpy(d_off, co.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(int32_t) * std::max<int64_t>(co[n], 1));
+  if (e == hipSuccess) {
+    launch_nb_out(st_, D_, cap / kTile, Ws, A_, 1, d_off, d_cnt, d_out);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st_);
+  if (e == hipSuccess && co[n] > 0) e = hipMemcpy(ent.data(), d_out, sizeof(int32_t) * co[n], hipMemcpyDeviceToHost);
+  if (d_cnt) (void)hipFree(d_cnt);
+  if (d_out) (void)hipFree(d_out);
+  if (d_off) (void)hipFree(d_off);
+  if (e != hipSuccess) { set_error(std::string("get_neighbors: ") + hipGetErrorString(e)); return SMG_ERR_HIP; }
   std::vector<int64_t> slot_of(n, -1);
   for (int64_t s = 0; s < n; s++) slot_of[id[J.p0 + s]] = s;
-  std::vector<int64_t> count_by_id(n, 0);
-  for (int64_t s = 0; s < n; s++) {
-    int64_t c = 0;
-    for (int64_t k = co[s]; k < co[s + 1]; k++) c += fl[k];
-    count_by_id[id[J.p0 + s]] = c;
-  }
   offs[0] = 0;
-  for (int64_t i = 0; i < n; i++) offs[i + 1] = offs[i] + count_by_id[i];
+  for (int64_t i = 0; i < n; i++) offs[i + 1] = offs[i] + cnt[slot_of[i]];
   if (!ids) return SMG_OK;
   if (offs[n] > capacity) { set_error("get_neighbors: capacity too small"); return SMG_ERR_CAPACITY; }
   std::vector<std::pair<int32_t, int32_t>> tmp;
   for (int64_t i = 0; i < n; i++) {
     const int64_t s = slot_of[i];
     tmp.clear();
-    for (int k = 0; k < cnt[s]; k++) {
-      if (!fl[co[s] + k]) continue;
-      uint32_t e;
-      if (k < K_) e = uint32_t(ent[int64_t(k) * cap + s]);
-      else {
-        const int64_t at = int64_t(spo[s]) - J.spill_base + (k - K_);
-        if (at < 0 || at >= J.spill_cap) { set_error("get_neighbors: spill entry out of range"); return SMG_ERR_STATE; }
-        e = uint32_t(spl[at]);
-      }
-      const int64_t q = e & kJMask;
-      tmp.push_back({id[q], int32_t(e >> kJBits)});
+    for (int64_t k = co[s]; k < co[s + 1]; k++) {
+      const uint32_t en = uint32_t(ent[k]);
+      tmp.push_back({id[en & kJMask], int32_t(en >> kJBits)});
     }
     std::sort(tmp.begin(), tmp.end());
     for (size_t k = 0; k < tmp.size(); k++) {

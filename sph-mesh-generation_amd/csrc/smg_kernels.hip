@@ -510,20 +510,40 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
       lp[a] = float(x[a] - double(hp[a]));
     }
     const float4 r0 = make_float4(hp[0], hp[1], hp[2], float(h));
-    const float4 r1 = make_float4(lp[0], lp[1], lp[2], __int_as_float(type));
+    bool jump = false;
+    if (mode == kModeStep) {
+      // drift since the last list build, as a fraction of h_b.  A particle whose drift
+      // reaches the limit becomes a jumper (its pairs are found without the lists until
+      // the next build); the job rebuilds when the other particles' drift reaches it or
+      // when there are more than kJumpMax jumpers.
+      const float4 b = W.recb[slot];
+      if (b.w < 0.f) {
+        jump = true;
+      } else {
+        const double dx0 = double(r0.x) - double(b.x), dx1 = double(r0.y) - double(b.y);
+        const double dx2 = D > 2 ? double(r0.z) - double(b.z) : 0.;
+        const double e = sqrt(dx0 * dx0 + dx1 * dx1 + dx2 * dx2) + 4.0 * double(J.pf_tol) +
+                         J.cut_off * fabs(h - double(b.w));
+        double rho = e / double(b.w) * (1.0 + 1.e-5) + 1.e-9;
+        rho = (rho <= 1.e300) ? rho : 1.e300;   // also for b.w = 0 or NaN
+        if (rho >= double(J.skin_lim) && J.skin_lim >= 0.f && W.red[j].built) {
+          const int ji = atomicAdd(&W.red[j].njump, 1);
+          if (ji < kJumpMax) {
+            W.jset[int64_t(j) * kJumpMax + ji] = int32_t(slot);
+            W.recb[slot].w = -b.w;
+            jump = true;
+          } else {
+            atomicOr(&W.red[j].jover, 1);
+          }
+        } else {
+          drift = rho;
+        }
+      }
+    }
+    const float4 r1 = make_float4(lp[0], lp[1], lp[2], __int_as_float(type | (jump ? kJumpBit : 0)));
     W.rec[4 * int64_t(slot) + 0] = r0;
     W.rec[4 * int64_t(slot) + 1] = r1;
     W.rec[4 * int64_t(slot) + 3] = make_float4(float(n[0]), D > 1 ? float(n[1]) : 0.f, D > 2 ? float(n[2]) : 0.f, 0.f);
-    if (mode == kModeStep) {
-      // drift since the last list build, as a fraction of h_b (rebuild test)
-      const float4 b = W.recb[slot];
-      const double dx0 = double(r0.x) - double(b.x), dx1 = double(r0.y) - double(b.y);
-      const double dx2 = D > 2 ? double(r0.z) - double(b.z) : 0.;
-      const double e = sqrt(dx0 * dx0 + dx1 * dx1 + dx2 * dx2) + 4.0 * double(J.pf_tol) +
-                       J.cut_off * fabs(h - double(b.w));
-      double rho = e / double(b.w) * (1.0 + 1.e-5) + 1.e-9;
-      drift = (rho <= 1.e300) ? rho : 1.e300;   // also for b.w = 0 or NaN
-    }
   }
   drift = block_reduce(drift, OpMax(), lds_d);
   res = block_reduce(res, OpMax(), lds_d);
@@ -561,12 +581,27 @@ __global__ void k_decide(Work W, int njobs, int force) {
     R.rebuild = 0;
     return;
   }
-  const int rb = force || !R.built || bitsd(R.drift) >= double(W.jp[j].skin_lim);
+  const int rb = force || !R.built || R.jover || bitsd(R.drift) >= double(W.jp[j].skin_lim);
   R.rebuild = rb;
   if (rb) {
     R.spill_used = 0ull;
+    R.njump = 0;
+    R.jover = 0;
     if (!force) R.nrebuild += 1;
     W.ctr[1] = 1ull;   // the table scan runs this step
+  } else {
+    // the jumpers in slot order, so every pass visits them in the same order
+    int32_t* js = W.jset + int64_t(j) * kJumpMax;
+    const int nj = R.njump < kJumpMax ? R.njump : kJumpMax;
+    for (int a = 1; a < nj; a++) {
+      const int32_t v = js[a];
+      int b = a - 1;
+      while (b >= 0 && js[b] > v) {
+        js[b + 1] = js[b];
+        b--;
+      }
+      js[b + 1] = v;
+    }
   }
 }
 
@@ -939,33 +974,278 @@ __device__ __forceinline__ bool is_neighbor(const JobParams& J, const Work& W, c
   return dist <= hid * J.cut_off + 1.e-10 || dist <= S.h[q] * J.cut_off + 1.e-10;
 }
 
-// per-entry neighbour flags of one job's lists (diagnostic download of the sets)
+// Candidate lists of this step's jumpers: the cell search of the last build's tables
+// around each jumper's current position (the other particles are within their drift
+// bound of their build positions, so the lists cover the jumper's neighbours), other
+// jumpers left out (every particle pairs with the jumpers separately).  One wavefront
+// per jumper: lane l takes the (mirror image, level, cell) triples l, l + 64, ...;
+// counts, a wave prefix sum and a second pass write the entries in triple order.  A
+// list longer than kJumpList makes the job rebuild.
 template <int D>
-__global__ __launch_bounds__(kTile) void k_nb_flags(Work W, PState S, const int64_t* offs, uint8_t* flags) {
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
-  if (j < 0) return;
+__global__ __launch_bounds__(64) void k_jump_lists(Work W, PState S) {
+  const int j = blockIdx.y, t = blockIdx.x;
   const JobParams& J = W.jp[j];
-  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
-  if (slot - J.p0 >= J.n) return;
-  const float4 r0 = W.rec[4 * int64_t(slot) + 0], r1 = W.rec[4 * int64_t(slot) + 1];
+  if (J.n == 0 || W.red[j].jover) return;
+  const int nj0 = W.red[j].njump;
+  if (t >= (nj0 < kJumpMax ? nj0 : kJumpMax)) return;
+  const int lane = threadIdx.x;
+  const int64_t slot = W.jset[int64_t(j) * kJumpMax + t];
+  int32_t* lst = W.jl + (int64_t(j) * kJumpMax + t) * kJumpList;
+  const float4 r0 = W.rec[4 * slot + 0], r1 = W.rec[4 * slot + 1];
+  const float xh[3] = {r0.x, r0.y, r0.z}, xl[3] = {r1.x, r1.y, r1.z};
+  double xi[3] = {0., 0., 0.};
+#pragma unroll
+  for (int a = 0; a < D; a++) xi[a] = double(xh[a]) + double(xl[a]);
+  const double hi = S.h[slot];
+  const float rfi = float(hi * J.cut_v);
+  const int li = level_of(J, hi);
+  const int32_t soff = int32_t(J.p0) - W.starts[J.tbl_base];
+  int opt[3][3], nopt[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    nopt[a] = 1;
+    opt[a][0] = 0;
+    if (a < D && J.sym[a]) {
+      if (xi[a] - J.box_l[a] < J.c0) opt[a][nopt[a]++] = 1;
+      if (J.box_r[a] - xi[a] < J.c0) opt[a][nopt[a]++] = 2;
+    }
+  }
+  const int ncell = D > 2 ? 27 : 9;
+  const int T = nopt[0] * nopt[1] * nopt[2] * J.nlev * ncell;
+  const float cutf = float(J.cut_v);
+  int total = 0;
+  for (int base = 0; base < T; base += 64) {
+    const int idx = base + lane;
+    int32_t lo = 0, hi2 = 0;
+    uint32_t code = 0u;
+    float A[3] = {0.f, 0.f, 0.f}, sg[3] = {0.f, 0.f, 0.f};
+    if (idx < T) {
+      const int cell = idx % ncell;
+      const int L = (idx / ncell) % J.nlev;
+      const int ci = idx / (ncell * J.nlev);
+      const int o0 = ci % nopt[0], o1 = (ci / nopt[0]) % nopt[1], o2 = ci / (nopt[0] * nopt[1]);
+      code = uint32_t(opt[0][o0] | (opt[1][o1] << 2) | (opt[2][o2] << 4));
+      double y[3] = {0., 0., 0.};
+      image_of<D>(J, code, xi, y);
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        const uint32_t m = (code >> (2 * a)) & 3u;
+        A[a] = m == 0 ? xh[a] : (m == 1 ? xh[a] - 2.f * J.wl_h[a] : xh[a] - 2.f * J.wh_h[a]);
+        sg[a] = m == 0 ? -1.f : 1.f;
+      }
+      const int sL = L < li ? L : li;
+      const int shift = D * (L - sL);
+      int cy[3] = {0, 0, 0};
+      cell_at<D>(J, sL, y, cy);
+      int c[3] = {cy[0] + cell % 3 - 1, cy[1] + (cell / 3) % 3 - 1, cy[2] + (D > 2 ? cell / 9 - 1 : 0)};
+      bool ok = c[0] >= 0 && c[0] < J.lev_n[sL][0] && c[1] >= 0 && c[1] < J.lev_n[sL][1];
+      if (D > 2) ok = ok && c[2] >= 0 && c[2] < J.lev_n[sL][2];
+      if (ok) {
+        const int64_t tb = J.tbl_base + J.lev_off[L];
+        const uint64_t m = morton<D>(c);
+        lo = W.starts[tb + int64_t(m << shift)] + soff;
+        hi2 = W.starts[tb + int64_t((m + 1) << shift)] + soff;
+      }
+    }
+    auto cand = [&](int32_t q) {
+      if (__float_as_int(W.rec[4 * int64_t(q) + 1].w) & kJumpBit) return false;
+      const float4 qr = W.recb[q];
+      float d2 = 0.f;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        const float tt = A[a] + sg[a] * (a == 0 ? qr.x : (a == 1 ? qr.y : qr.z));
+        d2 += tt * tt;
+      }
+      const float r = fmaxf(rfi, qr.w * cutf);
+      const float rr = r + (J.pf_tol + 1.e-6f * r);
+      return d2 <= rr * rr;
+    };
+    int cnt = 0;
+    for (int32_t q = lo; q < hi2; q++) cnt += cand(q);
+    // exclusive prefix over the lanes (lane order = triple order)
+    int pre = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(pre, o, 64);
+      if (lane >= o) pre += v;
+    }
+    const int sum = __shfl(pre, 63, 64);
+    pre -= cnt;
+    int w = total + pre;
+    for (int32_t q = lo; q < hi2; q++) {
+      if (!cand(q)) continue;
+      if (w < kJumpList) lst[w] = int32_t(uint32_t(q) | (code << kJBits));
+      w++;
+    }
+    total += sum;
+  }
+  if (lane == 0) {
+    if (total > kJumpList) atomicOr(&W.red[j].jover, 1);
+    W.jcnt[int64_t(j) * kJumpMax + t] = total < kJumpList ? total : kJumpList;
+    W.nbo[slot] = t;
+  }
+}
+
+// Jumpers that may pair with a tile: the tile's bounding box (hi parts of its live
+// particles) against every mirror image of each jumper, with the largest pair radius
+// c0.  Called by every thread of the workgroup; the survivors, in jset order, go to
+// LDS.
+struct JumpTile {
+  int n;
+  int32_t slot[kJumpMax];
+  float box[6];
+};
+
+template <int D>
+__device__ void jumper_prepare(const Work& W, const JobParams& J, int j, bool live, const float* xh, JumpTile& jt) {
+  const int nj0 = W.red[j].njump;
+  const int nj = nj0 < kJumpMax ? nj0 : kJumpMax;
+  if (nj == 0) {   // uniform over the workgroup
+    if (threadIdx.x == 0) jt.n = 0;
+    __syncthreads();
+    return;
+  }
+  __shared__ float red_lds[6][4];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    float lo = live ? xh[a] : 3.0e38f, hi = live ? xh[a] : -3.0e38f;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lo = fminf(lo, __shfl_xor(lo, o, 64));
+      hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+    }
+    if (l == 0) {
+      red_lds[2 * a][w] = lo;
+      red_lds[2 * a + 1][w] = hi;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    float box[6];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      box[2 * a] = fminf(fminf(red_lds[2 * a][0], red_lds[2 * a][1]), fminf(red_lds[2 * a][2], red_lds[2 * a][3]));
+      box[2 * a + 1] = fmaxf(fmaxf(red_lds[2 * a + 1][0], red_lds[2 * a + 1][1]),
+                             fmaxf(red_lds[2 * a + 1][2], red_lds[2 * a + 1][3]));
+    }
+    bool keep = false;
+    int32_t s = -1;
+    if (threadIdx.x < nj) {
+      s = W.jset[int64_t(j) * kJumpMax + threadIdx.x];
+      const float4 q = W.rec[4 * int64_t(s)];
+      const float qv[3] = {q.x, q.y, q.z};
+      const float R = float(J.c0) + 8.f * J.pf_tol;
+      float d2 = 0.f;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        // distance from the box to the nearest image of the jumper along this axis
+        float g = fmaxf(0.f, fmaxf(box[2 * a] - qv[a], qv[a] - box[2 * a + 1]));
+        if (J.sym[a]) {
+          const float lo = 2.f * J.wl_h[a] - qv[a], hi = 2.f * J.wh_h[a] - qv[a];
+          g = fminf(g, fmaxf(0.f, fmaxf(box[2 * a] - lo, lo - box[2 * a + 1])));
+          g = fminf(g, fmaxf(0.f, fmaxf(box[2 * a] - hi, hi - box[2 * a + 1])));
+        }
+        d2 += g * g;
+      }
+      keep = d2 <= R * R;
+    }
+    const uint64_t m = __ballot(keep);
+    if (keep) jt.slot[__popcll(m & ((1ull << threadIdx.x) - 1ull))] = s;
+    if (threadIdx.x == 0) jt.n = __popcll(m);
+  }
+  __syncthreads();
+}
+
+// Every neighbour of one particle, in a fixed order, handed to fn(k, q, code, qh, ql,
+// dr, d2), with k the list entry or -1: first the list entries (a non-jumper) or a
+// cell search of the last build's tables (a jumper), skipping jumpers; then the
+// jumpers near the tile, each with the particle's mirror options.  The neighbour
+// predicate is the reference's (is_neighbor).
+template <int D, class Fn>
+__device__ __forceinline__ void for_each_neighbor(const Work& W, const PState& S, const JobParams& J, int64_t slot,
+                                                  const JumpTile& jt, Fn fn, bool jumpers_only = false) {
+  const float4 r0 = W.rec[4 * slot + 0], r1 = W.rec[4 * slot + 1];
   const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
   const double hd = S.h[slot];
   const float rfi = float(hd * J.cut_off);
-  const int cnt = W.nbc[slot];
-  const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
-  const int64_t o = offs[slot - J.p0];
-  for (int k = 0; k < cnt; k++) {
-    const uint32_t e = nb_entry(W, slot, k, off);
-    const int64_t q = e & kJMask;
-    const uint32_t code = e >> kJBits;
-    const float4 qh = W.rec[4 * int64_t(q) + 0], ql = W.rec[4 * int64_t(q) + 1];
+  const bool ijump = __float_as_int(r1.w) & kJumpBit;
+  auto visit_rec = [&](int k, int64_t q, uint32_t code, const float4& qh, const float4& ql) {
     float dr[3] = {0.f, 0.f, 0.f};
     rel_pos<D>(J, qh, ql, code, xhi, xli, dr);
     float d2 = 0.f;
 #pragma unroll
     for (int a = 0; a < D; a++) d2 += dr[a] * dr[a];
-    flags[o + k] = is_neighbor<D>(J, W, S, q, code, d2, rfi, qh.w, xhi, xli, hd) ? 1 : 0;
+    if (is_neighbor<D>(J, W, S, q, code, d2, rfi, qh.w, xhi, xli, hd)) fn(k, q, code, qh, ql, dr, d2);
+  };
+  auto visit = [&](int k, int64_t q, uint32_t code) { visit_rec(k, q, code, W.rec[4 * q + 0], W.rec[4 * q + 1]); };
+  if (jumpers_only && !ijump) {
+    // the caller walked the list itself
+  } else if (!ijump) {
+    const int cnt = W.nbc[slot];
+    const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
+    for (int k = 0; k < cnt; k++) {
+      const uint32_t e = nb_entry(W, slot, k, off);
+      const int64_t q = e & kJMask;
+      const float4 qh = W.rec[4 * q + 0], ql = W.rec[4 * q + 1];
+      if (__float_as_int(ql.w) & kJumpBit) continue;   // jumpers come from the jumper scan
+      visit_rec(k, q, e >> kJBits, qh, ql);
+    }
+  } else {
+    // a jumper walks the list k_jump_lists built for it this step (nbo holds its index)
+    const int64_t t = int64_t(W.tile_job[slot / kTile]) * kJumpMax + W.nbo[slot];
+    const int cnt = W.jcnt[t];
+    const int32_t* lst = W.jl + t * kJumpList;
+    for (int k = 0; k < cnt; k++) {
+      const uint32_t e = uint32_t(lst[k]);
+      visit(-1, e & kJMask, e >> kJBits);
+    }
   }
+  if (jt.n == 0) return;
+  double xi[3] = {0., 0., 0.};
+#pragma unroll
+  for (int a = 0; a < D; a++) xi[a] = double(xhi[a]) + double(xli[a]);
+  int opt[3][3], nopt[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    nopt[a] = 1;
+    opt[a][0] = 0;
+    if (a < D && J.sym[a]) {
+      if (xi[a] - J.box_l[a] < J.c0) opt[a][nopt[a]++] = 1;
+      if (J.box_r[a] - xi[a] < J.c0) opt[a][nopt[a]++] = 2;
+    }
+  }
+  for (int t = 0; t < jt.n; t++) {
+    const int64_t q = jt.slot[t];
+    for (int o0 = 0; o0 < nopt[0]; o0++)
+      for (int o1 = 0; o1 < nopt[1]; o1++)
+        for (int o2 = 0; o2 < nopt[2]; o2++)
+          visit(-1, q, uint32_t(opt[0][o0] | (opt[1][o1] << 2) | (opt[2][o2] << 4)));
+  }
+}
+
+// neighbour sets for download: counts (pass 0) or (slot | code << 26) entries at offs
+template <int D>
+__global__ __launch_bounds__(kTile) void k_nb_out(Work W, PState S, int pass, const int64_t* offs, int32_t* cnt,
+                                                  int32_t* out) {
+  __shared__ JumpTile jt;
+  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  if (j < 0) return;
+  const JobParams& J = W.jp[j];
+  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const bool live = slot - J.p0 < J.n;
+  const float4 r0 = W.rec[4 * slot];
+  const float xh[3] = {r0.x, r0.y, r0.z};
+  jumper_prepare<D>(W, J, j, live, xh, jt);
+  if (!live) return;
+  int c = 0;
+  const int64_t o = pass ? offs[slot - J.p0] : 0;
+  for_each_neighbor<D>(W, S, J, slot, jt, [&](int, int64_t q, uint32_t code, const float4&, const float4&,
+                                               const float*, float) {
+    if (pass) out[o + c] = int32_t(uint32_t(q) | (code << kJBits));
+    c++;
+  });
+  if (!pass) cnt[slot - J.p0] = c;
 }
 
 // ---------------------------------------------------------------------------
@@ -975,23 +1255,22 @@ template <int D>
 __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
   __shared__ double lds_d[4];
   __shared__ int lds_i[4];
+  __shared__ JumpTile jt;
   const int j = W.tile_job[W.tile0 + blockIdx.x];
   if (j < 0) return;
   const JobParams& J = W.jp[j];
   const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
-  const int64_t loc = slot - J.p0;
+  const bool live = slot - J.p0 < J.n;
+  const float4 r0 = W.rec[4 * slot + 0], r1 = W.rec[4 * slot + 1];
+  const float xhi[3] = {r0.x, r0.y, r0.z};
+  jumper_prepare<D>(W, J, j, live, xhi, jt);
   double dt = 1.0e300;
   int ntr = 0;
-  if (loc < J.n) {
-    const float4 r0 = W.rec[4 * int64_t(slot) + 0], r1 = W.rec[4 * int64_t(slot) + 1];
-    const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
+  if (live) {
     float ni[3] = {0.f, 0.f, 0.f};
     const double hd = S.h[slot];
     const float h = r0.w;
-    const float rfi = float(hd * J.cut_off);
-    const int type = __float_as_int(r1.w);
-    const int cnt = W.nbc[slot];
-    const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
+    const int type = __float_as_int(r1.w) & ~kJumpBit;
     const bool surf = type == SMG_SURFACE;
     if (surf) {
 #pragma unroll
@@ -1008,32 +1287,24 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
     uint64_t mask[kMaskWords];
 #pragma unroll
     for (int w = 0; w < kMaskWords; w++) mask[w] = 0ull;
-    for (int k = 0; k < cnt; k++) {
-      const uint32_t e = nb_entry(W, slot, k, off);
-      const int64_t q = e & kJMask;
-      const uint32_t code = e >> kJBits;
-      const float4 qh = W.rec[4 * int64_t(q) + 0], ql = W.rec[4 * int64_t(q) + 1];
-      float dr[3] = {0.f, 0.f, 0.f};
-      rel_pos<D>(J, qh, ql, code, xhi, xli, dr);
-      float d2 = 0.f;
-#pragma unroll
-      for (int a = 0; a < D; a++) d2 += dr[a] * dr[a];
-      if (!is_neighbor<D>(J, W, S, q, code, d2, rfi, qh.w, xhi, xli, hd)) continue;
+    for_each_neighbor<D>(W, S, J, slot, jt, [&](int k, int64_t q, uint32_t code, const float4& qh, const float4& ql,
+                                                 const float* dr0, float d2) {
       ntr++;
-      const int tq = __float_as_int(ql.w);
+      const int tq = __float_as_int(ql.w) & ~kJumpBit;
       const bool self = q == slot && code == 0;
-      // entries the stage-2 pass visits: real neighbours other than itself, and for a
-      // surface particle not the REAL ones
-      if (k < 64 * kMaskWords && code == 0 && !self && !(surf && tq == SMG_REAL)) {
+      // list entries the stage-2 pass visits: real neighbours other than itself, and
+      // for a surface particle not the REAL ones (jumper pairs it finds itself)
+      if (k >= 0 && k < 64 * kMaskWords && code == 0 && !self && !(surf && tq == SMG_REAL)) {
 #pragma unroll
         for (int w = 0; w < kMaskWords; w++)
           if ((k >> 6) == w) mask[w] |= 1ull << (k & 63);
       }
+      float dr[3] = {dr0[0], dr0[1], dr0[2]};
       if (!surf) {
         const float rr = sqrtf(d2);
         sum += Wk<D>(rr, h);
         ncount++;
-        if (self) continue;
+        if (self) return;
         const float hq = qh.w;
         const float dxq = hq * J.kdx;
         const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
@@ -1041,7 +1312,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
 #pragma unroll
         for (int a = 0; a < D; a++) G[a] += dr[a] * aw;
       } else {
-        if (tq == SMG_REAL) continue;   // a surface particle sees surface neighbours only
+        if (tq == SMG_REAL) return;   // a surface particle sees surface neighbours only
         bool aligned = true;
         if (tq == SMG_SURFACE) {
           const float4 qn = W.rec[4 * q + 3];
@@ -1051,7 +1322,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
           for (int a = 0; a < D; a++) ca += ni[a] * nq[a];
           aligned = fabsf(ca) > J.cos_thres;
         }
-        if (!aligned) continue;
+        if (!aligned) return;
         float dn = 0.f;
 #pragma unroll
         for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
@@ -1066,7 +1337,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
           sum += Wk<D - 1>(rt, h);
           ncount++;
         }
-        if (self) continue;
+        if (self) return;
         const float hq = qh.w;
         const float dxq = hq * J.kdx;
         const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
@@ -1074,7 +1345,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
 #pragma unroll
         for (int a = 0; a < D; a++) G[a] += dr[a] * aw;
       }
-    }
+    });
     if (ncount > 1) sum = 1.f / (sum + 1.e-15f);
     W.sum[slot] = sum;
     W.ntrue[slot] = ntr;
@@ -1096,7 +1367,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
       W.a1[a][slot] = F[a];
       A += double(F[a]) * double(F[a]);
     }
-    W.rec[4 * int64_t(slot) + 2] = make_float4(F[0], F[1], D > 2 ? F[2] : 0.f, W.nu_b[slot]);
+    W.rec[4 * slot + 2] = make_float4(F[0], F[1], D > 2 ? F[2] : 0.f, W.nu_b[slot]);
     A = sqrt(A);
     // Set_timestep(1): v was zeroed by Reset_particle_force(1), so U = 0
     const double t1 = 0.25 * sqrt(hd / (A + 1.e-15));
@@ -1117,13 +1388,19 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
 template <int D>
 __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
   __shared__ double lds_d[4];
+  __shared__ JumpTile jt;
   const int j = W.tile_job[W.tile0 + blockIdx.x];
   if (j < 0) return;
   const JobParams& J = W.jp[j];
   const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
-  const int64_t loc = slot - J.p0;
+  const bool live = slot - J.p0 < J.n;
+  {
+    const float4 rb = W.rec[4 * slot];
+    const float xb[3] = {rb.x, rb.y, rb.z};
+    jumper_prepare<D>(W, J, j, live, xb, jt);
+  }
   double dt = 1.0e300, U = 0.;
-  if (loc < J.n) {
+  if (live) {
     const double dt1 = bitsd(W.red[j].dt1[par]);
     const float hdt = float(0.5 * dt1);
     const float4 r0 = W.rec[4 * int64_t(slot) + 0], r1 = W.rec[4 * int64_t(slot) + 1], r2 = W.rec[4 * int64_t(slot) + 2];
@@ -1139,7 +1416,8 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
     const double hd = S.h[slot];
     const float h = r0.w;
     const float rfi = float(hd * J.cut_off);
-    const int type = __float_as_int(r1.w);
+    const int type = __float_as_int(r1.w) & ~kJumpBit;
+    const bool ijump = __float_as_int(r1.w) & kJumpBit;
     const bool surf = type == SMG_SURFACE;
     if (surf) {
 #pragma unroll
@@ -1152,15 +1430,12 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
     const float rhoi = 1.f / voli;
     const float etai = r2.w * rhoi;
     float F[3] = {0.f, 0.f, 0.f};
-    // one stage-2 pair term; checked = the entry is a real, non-self neighbour that a
-    // surface particle keeps (from the stage-1 masks)
-    auto pair = [&](int k, bool checked) {
-      const uint32_t e = nb_entry(W, slot, k, off);
-      const uint32_t code = e >> kJBits;
-      const int64_t q = e & kJMask;
+    // one stage-2 pair term; checked = a list entry that the stage-1 masks marked as a
+    // real (not mirrored), non-self neighbour that a surface particle keeps
+    auto pair = [&](int64_t q, uint32_t code, bool checked) {
       if (!checked && (code != 0 || q == slot)) return;   // ghosts carry nu = 0 (Q6)
       const float4 qh = W.rec[4 * int64_t(q) + 0], ql = W.rec[4 * int64_t(q) + 1];
-      const int tq = __float_as_int(ql.w);
+      const int tq = __float_as_int(ql.w) & ~kJumpBit;
       if (!checked && surf && tq == SMG_REAL) return;
       const float4 qv = W.rec[4 * int64_t(q) + 2];
       const float hq = qh.w;
@@ -1214,17 +1489,30 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
       for (int a = 0; a < D; a++) F[a] += (vi[a] - vq[a]) * temp0;
     };
     if (etai != 0.f) {   // eta_ij = 0 for every pair when this particle's nu is 0
+      auto pair_any = [&](int, int64_t q, uint32_t code, const float4&, const float4&, const float*, float) {
+        pair(q, code, false);
+      };
+      if (!ijump) {
 #pragma unroll
-      for (int w = 0; w < kMaskWords; w++) {
-        if (64 * w >= cnt) break;
-        uint64_t m = W.nbm[int64_t(w) * W.cap_total + slot];
-        while (m) {
-          const int k = 64 * w + __builtin_ctzll(m);
-          m &= m - 1;
-          pair(k, true);
+        for (int w = 0; w < kMaskWords; w++) {
+          if (64 * w >= cnt) break;
+          uint64_t m = W.nbm[int64_t(w) * W.cap_total + slot];
+          while (m) {
+            const int k = 64 * w + __builtin_ctzll(m);
+            m &= m - 1;
+            const uint32_t e = nb_entry(W, slot, k, off);
+            pair(e & kJMask, e >> kJBits, true);
+          }
         }
+        for (int k = 64 * kMaskWords; k < cnt; k++) {
+          const uint32_t e = nb_entry(W, slot, k, off);
+          if (__float_as_int(W.rec[4 * int64_t(e & kJMask) + 1].w) & kJumpBit) continue;
+          pair(e & kJMask, e >> kJBits, false);
+        }
+        for_each_neighbor<D>(W, S, J, slot, jt, pair_any, true);   // the jumpers
+      } else {
+        for_each_neighbor<D>(W, S, J, slot, jt, pair_any);
       }
-      for (int k = 64 * kMaskWords; k < cnt; k++) pair(k, false);
     }
     if (surf) {
       float nf = 0.f;
@@ -1271,40 +1559,30 @@ constexpr int kVorVerts = 48;
 
 template <int D>
 __global__ __launch_bounds__(kTile) void k_vor2d(Work W, PState S, int32_t* out_nb, int32_t* out_cnt, int KE) {
+  __shared__ JumpTile jt;
   const int j = W.tile_job[W.tile0 + blockIdx.x];
   if (j < 0) return;
   const JobParams& J = W.jp[j];
   const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
   const int64_t loc = slot - J.p0;
-  if (loc >= J.n) return;
+  const bool live = loc < J.n;
   const float4 r0 = W.rec[4 * slot + 0], r1 = W.rec[4 * slot + 1];
-  const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
-  const double xi = double(xhi[0]) + double(xli[0]), yi = double(xhi[1]) + double(xli[1]);
-  const double hd = S.h[slot];
-  const float rfi = float(hd * J.cut_off);
-  const double sc = hd * J.cut_off * 1.5;
+  const float xhi[3] = {r0.x, r0.y, r0.z};
+  jumper_prepare<D>(W, J, j, live, xhi, jt);
+  if (!live) return;
+  const double xi = double(xhi[0]) + double(r1.x), yi = double(xhi[1]) + double(r1.y);
+  const double sc = S.h[slot] * J.cut_off * 1.5;
   double px[kVorVerts], py[kVorVerts];
-  int lab[kVorVerts];   // label of the edge from vertex v to v+1: -1 box wall, else list entry
+  int32_t lab[kVorVerts];   // label of the edge from vertex v to v+1: -1 box wall, else the neighbour entry
   int nv = 4;
   px[0] = xi - sc; py[0] = yi - sc; lab[0] = -1;
   px[1] = xi + sc; py[1] = yi - sc; lab[1] = -1;
   px[2] = xi + sc; py[2] = yi + sc; lab[2] = -1;
   px[3] = xi - sc; py[3] = yi + sc; lab[3] = -1;
   bool bad = false;
-  const int cnt = W.nbc[slot];
-  const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
-  for (int k = 0; k < cnt && !bad; k++) {
-    const uint32_t e = nb_entry(W, slot, k, off);
-    const int64_t q = e & kJMask;
-    const uint32_t code = e >> kJBits;
-    if (q == slot && code == 0) continue;
-    const float4 qh = W.rec[4 * q + 0], ql = W.rec[4 * q + 1];
-    float dr[3] = {0.f, 0.f, 0.f};
-    rel_pos<D>(J, qh, ql, code, xhi, xli, dr);
-    float d2 = 0.f;
-#pragma unroll
-    for (int a = 0; a < D; a++) d2 += dr[a] * dr[a];
-    if (!is_neighbor<D>(J, W, S, q, code, d2, rfi, qh.w, xhi, xli, hd)) continue;
+  for_each_neighbor<D>(W, S, J, slot, jt, [&](int, int64_t q, uint32_t code, const float4& qh, const float4& ql,
+                                               const float*, float) {
+    if (bad || nv == 0 || (q == slot && code == 0)) return;
     const double xq[3] = {double(qh.x) + double(ql.x), double(qh.y) + double(ql.y), 0.};
     double g[3];
     image_of<D>(J, code, xq, g);
@@ -1318,28 +1596,32 @@ __global__ __launch_bounds__(kTile) void k_vor2d(Work W, PState S, int32_t* out_
       if (f[v] > 0.) any_out = true;
       else any_in = true;
     }
-    if (!any_out) continue;        // the bisector misses the cell
-    if (!any_in) { nv = 0; break; }   // cannot happen for a neighbour other than a coincident point
+    if (!any_out) return;   // the bisector misses the cell
+    if (!any_in) {          // only for a point coincident with x_i
+      nv = 0;
+      return;
+    }
     double nx[kVorVerts], ny[kVorVerts];
-    int nl[kVorVerts];
+    int32_t nl[kVorVerts];
     int m = 0;
-    for (int v = 0; v < nv && !bad; v++) {
+    const int32_t label = int32_t(uint32_t(q) | (code << kJBits));
+    for (int v = 0; v < nv; v++) {
       const int w = v + 1 < nv ? v + 1 : 0;
       const bool vin = f[v] <= 0., win = f[w] <= 0.;
       if (vin) {
-        if (m >= kVorVerts) { bad = true; break; }
+        if (m >= kVorVerts) { bad = true; return; }
         nx[m] = px[v]; ny[m] = py[v];
         nl[m] = lab[v];   // the edge from v (to w or to the cut point) keeps its label
         m++;
       }
       if (vin != win) {   // the edge crosses the bisector: the cut point starts a new edge
         const double t = f[v] / (f[v] - f[w]);
-        if (m >= kVorVerts) { bad = true; break; }
+        if (m >= kVorVerts) { bad = true; return; }
         nx[m] = px[v] + t * (px[w] - px[v]);
         ny[m] = py[v] + t * (py[w] - py[v]);
-        // leaving the kept side: the new edge runs along the bisector (label k);
-        // entering it: the edge continues the old edge v -> w
-        nl[m] = vin ? k : lab[v];
+        // leaving the kept side: the new edge runs along the bisector; entering it: the
+        // edge continues the old edge v -> w
+        nl[m] = vin ? label : lab[v];
         m++;
       }
     }
@@ -1347,13 +1629,12 @@ __global__ __launch_bounds__(kTile) void k_vor2d(Work W, PState S, int32_t* out_
       px[v] = nx[v]; py[v] = ny[v]; lab[v] = nl[v];
     }
     nv = m;
-  }
+  });
   int nf = 0;
   if (!bad) {
     for (int v = 0; v < nv; v++) {
       if (lab[v] < 0) continue;
-      const uint32_t e = nb_entry(W, slot, lab[v], off);
-      const int32_t src = S.id[e & kJMask];
+      const int32_t src = S.id[uint32_t(lab[v]) & kJMask];
       if (src == S.id[slot]) continue;   // the cell face with the particle's own mirror image
       bool seen = false;
       for (int u = 0; u < nf; u++) seen = seen || out_nb[loc * KE + u] == src;
@@ -1454,10 +1735,15 @@ void launch_copyback(hipStream_t st, int D, int64_t ntiles, const Work& W, const
   if (D == 2) k_copyback<2><<<ntiles, kTile, 0, st>>>(W, A, B);
   else k_copyback<3><<<ntiles, kTile, 0, st>>>(W, A, B);
 }
-void launch_nb_flags(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, const int64_t* offs,
-                     uint8_t* flags) {
-  if (D == 2) k_nb_flags<2><<<ntiles, kTile, 0, st>>>(W, S, offs, flags);
-  else k_nb_flags<3><<<ntiles, kTile, 0, st>>>(W, S, offs, flags);
+void launch_nb_out(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int pass, const int64_t* offs,
+                   int32_t* cnt, int32_t* out) {
+  if (D == 2) k_nb_out<2><<<ntiles, kTile, 0, st>>>(W, S, pass, offs, cnt, out);
+  else k_nb_out<3><<<ntiles, kTile, 0, st>>>(W, S, pass, offs, cnt, out);
+}
+void launch_jump_lists(hipStream_t st, int D, const Work& W, const PState& S, int njobs) {
+  const dim3 g(kJumpMax, njobs);
+  if (D == 2) k_jump_lists<2><<<g, 64, 0, st>>>(W, S);
+  else k_jump_lists<3><<<g, 64, 0, st>>>(W, S);
 }
 void launch_reset(hipStream_t st, const Work& W, int njobs, int par) {
   k_reset<<<(njobs + 255) / 256, 256, 0, st>>>(W.red, njobs, par, W.ctr);

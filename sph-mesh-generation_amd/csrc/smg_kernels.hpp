@@ -12,10 +12,11 @@ enum { kModeInit = 0, kModePrerun = 1, kModeStep = 2 };
 
 void launch_reset(hipStream_t st, const Work& W, int njobs, int par);
 void launch_decide(hipStream_t st, const Work& W, int njobs, int force);
+void launch_jump_lists(hipStream_t st, int D, const Work& W, const PState& S, int njobs);
 void launch_key(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S);
 void launch_copyback(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& A, const PState& B);
-void launch_nb_flags(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, const int64_t* offs,
-                     uint8_t* flags);
+void launch_nb_out(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int pass, const int64_t* offs,
+                   int32_t* cnt, int32_t* out);
 void launch_seed(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S);
 void launch_map(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int mode, int par);
 void launch_scan(hipStream_t st, const Work& W, bool gated, bool per_job);
