@@ -178,6 +178,25 @@ int smg_capture(smg_job* job, int64_t steps_per_graph);
 /* device bytes held by the job (for admission by memory footprint) */
 int smg_footprint(smg_job* job, int64_t* bytes);
 
+/* ---- autotuner (no counterpart in the reference) ----
+ * Times candidate list shapes on a scratch copy of the job -- the cell-size multiple
+ * (1 + skin; cells hold (1 + skin) cut_off h) and the inline list depth max_neighbors
+ * -- each seeded, warmed for `warmup` steps and timed over `steps` graph-replayed
+ * steps, and returns the fastest.  The caller puts best_skin / best_max_neighbors
+ * into its smg_config before smg_create.  Results are bit-identical for every K;
+ * skin changes only the list and sort schedule, not the neighbour sets. */
+typedef struct smg_tune_result {
+  double best_skin;
+  int32_t best_max_neighbors;
+  int32_t n_tried;
+  double best_ms_per_step;
+  double skin[16];
+  int32_t max_neighbors[16];
+  double ms_per_step[16];   /* per candidate, in the order tried */
+} smg_tune_result;
+int smg_autotune(const smg_config* cfg, const double* phi, const float* normal, const int8_t* tag, int64_t n_particles,
+                 int32_t warmup, int32_t steps, smg_tune_result* result);
+
 /* ---- batched relaxation (many independent jobs per kernel launch) ----
  * No counterpart in the reference, which relaxes one geometry per MPI job.  Jobs of
  * the same dimension share one slot space: every kernel launch covers all of them,

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
 #include <array>
 #include <cmath>
 #include <cstring>
@@ -437,6 +438,61 @@ int smg_tri_edges(smg_job* job, int32_t* edges, int64_t capacity, int64_t* n_edg
     edges[2 * q] = e[q][0];
     edges[2 * q + 1] = e[q][1];
   }
+  return SMG_OK;
+}
+
+int smg_autotune(const smg_config* cfg, const double* phi, const float* normal, const int8_t* tag, int64_t n,
+                 int32_t warmup, int32_t steps, smg_tune_result* res) {
+  if (check_config(cfg) != SMG_OK) return SMG_ERR_ARG;
+  if (!phi || !normal || !tag || !res || n <= 0 || warmup < 0 || steps <= 0) {
+    set_error("smg_autotune: bad argument");
+    return SMG_ERR_ARG;
+  }
+  std::memset(res, 0, sizeof(*res));
+  const int64_t m = halo_cells(*cfg);
+  double mp = 0.;
+  for (int64_t q = 0; q < m; q++) mp = std::max(mp, phi[q]);
+  const double skins[] = {0.05, 0.1, 0.15, 0.25};
+  const int32_t depths[] = {64, 128};
+  const int64_t gsteps = 10;   // steps per captured graph (even)
+  const int64_t timed = std::max<int64_t>(gsteps, (steps + gsteps - 1) / gsteps * gsteps);
+  res->best_ms_per_step = 1.e300;
+  for (double sk : skins)
+    for (int32_t K : depths) {
+      smg::JobInput in;
+      in.cfg = *cfg;
+      in.cfg.skin = sk;
+      in.cfg.max_neighbors = K;
+      in.n = n;
+      in.max_phi = mp;
+      in.phi = phi;
+      in.normal = normal;
+      in.tag = tag;
+      smg::Engine eng;
+      int rc = eng.setup({in}, K, cfg->spill_per_slot, nullptr);
+      if (rc == SMG_OK) rc = eng.seed();
+      if (rc == SMG_OK) rc = eng.prerun();
+      if (rc == SMG_OK) rc = eng.step(warmup);
+      if (rc == SMG_OK) rc = eng.sync();
+      if (rc == SMG_OK) rc = eng.capture_graph(gsteps);
+      if (rc != SMG_OK) return rc;
+      const auto t0 = std::chrono::steady_clock::now();
+      rc = eng.run_graph(timed / gsteps);
+      if (rc == SMG_OK) rc = eng.sync();
+      if (rc == SMG_OK) rc = eng.check_health();
+      if (rc != SMG_OK) return rc;
+      const double ms =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / double(timed);
+      const int q = res->n_tried++;
+      res->skin[q] = sk;
+      res->max_neighbors[q] = K;
+      res->ms_per_step[q] = ms;
+      if (ms < res->best_ms_per_step) {
+        res->best_ms_per_step = ms;
+        res->best_skin = sk;
+        res->best_max_neighbors = K;
+      }
+    }
   return SMG_OK;
 }
 

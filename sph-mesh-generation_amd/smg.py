@@ -467,6 +467,25 @@ def tri_cliques(n_vertices, edges):
     return out[:m.value]
 
 
+class SmgTuneResult(C.Structure):
+    _fields_ = [("best_skin", C.c_double), ("best_max_neighbors", C.c_int32), ("n_tried", C.c_int32),
+                ("best_ms_per_step", C.c_double), ("skin", C.c_double * 16), ("max_neighbors", C.c_int32 * 16),
+                ("ms_per_step", C.c_double * 16)]
+
+
+def autotune(cfg, lset, n, warmup=50, steps=50):
+    """Fastest (skin, max_neighbors) for this job, timed on scratch copies (smg_autotune)."""
+    L = lib()
+    L.smg_autotune.argtypes = [C.POINTER(SmgConfig), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                               C.c_int32, C.POINTER(SmgTuneResult)]
+    r = SmgTuneResult()
+    _check(L.smg_autotune(C.byref(cfg), _ptr(lset["phi"]), _ptr(lset["normal"]), _ptr(lset["tag"]), int(n),
+                          int(warmup), int(steps), C.byref(r)), "smg_autotune")
+    k = r.n_tried
+    return {"skin": r.best_skin, "max_neighbors": r.best_max_neighbors, "ms_per_step": r.best_ms_per_step,
+            "tried": [(r.skin[i], r.max_neighbors[i], r.ms_per_step[i]) for i in range(k)]}
+
+
 class SmgServerConfig(C.Structure):
     _fields_ = [
         ("dim", C.c_int32), ("max_jobs", C.c_int32), ("max_slots", C.c_int64), ("max_table", C.c_int64),

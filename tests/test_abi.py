@@ -93,3 +93,16 @@ def test_server_without_gpu_fails_loudly(smg_mod):
     env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1", CUDA_VISIBLE_DEVICES="-1")
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
     assert "RC -2" in out.stdout, out.stdout + out.stderr
+
+
+def test_autotune_without_gpu_fails_loudly(smg_mod):
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path.insert(0, %r); import smg; cfg, sdf = smg.config_c1(num_pkg=8, dx=0.05); "
+        "L = smg.make_levelset(cfg, sdf)\n"
+        "try:\n    smg.autotune(cfg, L, 50, 1, 2)\nexcept smg.SmgError as e:\n    print('RC', e.rc)\nelse:\n    print('RC', 0)\n"
+    ) % os.path.join(ROOT, "sph-mesh-generation_amd")
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1", CUDA_VISIBLE_DEVICES="-1")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert "RC -2" in out.stdout, out.stdout + out.stderr
