@@ -298,10 +298,27 @@ def serve(args, smg):
     print(json.dumps(out), flush=True)
 
 
+def copy(args, smg):
+    """A plain device copy of known size (the reference point that scales the HBM
+    byte counters of a profiler run): 1 GiB copied args.steps times."""
+    import torch
+    nbytes = 1 << 30
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"mode": "copy", "bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes,
+                      "copies": args.steps, "GBs": 2 * nbytes * args.steps / el / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--mode", choices=("relax", "batch", "serve"), default="relax")
+    ap.add_argument("--mode", choices=("relax", "batch", "serve", "copy"), default="relax")
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--scale", type=float, default=0.564)
@@ -323,7 +340,7 @@ def main():
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
     import smg
-    {"relax": relax, "batch": batch, "serve": serve}[args.mode](args, smg)
+    {"relax": relax, "batch": batch, "serve": serve, "copy": copy}[args.mode](args, smg)
 
 
 if __name__ == "__main__":

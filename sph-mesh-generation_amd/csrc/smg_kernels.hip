@@ -82,6 +82,18 @@ struct OpAddU {
   __device__ unsigned long long operator()(unsigned long long a, unsigned long long b) const { return a + b; }
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The
+// tile of workgroup b is chosen so that every XCD works through one contiguous run of
+// tiles -- cell-sorted slots, hence a compact region of space -- and the neighbour
+// records a tile reads are mostly in the L2 of the XCD that reads them.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t tile_id(const Work& W) {
+  const int64_t b = blockIdx.x, G = gridDim.x;
+  const int64_t x = b % kXcds, i = b / kXcds;
+  const int64_t q = G / kXcds, r = G % kXcds;
+  return W.tile0 + x * q + (x < r ? x : r) + i;
+}
+
 // neighbour entry k of a slot (spill offset off is read only when the list spills)
 __device__ __forceinline__ uint32_t nb_entry(const Work& W, int64_t slot, int k, int32_t off) {
   return k < W.K ? uint32_t(W.nbl[int64_t(k) * W.cap_total + slot]) : uint32_t(W.spill[int64_t(off) + (k - W.K)]);
@@ -329,10 +341,10 @@ __global__ void k_reset(JobRed* red, int njobs, int par, unsigned long long* ctr
 
 template <int D>
 __global__ __launch_bounds__(kTile) void k_seed(Work W, PState S) {
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  const int j = W.tile_job[tile_id(W)];
   if (j < 0) return;   // tile not owned by a job
   const JobParams& J = W.jp[j];
-  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t slot = tile_id(W) * kTile + threadIdx.x;
   const int64_t loc = slot - J.p0;
   int bad = 0;
   if (loc < J.n) {
@@ -394,10 +406,10 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
   __shared__ double lds_d[4];
   __shared__ int lds_i[4];
   __shared__ unsigned long long lds_u[4];
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  const int j = W.tile_job[tile_id(W)];
   if (j < 0) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t slot = tile_id(W) * kTile + threadIdx.x;
   const int64_t loc = slot - J.p0;
   double res = 0., md = 0., drift = 0.;
   unsigned long long rq = 0ull;
@@ -608,10 +620,10 @@ __global__ void k_decide(Work W, int njobs, int force) {
 // cell key and count of every slot of a rebuilding job
 template <int D>
 __global__ __launch_bounds__(kTile) void k_key(Work W, PState S) {
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t slot = tile_id(W) * kTile + threadIdx.x;
   uint32_t key;
   if (slot - J.p0 < J.n) {
     const double h = S.h[slot];
@@ -719,10 +731,10 @@ __global__ __launch_bounds__(kTile) void k_scan_down(const int32_t* __restrict__
 // A job's cell starts are taken relative to the start of its table segment, so the
 // slot ranges of the jobs may sit anywhere (gaps included) and still sort in place.
 __global__ __launch_bounds__(kTile) void k_scatter(Work W) {
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
-  const int64_t p = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t p = tile_id(W) * kTile + threadIdx.x;
   const uint32_t k = W.key[p];
   const int r = atomicSub(&W.counts[k], 1) - 1;
   const int64_t d = J.p0 + int64_t(W.starts[k] - W.starts[J.tbl_base]) + r;
@@ -731,10 +743,10 @@ __global__ __launch_bounds__(kTile) void k_scatter(Work W) {
 }
 
 __global__ __launch_bounds__(kTile) void k_segsort(Work W, PState S) {
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
-  const int64_t d = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t d = tile_id(W) * kTile + threadIdx.x;
   const uint32_t k = W.skey[d];
   if (k == uint32_t(J.tbl_base + J.tbl_size - 1)) return;   // dead slots: no order needed
   if (d > J.p0 && W.skey[d - 1] == k) return;                 // not the head of its cell
@@ -758,9 +770,9 @@ __global__ __launch_bounds__(kTile) void k_segsort(Work W, PState S) {
 // current state at every step whichever jobs re-sorted
 template <int D>
 __global__ __launch_bounds__(kTile) void k_gather(Work W, PState A, PState B) {
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
-  const int64_t d = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t d = tile_id(W) * kTile + threadIdx.x;
   const int64_t p = W.perm[d];
 #pragma unroll
   for (int a = 0; a < D; a++) {
@@ -776,9 +788,9 @@ __global__ __launch_bounds__(kTile) void k_gather(Work W, PState A, PState B) {
 
 template <int D>
 __global__ __launch_bounds__(kTile) void k_copyback(Work W, PState A, PState B) {
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
-  const int64_t d = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t d = tile_id(W) * kTile + threadIdx.x;
   float xh[3] = {0.f, 0.f, 0.f}, xl[3] = {0.f, 0.f, 0.f}, nn[3] = {0.f, 0.f, 0.f};
 #pragma unroll
   for (int a = 0; a < D; a++) {
@@ -898,10 +910,10 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
 // more joins the spill queue (k_copyback left the build positions and h in recb)
 template <int D>
 __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t slot = tile_id(W) * kTile + threadIdx.x;
   const int64_t loc = slot - J.p0;
   int cnt = 0;
   if (loc < J.n) {
@@ -1184,12 +1196,23 @@ __device__ __forceinline__ void for_each_neighbor(const Work& W, const PState& S
   } else if (!ijump) {
     const int cnt = W.nbc[slot];
     const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
+    // software pipelined: the next entry and its record load while this one is used
+    uint32_t en = cnt > 0 ? nb_entry(W, slot, 0, off) : 0u;
+    float4 qhn = make_float4(0.f, 0.f, 0.f, 0.f), qln = qhn;
+    if (cnt > 0) {
+      qhn = W.rec[4 * int64_t(en & kJMask) + 0];
+      qln = W.rec[4 * int64_t(en & kJMask) + 1];
+    }
     for (int k = 0; k < cnt; k++) {
-      const uint32_t e = nb_entry(W, slot, k, off);
-      const int64_t q = e & kJMask;
-      const float4 qh = W.rec[4 * q + 0], ql = W.rec[4 * q + 1];
+      const uint32_t e = en;
+      const float4 qh = qhn, ql = qln;
+      if (k + 1 < cnt) {
+        en = nb_entry(W, slot, k + 1, off);
+        qhn = W.rec[4 * int64_t(en & kJMask) + 0];
+        qln = W.rec[4 * int64_t(en & kJMask) + 1];
+      }
       if (__float_as_int(ql.w) & kJumpBit) continue;   // jumpers come from the jumper scan
-      visit_rec(k, q, e >> kJBits, qh, ql);
+      visit_rec(k, e & kJMask, e >> kJBits, qh, ql);
     }
   } else {
     // a jumper walks the list k_jump_lists built for it this step (nbo holds its index)
@@ -1229,10 +1252,10 @@ template <int D>
 __global__ __launch_bounds__(kTile) void k_nb_out(Work W, PState S, int pass, const int64_t* offs, int32_t* cnt,
                                                   int32_t* out) {
   __shared__ JumpTile jt;
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  const int j = W.tile_job[tile_id(W)];
   if (j < 0) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t slot = tile_id(W) * kTile + threadIdx.x;
   const bool live = slot - J.p0 < J.n;
   const float4 r0 = W.rec[4 * slot];
   const float xh[3] = {r0.x, r0.y, r0.z};
@@ -1256,10 +1279,10 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
   __shared__ double lds_d[4];
   __shared__ int lds_i[4];
   __shared__ JumpTile jt;
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  const int j = W.tile_job[tile_id(W)];
   if (j < 0) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t slot = tile_id(W) * kTile + threadIdx.x;
   const bool live = slot - J.p0 < J.n;
   const float4 r0 = W.rec[4 * slot + 0], r1 = W.rec[4 * slot + 1];
   const float xhi[3] = {r0.x, r0.y, r0.z};
@@ -1389,10 +1412,10 @@ template <int D>
 __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
   __shared__ double lds_d[4];
   __shared__ JumpTile jt;
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  const int j = W.tile_job[tile_id(W)];
   if (j < 0) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t slot = tile_id(W) * kTile + threadIdx.x;
   const bool live = slot - J.p0 < J.n;
   {
     const float4 rb = W.rec[4 * slot];
@@ -1432,12 +1455,11 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
     float F[3] = {0.f, 0.f, 0.f};
     // one stage-2 pair term; checked = a list entry that the stage-1 masks marked as a
     // real (not mirrored), non-self neighbour that a surface particle keeps
-    auto pair = [&](int64_t q, uint32_t code, bool checked) {
+    auto pair_rec = [&](int64_t q, uint32_t code, bool checked, const float4& qh, const float4& ql,
+                        const float4& qv) {
       if (!checked && (code != 0 || q == slot)) return;   // ghosts carry nu = 0 (Q6)
-      const float4 qh = W.rec[4 * int64_t(q) + 0], ql = W.rec[4 * int64_t(q) + 1];
       const int tq = __float_as_int(ql.w) & ~kJumpBit;
       if (!checked && surf && tq == SMG_REAL) return;
-      const float4 qv = W.rec[4 * int64_t(q) + 2];
       const float hq = qh.w;
       const float dxq = hq * J.kdx;
       const float volq = ipow<D>(dxq);
@@ -1488,21 +1510,49 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
 #pragma unroll
       for (int a = 0; a < D; a++) F[a] += (vi[a] - vq[a]) * temp0;
     };
+    auto pair = [&](int64_t q, uint32_t code, bool checked) {
+      pair_rec(q, code, checked, W.rec[4 * q + 0], W.rec[4 * q + 1], W.rec[4 * q + 2]);
+    };
     if (etai != 0.f) {   // eta_ij = 0 for every pair when this particle's nu is 0
       auto pair_any = [&](int, int64_t q, uint32_t code, const float4&, const float4&, const float*, float) {
         pair(q, code, false);
       };
       if (!ijump) {
+        // the masked entries, software pipelined: the next entry and its record load
+        // while this one is used
+        uint64_t m[kMaskWords];
 #pragma unroll
-        for (int w = 0; w < kMaskWords; w++) {
-          if (64 * w >= cnt) break;
-          uint64_t m = W.nbm[int64_t(w) * W.cap_total + slot];
-          while (m) {
-            const int k = 64 * w + __builtin_ctzll(m);
-            m &= m - 1;
-            const uint32_t e = nb_entry(W, slot, k, off);
-            pair(e & kJMask, e >> kJBits, true);
+        for (int w = 0; w < kMaskWords; w++) m[w] = 64 * w < cnt ? W.nbm[int64_t(w) * W.cap_total + slot] : 0ull;
+        auto next_k = [&]() {
+#pragma unroll
+          for (int w = 0; w < kMaskWords; w++)
+            if (m[w]) {
+              const int k = 64 * w + __builtin_ctzll(m[w]);
+              m[w] &= m[w] - 1;
+              return k;
+            }
+          return -1;
+        };
+        int kn = next_k();
+        uint32_t en = 0u;
+        float4 qh = make_float4(0.f, 0.f, 0.f, 0.f), ql = qh, qv = qh;
+        if (kn >= 0) {
+          en = nb_entry(W, slot, kn, off);
+          qh = W.rec[4 * int64_t(en & kJMask) + 0];
+          ql = W.rec[4 * int64_t(en & kJMask) + 1];
+          qv = W.rec[4 * int64_t(en & kJMask) + 2];
+        }
+        while (kn >= 0) {
+          const uint32_t e = en;
+          const float4 ch = qh, cl = ql, cv = qv;
+          kn = next_k();
+          if (kn >= 0) {
+            en = nb_entry(W, slot, kn, off);
+            qh = W.rec[4 * int64_t(en & kJMask) + 0];
+            ql = W.rec[4 * int64_t(en & kJMask) + 1];
+            qv = W.rec[4 * int64_t(en & kJMask) + 2];
           }
+          pair_rec(e & kJMask, e >> kJBits, true, ch, cl, cv);
         }
         for (int k = 64 * kMaskWords; k < cnt; k++) {
           const uint32_t e = nb_entry(W, slot, k, off);
@@ -1560,10 +1610,10 @@ constexpr int kVorVerts = 48;
 template <int D>
 __global__ __launch_bounds__(kTile) void k_vor2d(Work W, PState S, int32_t* out_nb, int32_t* out_cnt, int KE) {
   __shared__ JumpTile jt;
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  const int j = W.tile_job[tile_id(W)];
   if (j < 0) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t slot = tile_id(W) * kTile + threadIdx.x;
   const int64_t loc = slot - J.p0;
   const bool live = loc < J.n;
   const float4 r0 = W.rec[4 * slot + 0], r1 = W.rec[4 * slot + 1];
@@ -1665,10 +1715,10 @@ template <int D>
 __global__ __launch_bounds__(kTile) void k_residual(Work W, PState S, int par) {
   __shared__ double lds_d[4];
   __shared__ unsigned long long lds_u[4];
-  const int j = W.tile_job[W.tile0 + blockIdx.x];
+  const int j = W.tile_job[tile_id(W)];
   if (j < 0) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = (W.tile0 + int64_t(blockIdx.x)) * kTile + threadIdx.x;
+  const int64_t slot = tile_id(W) * kTile + threadIdx.x;
   double res = 0., md = 0.;
   unsigned long long rq = 0ull;
   if (slot - J.p0 < J.n) {
