@@ -120,6 +120,16 @@ def relax(args, smg):
     units = n * (rebuild_frac_prof if dom in REBUILD_ONLY else 1.0)
     bpl = BYTES.get(dom, (0, 0))[0 if D == 3 else 1] * units
     achieved = bpl / (avg_ms * 1e-3) / 1e9
+    # HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc
+    # FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated), if present
+    traffic, traffic_src = None, None
+    pmc = os.path.join(ROOT, "profiles", "pmc_traffic_relax3d_1M.json")
+    if os.path.exists(pmc) and D == 3:
+        with open(pmc) as f:
+            t = json.load(f).get("kernels", {}).get(dom)
+        if t:
+            traffic = t["traffic_bytes"]
+            traffic_src = "profiles/pmc_traffic_relax3d_1M.json (FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated)"
     out = {
         "metric": METRIC,
         "value": value,
@@ -148,7 +158,8 @@ def relax(args, smg):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel_avg_ms": avg_ms,
             "algorithmic_bytes_per_launch": bpl,
         },

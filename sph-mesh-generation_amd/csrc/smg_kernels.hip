@@ -1520,17 +1520,20 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
       if (!ijump) {
         // the masked entries, software pipelined: the next entry and its record load
         // while this one is used
-        uint64_t m[kMaskWords];
-#pragma unroll
-        for (int w = 0; w < kMaskWords; w++) m[w] = 64 * w < cnt ? W.nbm[int64_t(w) * W.cap_total + slot] : 0ull;
+        static_assert(kMaskWords == 2, "two mask words");
+        uint64_t m0 = W.nbm[slot];
+        uint64_t m1 = 64 < cnt ? W.nbm[W.cap_total + slot] : 0ull;
         auto next_k = [&]() {
-#pragma unroll
-          for (int w = 0; w < kMaskWords; w++)
-            if (m[w]) {
-              const int k = 64 * w + __builtin_ctzll(m[w]);
-              m[w] &= m[w] - 1;
-              return k;
-            }
+          if (m0) {
+            const int k = __builtin_ctzll(m0);
+            m0 &= m0 - 1;
+            return k;
+          }
+          if (m1) {
+            const int k = 64 + __builtin_ctzll(m1);
+            m1 &= m1 - 1;
+            return k;
+          }
           return -1;
         };
         int kn = next_k();
