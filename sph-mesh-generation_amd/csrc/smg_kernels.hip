@@ -297,6 +297,35 @@ __device__ __forceinline__ float dWk(float r, float h) {   // (dW/dr) / (r + 1e-
   return v / (r + 1.e-20f);
 }
 
+// Pair-loop forms with the hardware reciprocal and square root (v_rcp_f32,
+// v_sqrt_f32, 1 ulp) in place of the IEEE division sequences: W from s = r / h and
+// C = kC / h^DK of the particle's own h, both computed once per particle; dW from the
+// pair's mean h with one reciprocal.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <int DK>
+__device__ __forceinline__ float Wk_s(float s, float C) {
+  if (s < 1.f) return C * (6.0f - 6.0f * s + s * s * s);
+  if (s < 2.f) {
+    const float t = 2.f - s;
+    return C * t * t * t;
+  }
+  return 0.f;
+}
+template <int DK>
+__device__ __forceinline__ float dWk_f(float r, float h) {
+  const float ih = frcp(h);
+  const float C = kC<DK>() * ipow<DK + 1>(ih);
+  const float s = r * ih;
+  float v = 0.f;
+  if (s < 1.f) v = C * (-6.0f + 3.f * s * s);
+  else if (s < 2.f) {
+    const float t = 2.f - s;
+    v = C * (-3.0f) * t * t;
+  }
+  return v * frcp(r + 1.e-20f);
+}
+
 // relative position x_i - image(x_q) in FP32 from the hi/lo pairs; the image of a
 // mirror ghost is 2w - x_q, so its difference is (x_i - w) + (x_q - w)
 template <int D>
@@ -1301,6 +1330,8 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
     }
     const float dxi = h * J.kdx;
     const float vol2i = ipow<D>(dxi) * ipow<D>(dxi);
+    const float ih = 1.f / h;
+    const float Cw = kC<D>() * ipow<D>(ih), Cs = kC<D - 1>() * ipow<D - 1>(ih);
     // Get_kernel_summation (particle_mesh_generation.cpp:690-765) and the stage-1
     // Accumulate_particle_force_real / _surf in one pass: the force is
     // -sum_i * sum_j (vol_i^2 + vol_j^2) dW dr with sum_i known only at the end
@@ -1324,14 +1355,14 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
       }
       float dr[3] = {dr0[0], dr0[1], dr0[2]};
       if (!surf) {
-        const float rr = sqrtf(d2);
-        sum += Wk<D>(rr, h);
+        const float rr = fsqrt(d2);
+        sum += Wk_s<D>(rr * ih, Cw);
         ncount++;
         if (self) return;
         const float hq = qh.w;
         const float dxq = hq * J.kdx;
         const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
-        const float aw = dWk<D>(rr, 0.5f * (h + hq)) * (vol2i + vol2q);
+        const float aw = dWk_f<D>(rr, 0.5f * (h + hq)) * (vol2i + vol2q);
 #pragma unroll
         for (int a = 0; a < D; a++) G[a] += dr[a] * aw;
       } else {
@@ -1355,16 +1386,16 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
           dr[a] = dr[a] - ni[a] * dn;
           rt += dr[a] * dr[a];
         }
-        rt = sqrtf(rt);
+        rt = fsqrt(rt);
         if (tq == SMG_SURFACE) {
-          sum += Wk<D - 1>(rt, h);
+          sum += Wk_s<D - 1>(rt * ih, Cs);
           ncount++;
         }
         if (self) return;
         const float hq = qh.w;
         const float dxq = hq * J.kdx;
         const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
-        const float aw = dWk<D - 1>(rt, 0.5f * (h + hq)) * (vol2i + vol2q);
+        const float aw = dWk_f<D - 1>(rt, 0.5f * (h + hq)) * (vol2i + vol2q);
 #pragma unroll
         for (int a = 0; a < D; a++) G[a] += dr[a] * aw;
       }
@@ -1463,7 +1494,7 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
       const float hq = qh.w;
       const float dxq = hq * J.kdx;
       const float volq = ipow<D>(dxq);
-      const float rhoq = 1.f / volq;
+      const float rhoq = frcp(volq);
       const float etaq = qv.w * rhoq;
       if (etaq == 0.f) return;   // eta_ij = 0
       float dr[3] = {0.f, 0.f, 0.f};
@@ -1475,7 +1506,7 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
       const float hij = 0.5f * (h + hq);
       float aw;
       if (!surf) {
-        aw = dWk<D>(sqrtf(rr), hij);
+        aw = dWk_f<D>(fsqrt(rr), hij);
       } else {
         if (tq == SMG_SURFACE) {
           const float4 qn = W.rec[4 * q + 3];
@@ -1494,7 +1525,7 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
           const float t = dr[a] - ni[a] * dn;
           rt += t * t;
         }
-        aw = dWk<D - 1>(sqrtf(rt), hij);
+        aw = dWk_f<D - 1>(fsqrt(rt), hij);
       }
       const float qa[3] = {qv.x, qv.y, qv.z};
       float vq[3];
@@ -1505,7 +1536,7 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
 #pragma unroll
         for (int a = 0; a < D; a++) vq[a] = qa[a] * hdt;
       }
-      const float etaij = 2.f * etaq * etai / (etai + etaq + 1.e-15f);
+      const float etaij = 2.f * etaq * etai * frcp(etai + etaq + 1.e-15f);
       const float temp0 = aw * etaij * (voli * voli + volq * volq);
 #pragma unroll
       for (int a = 0; a < D; a++) F[a] += (vi[a] - vq[a]) * temp0;
