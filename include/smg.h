@@ -61,8 +61,9 @@ typedef struct smg_config {
   double lin_band;           /* linear profile band width in phi              */
   uint64_t seed;             /* counter-based RNG key (replaces rand())       */
   int32_t remap_max_attempts;/* bound on the re-seeding rejection loop        */
-  int32_t max_neighbors;     /* inline list entries per particle, 0 = 128;    
-                                longer lists spill to a shared pool          */
+  int32_t max_neighbors;     /* inline list entries per particle, 0 = 128,    
+                                rounded up to a multiple of 4; longer lists  
+                                spill to a shared pool                       */
   int32_t spill_per_slot;    /* spill pool entries per particle slot, 0 = 64  */
   int32_t reserved0;
   double skin;               /* neighbour lists cover (1 + skin) * cut_off * h
@@ -270,7 +271,7 @@ typedef struct smg_server_config {
   int64_t max_slots;         /* particle slots (a job takes n rounded up to 256)      */
   int64_t max_table;         /* cell-table entries (a job takes its cell-list tables) */
   int64_t max_lset_cells;    /* level-set cells (a job takes (4 num_pkg + 2)^dim)     */
-  int32_t max_neighbors;     /* inline list entries per slot, 0 = 128                 */
+  int32_t max_neighbors;     /* inline list entries per slot, 0 = 128 (multiple of 4) */
   int32_t spill_per_slot;    /* spill pool entries per slot, 0 = 64                   */
   int32_t chunk_steps;       /* steps between admissions / convergence checks (> 0)  */
   int32_t min_steps;         /* no retirement by convergence before this many steps  */

@@ -110,8 +110,10 @@ struct Work {
   // [2] = (a1.xyz, nu of the previous step), [3] = (normal.xyz, 0)
   float4* rec;
   // candidate lists (all pairs within (1 + skin) cut_off h at the last build): the
-  // first K entries of a slot inline at [k * cap_total + slot]; entries K.. of a
-  // longer list in the job's part of the spill pool at spill[nbo[slot] + k - K]
+  // first K entries of a slot inline, in groups of four so that one 16-byte load
+  // brings four entries -- entry k at [((k / 4) * cap_total + slot) * 4 + k % 4], K a
+  // multiple of 4; entries K.. of a longer list in the job's part of the spill pool
+  // at spill[nbo[slot] + k - K]
   int32_t* nbl;
   int32_t* nbc;     // candidate count per slot (inline + spilled)
   int32_t* nbo;     // spill offset per slot (valid when nbc > K)

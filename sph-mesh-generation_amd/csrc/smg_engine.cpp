@@ -283,7 +283,7 @@ int Engine::upload_job_(int j, const JobInput& in) {
 int Engine::begin_setup_(int D, int max_neighbors, hipStream_t stream) {
   D_ = D;
   if (D_ != 2 && D_ != 3) { set_error("setup: dim must be 2 or 3"); return SMG_ERR_ARG; }
-  K_ = max_neighbors > 0 ? max_neighbors : 128;
+  K_ = max_neighbors > 0 ? (max_neighbors + 3) / 4 * 4 : 128;   // whole groups of four entries
   if (stream) st_ = stream;
   else {
     HIP_TRY(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));

@@ -95,8 +95,15 @@ __device__ __forceinline__ int64_t tile_id(const Work& W) {
 }
 
 // neighbour entry k of a slot (spill offset off is read only when the list spills)
+__device__ __forceinline__ int64_t nbl_index(const Work& W, int64_t slot, int k) {
+  return (int64_t(k >> 2) * W.cap_total + slot) * 4 + (k & 3);
+}
 __device__ __forceinline__ uint32_t nb_entry(const Work& W, int64_t slot, int k, int32_t off) {
-  return k < W.K ? uint32_t(W.nbl[int64_t(k) * W.cap_total + slot]) : uint32_t(W.spill[int64_t(off) + (k - W.K)]);
+  return k < W.K ? uint32_t(W.nbl[nbl_index(W, slot, k)]) : uint32_t(W.spill[int64_t(off) + (k - W.K)]);
+}
+// entries 4g .. 4g + 3 of a slot, g < K / 4, in one 16-byte load
+__device__ __forceinline__ uint4 nb_group(const Work& W, int64_t slot, int g) {
+  return reinterpret_cast<const uint4*>(W.nbl)[int64_t(g) * W.cap_total + slot];
 }
 
 // |v| dt2 / dx of one particle, its fixed-point form for the order-free sum, and |v| dt2
@@ -449,6 +456,10 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
     for (int a = 0; a < D; a++) x[a] = double(S.xh[a][slot]) + double(S.xl[a][slot]);
     double h = S.h[slot];
     int type = S.type[slot];
+    // the build position for the drift test, loaded with the particle's state so that
+    // its latency overlaps the level-set gathers
+    const float4 b = mode == kModeStep ? W.recb[slot] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool built = mode == kModeStep && W.red[j].built;
     if (mode == kModeStep) {
       // the two Update_velocity_half_step of the previous step, then Update_coord_full_step
       const double dt1 = bitsd(W.red[j].dt1[par ^ 1]), dt2 = bitsd(W.red[j].dt2[par ^ 1]);
@@ -557,7 +568,6 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
       // reaches the limit becomes a jumper (its pairs are found without the lists until
       // the next build); the job rebuilds when the other particles' drift reaches it or
       // when there are more than kJumpMax jumpers.
-      const float4 b = W.recb[slot];
       if (b.w < 0.f) {
         jump = true;
       } else {
@@ -567,7 +577,7 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
                          J.cut_off * fabs(h - double(b.w));
         double rho = e / double(b.w) * (1.0 + 1.e-5) + 1.e-9;
         rho = (rho <= 1.e300) ? rho : 1.e300;   // also for b.w = 0 or NaN
-        if (rho >= double(J.skin_lim) && J.skin_lim >= 0.f && W.red[j].built) {
+        if (rho >= double(J.skin_lim) && J.skin_lim >= 0.f && built) {
           const int ji = atomicAdd(&W.red[j].njump, 1);
           if (ji < kJumpMax) {
             W.jset[int64_t(j) * kJumpMax + ji] = int32_t(slot);
@@ -948,9 +958,20 @@ __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
   if (loc < J.n) {
     const int64_t cap = W.cap_total;
     const int K = W.K;
+    // each group of four inline entries is collected in registers and written with
+    // one 16-byte store; a last partial group is written as it stands
+    uint4* nb4 = reinterpret_cast<uint4*>(W.nbl);
+    uint4 grp = make_uint4(0u, 0u, 0u, 0u);
     cnt = nb_enumerate<D>(W, S, J, slot, [&](int k, int32_t e) {
-      if (k < K) W.nbl[int64_t(k) * cap + slot] = e;
+      if (k >= K) return;
+      const int u = k & 3;
+      grp.x = u == 0 ? uint32_t(e) : grp.x;
+      grp.y = u == 1 ? uint32_t(e) : grp.y;
+      grp.z = u == 2 ? uint32_t(e) : grp.z;
+      grp.w = u == 3 ? uint32_t(e) : grp.w;
+      if (u == 3) nb4[int64_t(k >> 2) * cap + slot] = grp;
     });
+    if ((cnt & 3) && cnt < K) nb4[int64_t(cnt >> 2) * cap + slot] = grp;
     if (cnt > K) {
       const unsigned long long qi = atomicAdd(&W.ctr[0], 1ull);
       W.ovq[qi] = int32_t(slot);   // at most one entry per slot: qi < cap_total
@@ -1224,23 +1245,35 @@ __device__ __forceinline__ void for_each_neighbor(const Work& W, const PState& S
     // the caller walked the list itself
   } else if (!ijump) {
     const int cnt = W.nbc[slot];
-    const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
-    // software pipelined: the next entry and its record load while this one is used
-    uint32_t en = cnt > 0 ? nb_entry(W, slot, 0, off) : 0u;
-    float4 qhn = make_float4(0.f, 0.f, 0.f, 0.f), qln = qhn;
-    if (cnt > 0) {
-      qhn = W.rec[4 * int64_t(en & kJMask) + 0];
-      qln = W.rec[4 * int64_t(en & kJMask) + 1];
-    }
-    for (int k = 0; k < cnt; k++) {
-      const uint32_t e = en;
-      const float4 qh = qhn, ql = qln;
-      if (k + 1 < cnt) {
-        en = nb_entry(W, slot, k + 1, off);
-        qhn = W.rec[4 * int64_t(en & kJMask) + 0];
-        qln = W.rec[4 * int64_t(en & kJMask) + 1];
+    const int ni = cnt < W.K ? cnt : W.K;
+    // inline entries in groups of four: one 16-byte load brings a group, its four
+    // records are gathered together, then the next group's load is issued while this
+    // group is used.  Jumpers come from the jumper scan.
+    uint4 gn = make_uint4(0u, 0u, 0u, 0u);
+    if (ni > 0) gn = nb_group(W, slot, 0);
+    for (int g = 0; 4 * g < ni; g++) {
+      const uint32_t ev[4] = {gn.x, gn.y, gn.z, gn.w};
+      float4 qh[4], ql[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        qh[u] = ql[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (4 * g + u < ni) {
+          const int64_t q = ev[u] & kJMask;
+          qh[u] = W.rec[4 * q + 0];
+          ql[u] = W.rec[4 * q + 1];
+        }
       }
-      if (__float_as_int(ql.w) & kJumpBit) continue;   // jumpers come from the jumper scan
+      if (4 * g + 4 < ni) gn = nb_group(W, slot, g + 1);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (4 * g + u >= ni || (__float_as_int(ql[u].w) & kJumpBit)) continue;
+        visit_rec(4 * g + u, ev[u] & kJMask, ev[u] >> kJBits, qh[u], ql[u]);
+      }
+    }
+    for (int k = ni; k < cnt; k++) {   // the spilled tail
+      const uint32_t e = uint32_t(W.spill[int64_t(W.nbo[slot]) + (k - W.K)]);
+      const float4 qh = W.rec[4 * int64_t(e & kJMask) + 0], ql = W.rec[4 * int64_t(e & kJMask) + 1];
+      if (__float_as_int(ql.w) & kJumpBit) continue;
       visit_rec(k, e & kJMask, e >> kJBits, qh, ql);
     }
   } else {
@@ -1328,6 +1361,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
 #pragma unroll
       for (int a = 0; a < D; a++) ni[a] = S.nrm[a][slot];
     }
+    const float nub = W.nu_b[slot];   // loaded early: stored with a1 after the loop
     const float dxi = h * J.kdx;
     const float vol2i = ipow<D>(dxi) * ipow<D>(dxi);
     const float ih = 1.f / h;
@@ -1421,7 +1455,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
       W.a1[a][slot] = F[a];
       A += double(F[a]) * double(F[a]);
     }
-    W.rec[4 * slot + 2] = make_float4(F[0], F[1], D > 2 ? F[2] : 0.f, W.nu_b[slot]);
+    W.rec[4 * slot + 2] = make_float4(F[0], F[1], D > 2 ? F[2] : 0.f, nub);
     A = sqrt(A);
     // Set_timestep(1): v was zeroed by Reset_particle_force(1), so U = 0
     const double t1 = 0.25 * sqrt(hd / (A + 1.e-15));
@@ -1549,44 +1583,58 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
         pair(q, code, false);
       };
       if (!ijump) {
-        // the masked entries, software pipelined: the next entry and its record load
-        // while this one is used
+        // the masked entries by groups of four (k = 4g .. 4g + 3): the group of the
+        // lowest remaining mask bit, with its four mask bits taken out of m0 / m1; the
+        // records of a group's masked entries are gathered together, then the next
+        // group's load is issued while this group is used
         static_assert(kMaskWords == 2, "two mask words");
         uint64_t m0 = W.nbm[slot];
         uint64_t m1 = 64 < cnt ? W.nbm[W.cap_total + slot] : 0ull;
-        auto next_k = [&]() {
+        auto next_group = [&](uint32_t& nib) -> int {
+          int g = -1;
           if (m0) {
-            const int k = __builtin_ctzll(m0);
-            m0 &= m0 - 1;
-            return k;
+            g = __builtin_ctzll(m0) >> 2;
+            nib = uint32_t(m0 >> (4 * g)) & 0xFu;
+            m0 &= ~(0xFull << (4 * g));
+          } else if (m1) {
+            const int g1 = __builtin_ctzll(m1) >> 2;
+            nib = uint32_t(m1 >> (4 * g1)) & 0xFu;
+            m1 &= ~(0xFull << (4 * g1));
+            g = 16 + g1;
           }
-          if (m1) {
-            const int k = 64 + __builtin_ctzll(m1);
-            m1 &= m1 - 1;
-            return k;
-          }
-          return -1;
+          return g;
         };
-        int kn = next_k();
-        uint32_t en = 0u;
-        float4 qh = make_float4(0.f, 0.f, 0.f, 0.f), ql = qh, qv = qh;
-        if (kn >= 0) {
-          en = nb_entry(W, slot, kn, off);
-          qh = W.rec[4 * int64_t(en & kJMask) + 0];
-          ql = W.rec[4 * int64_t(en & kJMask) + 1];
-          qv = W.rec[4 * int64_t(en & kJMask) + 2];
-        }
-        while (kn >= 0) {
-          const uint32_t e = en;
-          const float4 ch = qh, cl = ql, cv = qv;
-          kn = next_k();
-          if (kn >= 0) {
-            en = nb_entry(W, slot, kn, off);
-            qh = W.rec[4 * int64_t(en & kJMask) + 0];
-            ql = W.rec[4 * int64_t(en & kJMask) + 1];
-            qv = W.rec[4 * int64_t(en & kJMask) + 2];
+        auto load_group = [&](int g, uint32_t nib) -> uint4 {
+          if (4 * g < W.K) return nb_group(W, slot, g);
+          uint32_t ev[4] = {0u, 0u, 0u, 0u};   // a spilled group (K < 128): its masked entries
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            if ((nib >> u) & 1u) ev[u] = uint32_t(W.spill[int64_t(off) + (4 * g + u - W.K)]);
+          return make_uint4(ev[0], ev[1], ev[2], ev[3]);
+        };
+        uint32_t nibn = 0u;
+        int gn = next_group(nibn);
+        uint4 en = make_uint4(0u, 0u, 0u, 0u);
+        if (gn >= 0) en = load_group(gn, nibn);
+        while (gn >= 0) {
+          const uint32_t nib = nibn;
+          const uint32_t ev[4] = {en.x, en.y, en.z, en.w};
+          float4 qh[4], ql[4], qv[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            qh[u] = ql[u] = qv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if ((nib >> u) & 1u) {
+              const int64_t q = ev[u] & kJMask;
+              qh[u] = W.rec[4 * q + 0];
+              ql[u] = W.rec[4 * q + 1];
+              qv[u] = W.rec[4 * q + 2];
+            }
           }
-          pair_rec(e & kJMask, e >> kJBits, true, ch, cl, cv);
+          gn = next_group(nibn);
+          if (gn >= 0) en = load_group(gn, nibn);
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            if ((nib >> u) & 1u) pair_rec(ev[u] & kJMask, ev[u] >> kJBits, true, qh[u], ql[u], qv[u]);
         }
         for (int k = 64 * kMaskWords; k < cnt; k++) {
           const uint32_t e = nb_entry(W, slot, k, off);

@@ -61,7 +61,7 @@ extern "C" {
 
 int smg_server_footprint_of(const smg_server_config* c, int64_t* bytes) {
   if (!c || !bytes) { set_error("smg_server_footprint_of: NULL argument"); return SMG_ERR_ARG; }
-  const Sizes z = sizes_for(c->max_neighbors > 0 ? c->max_neighbors : 128, c->spill_per_slot > 0 ? c->spill_per_slot : 64);
+  const Sizes z = sizes_for(c->max_neighbors > 0 ? (c->max_neighbors + 3) / 4 * 4 : 128, c->spill_per_slot > 0 ? c->spill_per_slot : 64);
   *bytes = c->max_slots * z.per_slot + c->max_table * z.per_table + c->max_lset_cells * z.per_cell +
            int64_t(c->max_jobs) * z.per_job;
   return SMG_OK;
