@@ -28,6 +28,21 @@ struct alignas(8) LsCell {
 };
 static_assert(sizeof(LsCell) == 24, "LsCell layout");
 
+// Level-set cells are stored in bricks of 4 x 4 (x 4 in 3D) cells, so that the 2^D
+// corners of a sample, and the samples of nearby particles, share few cache lines.
+// Cell (a, b, c) of a job's one-cell-halo grid of ng[0] x ng[1] x ng[2] cells sits at
+// this offset from the job's first cell; bricks run x fastest, cells inside a brick
+// too, and the grid is padded to whole bricks.  (constexpr: host and device.)
+constexpr int64_t ls_cell_offset(int D, const int32_t* ng, int a, int b, int c) {
+  const int64_t nb0 = (ng[0] + 3) >> 2, nb1 = (ng[1] + 3) >> 2;
+  const int bz = D > 2 ? 4 : 1;
+  const int64_t brick = (int64_t(c >> 2) * nb1 + (b >> 2)) * nb0 + (a >> 2);
+  return brick * (16 * bz) + ((((c & 3) << 2) + (b & 3)) << 2) + (a & 3);
+}
+constexpr int64_t ls_cells_padded(int D, const int32_t* ng) {
+  return int64_t((ng[0] + 3) >> 2) * ((ng[1] + 3) >> 2) * (D > 2 ? (ng[2] + 3) >> 2 : 1) * (D > 2 ? 64 : 16);
+}
+
 struct JobParams {
   int32_t D, nlev, profile, remap_max;
   int32_t sym[3], ng[3];

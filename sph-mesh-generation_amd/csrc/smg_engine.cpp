@@ -150,7 +150,7 @@ int make_job_params(const JobInput& in, int D, JobParams& J) {
   return SMG_OK;
 }
 
-int64_t job_lset_cells(const JobParams& J) { return int64_t(J.ng[0]) * J.ng[1] * J.ng[2]; }
+int64_t job_lset_cells(const JobParams& J) { return ls_cells_padded(J.D, J.ng); }
 
 // 4096-entry scan blocks of a job's table segment (+ one entry for a sub-scan's end
 // sentinel); segments start on block boundaries so that no scan block spans two jobs
@@ -260,15 +260,18 @@ int Engine::upload_job_(int j, const JobInput& in) {
   jp_[j].spill_cap = jp_[j].cap * sps_;
   const JobParams& J = jp_[j];
   const int64_t m = job_lset_cells(J);
-  std::vector<LsCell> hls(m);
-  for (int64_t q = 0; q < m; q++) {
-    LsCell& cc = hls[q];
-    cc.phi = in.phi[q];
-    cc.n[0] = in.normal[3 * q + 0];
-    cc.n[1] = in.normal[3 * q + 1];
-    cc.n[2] = in.normal[3 * q + 2];
-    cc.tag = in.tag[q];
-  }
+  std::vector<LsCell> hls(m, LsCell{0., {0.f, 0.f, 0.f}, 0});   // the padding stays zero
+  int64_t q = 0;   // the caller's grid: x fastest
+  for (int c = 0; c < J.ng[2]; c++)
+    for (int b = 0; b < J.ng[1]; b++)
+      for (int a = 0; a < J.ng[0]; a++, q++) {
+        LsCell& cc = hls[ls_cell_offset(J.D, J.ng, a, b, c)];
+        cc.phi = in.phi[q];
+        cc.n[0] = in.normal[3 * q + 0];
+        cc.n[1] = in.normal[3 * q + 1];
+        cc.n[2] = in.normal[3 * q + 2];
+        cc.tag = in.tag[q];
+      }
   HIP_TRY(hipMemcpyAsync(d_ls_ + J.lset_off, hls.data(), sizeof(LsCell) * m, hipMemcpyHostToDevice, st_));
   HIP_TRY(hipMemcpyAsync(d_jp_ + j, &J, sizeof(JobParams), hipMemcpyHostToDevice, st_));
   const int64_t t0 = J.p0 / kTile, nt = J.cap / kTile;

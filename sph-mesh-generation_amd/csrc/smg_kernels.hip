@@ -121,8 +121,8 @@ __device__ __forceinline__ Disp disp_of(double U, double dt2, double h, const Jo
 }
 
 __device__ __forceinline__ int64_t ls_index(const JobParams& J, int D, int gx, int gy, int gz) {
-  const int64_t a = gx + 1, b = (D > 1) ? gy + 1 : 0, c = (D > 2) ? gz + 1 : 0;
-  return J.lset_off + (c * J.ng[1] + b) * J.ng[0] + a;
+  const int a = gx + 1, b = (D > 1) ? gy + 1 : 0, c = (D > 2) ? gz + 1 : 0;
+  return J.lset_off + ls_cell_offset(D, J.ng, a, b, c);
 }
 
 // clamped shift into the level-set box (particle_mesh_generation.cpp:182-185) and
@@ -310,27 +310,24 @@ __device__ __forceinline__ float dWk(float r, float h) {   // (dW/dr) / (r + 1e-
 // pair's mean h with one reciprocal.
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// Both branches are evaluated and selected (no divergent control flow); t = 0 for
+// s >= 2 makes the outer branch the zero of the kernel's support.
 template <int DK>
 __device__ __forceinline__ float Wk_s(float s, float C) {
-  if (s < 1.f) return C * (6.0f - 6.0f * s + s * s * s);
-  if (s < 2.f) {
-    const float t = 2.f - s;
-    return C * t * t * t;
-  }
-  return 0.f;
+  const float t = fmaxf(2.f - s, 0.f);
+  const float a = C * (6.0f - 6.0f * s + s * s * s);
+  const float b = C * t * t * t;
+  return s < 1.f ? a : b;
 }
 template <int DK>
 __device__ __forceinline__ float dWk_f(float r, float h) {
   const float ih = frcp(h);
   const float C = kC<DK>() * ipow<DK + 1>(ih);
   const float s = r * ih;
-  float v = 0.f;
-  if (s < 1.f) v = C * (-6.0f + 3.f * s * s);
-  else if (s < 2.f) {
-    const float t = 2.f - s;
-    v = C * (-3.0f) * t * t;
-  }
-  return v * frcp(r + 1.e-20f);
+  const float t = fmaxf(2.f - s, 0.f);
+  const float a = C * (-6.0f + 3.f * s * s);
+  const float b = C * (-3.0f) * t * t;
+  return (s < 1.f ? a : b) * frcp(r + 1.e-20f);
 }
 
 // relative position x_i - image(x_q) in FP32 from the hi/lo pairs; the image of a
@@ -624,36 +621,39 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
 // list rebuild decision per job: no lists yet, forced, or some particle's drift
 // reached skin * cut_off / 2 of its h at the last build
 
-__global__ void k_decide(Work W, int njobs, int force) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per job: the rebuild decision, and otherwise the job's jumpers put in
+// slot order (a rank sort in registers: the slots are distinct), so that every pass
+// visits them in the same order.
+static_assert(kJumpMax == 64, "one lane per jumper");
+__global__ __launch_bounds__(64) void k_decide(Work W, int njobs, int force) {
+  const int j = blockIdx.x, l = threadIdx.x;
   if (j >= njobs) return;
   JobRed& R = W.red[j];
   if (W.jp[j].n == 0) {   // an empty job index of a pool
-    R.rebuild = 0;
+    if (l == 0) R.rebuild = 0;
     return;
   }
   const int rb = force || !R.built || R.jover || bitsd(R.drift) >= double(W.jp[j].skin_lim);
-  R.rebuild = rb;
+  const int njump = R.njump;
+  __syncthreads();   // every lane has read R before lane 0 writes it
   if (rb) {
-    R.spill_used = 0ull;
-    R.njump = 0;
-    R.jover = 0;
-    if (!force) R.nrebuild += 1;
-    W.ctr[1] = 1ull;   // the table scan runs this step
-  } else {
-    // the jumpers in slot order, so every pass visits them in the same order
-    int32_t* js = W.jset + int64_t(j) * kJumpMax;
-    const int nj = R.njump < kJumpMax ? R.njump : kJumpMax;
-    for (int a = 1; a < nj; a++) {
-      const int32_t v = js[a];
-      int b = a - 1;
-      while (b >= 0 && js[b] > v) {
-        js[b + 1] = js[b];
-        b--;
-      }
-      js[b + 1] = v;
+    if (l == 0) {
+      R.rebuild = 1;
+      R.spill_used = 0ull;
+      R.njump = 0;
+      R.jover = 0;
+      if (!force) R.nrebuild += 1;
+      W.ctr[1] = 1ull;   // the table scan runs this step
     }
+    return;
   }
+  if (l == 0) R.rebuild = 0;
+  int32_t* js = W.jset + int64_t(j) * kJumpMax;
+  const int nj = njump < kJumpMax ? njump : kJumpMax;
+  const int32_t v = l < nj ? js[l] : 0x7fffffff;
+  int rank = 0;
+  for (int k = 0; k < nj; k++) rank += __shfl(v, k, 64) < v;
+  if (l < nj) js[rank] = v;
 }
 
 // cell key and count of every slot of a rebuilding job
@@ -1375,6 +1375,12 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
     uint64_t mask[kMaskWords];
 #pragma unroll
     for (int w = 0; w < kMaskWords; w++) mask[w] = 0ull;
+    // One straight-line body for both particle types.  A REAL particle has ni = 0, so
+    // its "projected" dr is dr itself and rt^2 = d2 bit for bit; a surface particle
+    // uses the (D-1)-dimensional kernel, whose constant kC<D-1> ih^D equals the D form's
+    // kC<D> ih^(D+1) with one factor ih dropped.
+    const float Ck = surf ? Cs : Cw;
+    const float kd = surf ? kC<D - 1>() : kC<D>();
     for_each_neighbor<D>(W, S, J, slot, jt, [&](int k, int64_t q, uint32_t code, const float4& qh, const float4& ql,
                                                  const float* dr0, float d2) {
       ntr++;
@@ -1387,52 +1393,45 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
         for (int w = 0; w < kMaskWords; w++)
           if ((k >> 6) == w) mask[w] |= 1ull << (k & 63);
       }
-      float dr[3] = {dr0[0], dr0[1], dr0[2]};
-      if (!surf) {
-        const float rr = fsqrt(d2);
-        sum += Wk_s<D>(rr * ih, Cw);
-        ncount++;
-        if (self) return;
-        const float hq = qh.w;
-        const float dxq = hq * J.kdx;
-        const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
-        const float aw = dWk_f<D>(rr, 0.5f * (h + hq)) * (vol2i + vol2q);
+      // a surface particle sees surface neighbours only (Accumulate_particle_force_surf),
+      // and of those only the ones with aligned normals
+      bool use = !(surf && tq == SMG_REAL);
+      if (surf && tq == SMG_SURFACE) {
+        const float4 qn = W.rec[4 * q + 3];
+        const float nq[3] = {qn.x, qn.y, qn.z};
+        float ca = 0.f;
 #pragma unroll
-        for (int a = 0; a < D; a++) G[a] += dr[a] * aw;
-      } else {
-        if (tq == SMG_REAL) return;   // a surface particle sees surface neighbours only
-        bool aligned = true;
-        if (tq == SMG_SURFACE) {
-          const float4 qn = W.rec[4 * q + 3];
-          const float nq[3] = {qn.x, qn.y, qn.z};
-          float ca = 0.f;
-#pragma unroll
-          for (int a = 0; a < D; a++) ca += ni[a] * nq[a];
-          aligned = fabsf(ca) > J.cos_thres;
-        }
-        if (!aligned) return;
-        float dn = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
-        float rt = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; a++) {
-          dr[a] = dr[a] - ni[a] * dn;
-          rt += dr[a] * dr[a];
-        }
-        rt = fsqrt(rt);
-        if (tq == SMG_SURFACE) {
-          sum += Wk_s<D - 1>(rt * ih, Cs);
-          ncount++;
-        }
-        if (self) return;
-        const float hq = qh.w;
-        const float dxq = hq * J.kdx;
-        const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
-        const float aw = dWk_f<D - 1>(rt, 0.5f * (h + hq)) * (vol2i + vol2q);
-#pragma unroll
-        for (int a = 0; a < D; a++) G[a] += dr[a] * aw;
+        for (int a = 0; a < D; a++) ca += ni[a] * nq[a];
+        use = fabsf(ca) > J.cos_thres;
       }
+      float dn = 0.f;
+#pragma unroll
+      for (int a = 0; a < D; a++) dn += dr0[a] * ni[a];
+      float dr[3] = {0.f, 0.f, 0.f};
+      float rt = 0.f;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        dr[a] = dr0[a] - ni[a] * dn;
+        rt += dr[a] * dr[a];
+      }
+      rt = fsqrt(rt);
+      const bool counts = use && (!surf || tq == SMG_SURFACE);   // the kernel summation
+      const float w = Wk_s<D>(rt * ih, Ck);
+      sum = counts ? sum + w : sum;
+      ncount += counts ? 1 : 0;
+      const float hq = qh.w;
+      const float dxq = hq * J.kdx;
+      const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
+      const float hm = 0.5f * (h + hq);
+      const float ihm = frcp(hm);
+      const float C = kd * (ipow<D>(ihm) * (surf ? 1.f : ihm));
+      const float s2 = rt * ihm;
+      const float t = fmaxf(2.f - s2, 0.f);
+      const float dw = (s2 < 1.f ? C * (-6.0f + 3.f * s2 * s2) : C * (-3.0f) * t * t) * frcp(rt + 1.e-20f);
+      const float aw = dw * (vol2i + vol2q);
+      const bool f = use && !self;
+#pragma unroll
+      for (int a = 0; a < D; a++) G[a] = f ? G[a] + dr[a] * aw : G[a];
     });
     if (ncount > 1) sum = 1.f / (sum + 1.e-15f);
     W.sum[slot] = sum;
@@ -1520,60 +1519,58 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
     float F[3] = {0.f, 0.f, 0.f};
     // one stage-2 pair term; checked = a list entry that the stage-1 masks marked as a
     // real (not mirrored), non-self neighbour that a surface particle keeps
+    // one straight-line body for both particle types, as in the stage-1 pass: with
+    // ni = 0 a REAL particle's projected rt^2 is its rr bit for bit
+    const float kd = surf ? kC<D - 1>() : kC<D>();
     auto pair_rec = [&](int64_t q, uint32_t code, bool checked, const float4& qh, const float4& ql,
                         const float4& qv) {
-      if (!checked && (code != 0 || q == slot)) return;   // ghosts carry nu = 0 (Q6)
       const int tq = __float_as_int(ql.w) & ~kJumpBit;
-      if (!checked && surf && tq == SMG_REAL) return;
+      // ghosts carry nu = 0 (Q6); a surface particle sees no REAL neighbour
+      bool use = checked || (code == 0 && q != slot && !(surf && tq == SMG_REAL));
       const float hq = qh.w;
       const float dxq = hq * J.kdx;
       const float volq = ipow<D>(dxq);
       const float rhoq = frcp(volq);
       const float etaq = qv.w * rhoq;
-      if (etaq == 0.f) return;   // eta_ij = 0
+      use = use && etaq != 0.f;   // eta_ij = 0
       float dr[3] = {0.f, 0.f, 0.f};
       rel_pos<D>(J, qh, ql, 0u, xhi, xli, dr);
       float rr = 0.f;
 #pragma unroll
       for (int a = 0; a < D; a++) rr += dr[a] * dr[a];
-      if (!checked && !is_neighbor<D>(J, W, S, q, 0u, rr, rfi, hq, xhi, xli, hd)) return;
-      const float hij = 0.5f * (h + hq);
-      float aw;
-      if (!surf) {
-        aw = dWk_f<D>(fsqrt(rr), hij);
-      } else {
-        if (tq == SMG_SURFACE) {
-          const float4 qn = W.rec[4 * q + 3];
-          const float nq[3] = {qn.x, qn.y, qn.z};
-          float ca = 0.f;
+      if (!checked && use) use = is_neighbor<D>(J, W, S, q, 0u, rr, rfi, hq, xhi, xli, hd);
+      if (surf && tq == SMG_SURFACE) {
+        const float4 qn = W.rec[4 * q + 3];
+        const float nq[3] = {qn.x, qn.y, qn.z};
+        float ca = 0.f;
 #pragma unroll
-          for (int a = 0; a < D; a++) ca += ni[a] * nq[a];
-          if (fabsf(ca) <= J.cos_thres) return;
-        }
-        float dn = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
-        float rt = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; a++) {
-          const float t = dr[a] - ni[a] * dn;
-          rt += t * t;
-        }
-        aw = dWk_f<D - 1>(fsqrt(rt), hij);
+        for (int a = 0; a < D; a++) ca += ni[a] * nq[a];
+        use = use && fabsf(ca) > J.cos_thres;
       }
-      const float qa[3] = {qv.x, qv.y, qv.z};
-      float vq[3];
-      if (!surf && tq != SMG_REAL) {   // Q8: v_j := -v_i
+      float dn = 0.f;
 #pragma unroll
-        for (int a = 0; a < D; a++) vq[a] = -vi[a];
-      } else {
+      for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
+      float rt = 0.f;
 #pragma unroll
-        for (int a = 0; a < D; a++) vq[a] = qa[a] * hdt;
+      for (int a = 0; a < D; a++) {
+        const float t = dr[a] - ni[a] * dn;
+        rt += t * t;
       }
+      rt = fsqrt(rt);
+      const float ihm = frcp(0.5f * (h + hq));
+      const float C = kd * (ipow<D>(ihm) * (surf ? 1.f : ihm));
+      const float s2 = rt * ihm;
+      const float t2 = fmaxf(2.f - s2, 0.f);
+      const float aw = (s2 < 1.f ? C * (-6.0f + 3.f * s2 * s2) : C * (-3.0f) * t2 * t2) * frcp(rt + 1.e-20f);
+      const bool mirror_v = !surf && tq != SMG_REAL;   // Q8: v_j := -v_i
       const float etaij = 2.f * etaq * etai * frcp(etai + etaq + 1.e-15f);
       const float temp0 = aw * etaij * (voli * voli + volq * volq);
 #pragma unroll
-      for (int a = 0; a < D; a++) F[a] += (vi[a] - vq[a]) * temp0;
+      for (int a = 0; a < D; a++) {
+        const float qa = a == 0 ? qv.x : (a == 1 ? qv.y : qv.z);
+        const float vq = mirror_v ? -vi[a] : qa * hdt;
+        F[a] = use ? F[a] + (vi[a] - vq) * temp0 : F[a];
+      }
     };
     auto pair = [&](int64_t q, uint32_t code, bool checked) {
       pair_rec(q, code, checked, W.rec[4 * q + 0], W.rec[4 * q + 1], W.rec[4 * q + 2]);
@@ -1857,7 +1854,7 @@ __global__ void k_fp64_probe(int64_t n, const double* a, const double* b, double
   } while (0)
 
 void launch_decide(hipStream_t st, const Work& W, int njobs, int force) {
-  k_decide<<<(njobs + 255) / 256, 256, 0, st>>>(W, njobs, force);
+  if (njobs > 0) k_decide<<<njobs, 64, 0, st>>>(W, njobs, force);
 }
 void launch_key(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S) {
   if (D == 2) k_key<2><<<ntiles, kTile, 0, st>>>(W, S);

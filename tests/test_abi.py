@@ -66,13 +66,14 @@ def test_no_cpu_fallback_without_gpu(smg_mod, c1):
 
 def test_server_sizing_is_host_only(smg_mod):
     """Pool sizing needs no GPU: a job's slots are n rounded up to the 256-slot tile,
-    its level-set cells (4 num_pkg + 2)^dim, its table at least one entry per finest
+    its level-set cells the (4 num_pkg + 2)^dim halo grid padded to whole 4^dim
+    bricks, its table at least one entry per finest
     cell plus the pad cell and the scan sentinel; the footprint grows with every
     capacity."""
     cfg, _ = smg_mod.config_c1(num_pkg=8, dx=0.05)
     slots, table, cells = smg_mod.job_size(cfg, 1000)
     assert slots == 1024
-    assert cells == (4 * 8 + 2) ** 2
+    assert cells == ((4 * 8 + 2 + 3) // 4 * 4) ** 2   # 34 -> 9 bricks of 4 per axis
     assert table >= 2 + 1
     base = smg_mod.server_config(3, 8, 1 << 16, 1 << 20, 1 << 18)
     b0 = smg_mod.server_footprint(base)
