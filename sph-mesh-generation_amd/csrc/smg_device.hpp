@@ -74,7 +74,24 @@ constexpr double kResScale = 1099511627776.0;   // 2^40
 
 // per-job reductions, double buffered by step parity; doubles stored as their bit
 // patterns (all values >= 0) so that atomicMin / atomicMax on the integer order work
+// Accumulators that every workgroup of a pass updates once, kept in kAccCopies copies
+// per job, each on cache lines of its own: workgroup b adds into copy b % kAccCopies,
+// so one launch's end-of-workgroup atomics spread over that many lines instead of
+// queueing on one.  The copies are the accumulation targets; readers fold them (min /
+// max / sum), which gives the same value in any order.  Field meanings as in JobRed.
+constexpr int kAccCopies = 16;
+struct alignas(256) JobAcc {
+  unsigned long long dt1[2], dt2[2], maxv[2];
+  unsigned long long res[2], maxdisp[2], rsum[2];
+  unsigned long long rcum, drift;
+  int32_t maxnb, pad_;
+};
+static_assert(sizeof(JobAcc) == 256, "one JobAcc per 256 bytes");
+
 struct JobRed {
+  // dt1 / dt2 / maxv of the previous step folded from the JobAcc copies at the start
+  // of each step (the integration in k_map reads them); every other field of the same
+  // name in JobAcc is read from the copies
   unsigned long long dt1[2], dt2[2], maxv[2];
   unsigned long long res[2];       // max_i |v_i| dt2 / dx_i
   unsigned long long maxdisp[2];   // max_i |v_i| dt2
@@ -85,7 +102,8 @@ struct JobRed {
   unsigned long long spill_used;   // entries of the job's spill part handed out
   int64_t nrebuild;                // list builds so far
   int32_t nremap[2];
-  int32_t fail, overflow, maxnb, nsurf;
+  int32_t fail, overflow;
+  int32_t maxnb;                   // host copy only: folded from the JobAcc copies
   int32_t rebuild;                 // this step re-sorts and rebuilds the lists
   int32_t built;                   // the lists match the current slot order
   int32_t njump;                   // jumpers since the last build (jset holds the first kJumpMax)
@@ -108,6 +126,7 @@ struct Work {
   const int32_t* tile_job;
   const int32_t* sblk_job;   // job owning each 4096-entry block of the cell tables (-1: none)
   JobRed* red;
+  JobAcc* acc;      // kAccCopies per job, [job * kAccCopies + copy]
   const LsCell* ls;
   float* nu_a;   // nu read by the stage-2 pass (previous step's value)
   float* nu_b;   // nu written by the stage-2 pass

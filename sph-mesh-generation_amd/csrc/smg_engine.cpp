@@ -174,6 +174,7 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   d_tile_job_ = alloc_<int32_t>(slots / kTile);
   d_sblk_job_ = alloc_<int32_t>((tbl + kScanItems - 1) / kScanItems + 1);
   d_red_ = alloc_<JobRed>(njobs);
+  d_acc_ = alloc_<JobAcc>(int64_t(njobs) * kAccCopies);
   d_ls_ = alloc_<LsCell>(n_ls_);
   PState* S2[2] = {&A_, &B_};
   for (PState* S : S2) {
@@ -225,12 +226,13 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   W_.tile_job = d_tile_job_;
   W_.sblk_job = d_sblk_job_;
   W_.red = d_red_;
+  W_.acc = d_acc_;
   W_.ls = d_ls_;
   W_.cap_total = cap_total_;
   W_.n_tbl = n_tbl_;
   W_.K = K_;
   W_.tile0 = 0;
-  HIP_TRY(hipMemsetAsync(d_red_, 0, sizeof(JobRed) * njobs, st_));
+  SMG_TRY(clear_reds_(0, njobs));
   HIP_TRY(hipMemsetAsync(d_jp_, 0, sizeof(JobParams) * njobs, st_));   // n = 0: no job
   HIP_TRY(hipMemsetAsync(d_tile_job_, 0xFF, sizeof(int32_t) * (slots / kTile), st_));   // -1: no job
   sblk_job_h_.assign((tbl + kScanItems - 1) / kScanItems + 1, -1);
@@ -251,7 +253,41 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
     HIP_TRY(hipMemsetAsync(W_.a2[a], 0, sizeof(float) * cap_total_, st_));
   }
   red_h_.assign(njobs, JobRed{});
+  acc_h_.assign(size_t(njobs) * kAccCopies, JobAcc{});
   return SMG_OK;
+}
+
+// reductions and their accumulator copies of jobs [j0, j0 + nj) to zero (dt = 0: the
+// first step after Pre_run moves nothing)
+int Engine::clear_reds_(int j0, int nj) {
+  HIP_TRY(hipMemsetAsync(d_red_ + j0, 0, sizeof(JobRed) * nj, st_));
+  HIP_TRY(hipMemsetAsync(d_acc_ + int64_t(j0) * kAccCopies, 0, sizeof(JobAcc) * nj * kAccCopies, st_));
+  return SMG_OK;
+}
+
+// the host copy of job j's reductions with the accumulator copies folded in
+void Engine::fold_acc_(int j) {
+  JobRed& R = red_h_[j];
+  const JobAcc* A = acc_h_.data() + size_t(j) * kAccCopies;
+  for (int p = 0; p < 2; p++) {
+    R.dt1[p] = R.dt2[p] = ~0ull;
+    R.maxv[p] = R.res[p] = R.maxdisp[p] = R.rsum[p] = 0ull;
+  }
+  R.rcum = R.drift = 0ull;
+  R.maxnb = 0;
+  for (int c = 0; c < kAccCopies; c++) {
+    for (int p = 0; p < 2; p++) {   // non-negative doubles order as their bit patterns
+      R.dt1[p] = std::min(R.dt1[p], A[c].dt1[p]);
+      R.dt2[p] = std::min(R.dt2[p], A[c].dt2[p]);
+      R.maxv[p] = std::max(R.maxv[p], A[c].maxv[p]);
+      R.res[p] = std::max(R.res[p], A[c].res[p]);
+      R.maxdisp[p] = std::max(R.maxdisp[p], A[c].maxdisp[p]);
+      R.rsum[p] += A[c].rsum[p];
+    }
+    R.rcum += A[c].rcum;
+    R.drift = std::max(R.drift, A[c].drift);
+    R.maxnb = std::max(R.maxnb, A[c].maxnb);
+  }
 }
 
 // level-set cells, parameters and tile ownership of job j (ranges already set in jp_[j])
@@ -446,6 +482,7 @@ int Engine::admit(const JobInput& in, int* job) {
   }
   // the job starts from clean reductions, as after Pre_run of a job of its own
   HIP_TRY(hipMemcpyAsync(d_red_ + j, &zero, sizeof(JobRed), hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipMemsetAsync(d_acc_ + int64_t(j) * kAccCopies, 0, sizeof(JobAcc) * kAccCopies, st_));
   HIP_TRY(hipStreamSynchronize(st_));
   *job = j;
   return SMG_OK;
@@ -493,7 +530,10 @@ int Engine::retire(int j) {
 
 int Engine::red_of(int j, JobRed* r) {
   HIP_TRY(hipMemcpyAsync(&red_h_[j], d_red_ + j, sizeof(JobRed), hipMemcpyDeviceToHost, st_));
+  HIP_TRY(hipMemcpyAsync(acc_h_.data() + size_t(j) * kAccCopies, d_acc_ + int64_t(j) * kAccCopies,
+                         sizeof(JobAcc) * kAccCopies, hipMemcpyDeviceToHost, st_));
   HIP_TRY(hipStreamSynchronize(st_));
+  fold_acc_(j);
   *r = red_h_[j];
   return SMG_OK;
 }
@@ -538,7 +578,7 @@ int Engine::sort_particles() {
 }
 
 int Engine::seed() {
-  HIP_TRY(hipMemsetAsync(d_red_, 0, sizeof(JobRed) * jp_.size(), st_));
+  SMG_TRY(clear_reds_(0, int(jp_.size())));
   launch_seed(st_, D_, ntiles_, W_, A_);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st_));
@@ -582,7 +622,7 @@ int Engine::init_positions(int job, const double* x, const int32_t* type) {
 
 int Engine::init_finish() {
   // Set_particle_info at the given positions, cell keys, first sort
-  HIP_TRY(hipMemsetAsync(d_red_, 0, sizeof(JobRed) * jp_.size(), st_));
+  SMG_TRY(clear_reds_(0, int(jp_.size())));
   launch_map(st_, D_, ntiles_, W_, A_, kModeInit, 0);
   HIP_TRY(hipGetLastError());
   SMG_TRY(sort_particles());
@@ -595,7 +635,7 @@ int Engine::init_finish() {
 int Engine::prerun() {
   // Pre_run: Get_particle_info, Set_particle_scale, Find_and_map_particle; the force
   // passes that follow see sum = 0 and nu = 0 and leave a = 0, v = 0, nu = 0.
-  HIP_TRY(hipMemsetAsync(d_red_, 0, sizeof(JobRed) * jp_.size(), st_));
+  SMG_TRY(clear_reds_(0, int(jp_.size())));
   begin_("k_map");
   launch_map(st_, D_, ntiles_, W_, A_, kModePrerun, 0);
   end_();
@@ -720,7 +760,9 @@ int Engine::query(bool* busy) {
 
 int Engine::download_red_() {
   HIP_TRY(hipMemcpyAsync(red_h_.data(), d_red_, sizeof(JobRed) * jp_.size(), hipMemcpyDeviceToHost, st_));
+  HIP_TRY(hipMemcpyAsync(acc_h_.data(), d_acc_, sizeof(JobAcc) * acc_h_.size(), hipMemcpyDeviceToHost, st_));
   HIP_TRY(hipStreamSynchronize(st_));
+  for (size_t j = 0; j < red_h_.size(); j++) fold_acc_(int(j));
   return SMG_OK;
 }
 

@@ -121,6 +121,8 @@ class Engine {
   void begin_(const char* name);
   void end_();
   int download_red_();
+  int clear_reds_(int j0, int nj);
+  void fold_acc_(int j);
 
   int D_ = 0;
   hipStream_t st_ = nullptr;
@@ -143,12 +145,14 @@ class Engine {
   int32_t* d_sblk_job_ = nullptr;
   std::vector<int32_t> sblk_job_h_;
   JobRed* d_red_ = nullptr;
+  JobAcc* d_acc_ = nullptr;
   LsCell* d_ls_ = nullptr;
   PState A_{}, B_{};
   Work W_{};
   std::vector<void*> allocs_;
   int64_t bytes_ = 0;
   std::vector<JobRed> red_h_;
+  std::vector<JobAcc> acc_h_;
 
   // profiling
   bool profiling_ = false;

@@ -349,24 +349,62 @@ __device__ __forceinline__ void rel_pos(const JobParams& J, const float4& qh, co
 // ---------------------------------------------------------------------------
 // reduction slots of this step
 
-__global__ void k_reset(JobRed* red, int njobs, int par, unsigned long long* ctr) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j == 0) {   // spill queue and the any-job-rebuilds flag of this step
-    ctr[0] = 0ull;
-    ctr[1] = 0ull;
+// the accumulator copies of job j, and the copy workgroup b adds into
+__device__ __forceinline__ JobAcc* acc_of(const Work& W, int j) { return W.acc + int64_t(j) * kAccCopies; }
+__device__ __forceinline__ JobAcc& acc_mine(const Work& W, int j) { return W.acc[int64_t(j) * kAccCopies + blockIdx.x % kAccCopies]; }
+// dt1 of this step folded from the copies (wave-uniform scalar loads)
+__device__ __forceinline__ double acc_dt(const Work& W, int j, int par, bool second) {
+  const JobAcc* A = acc_of(W, j);
+  unsigned long long m = second ? A[0].dt2[par] : A[0].dt1[par];
+#pragma unroll
+  for (int c = 1; c < kAccCopies; c++) {
+    const unsigned long long v = second ? A[c].dt2[par] : A[c].dt1[par];
+    m = v < m ? v : m;   // non-negative doubles order as their bit patterns
+  }
+  return bitsd(m);
+}
+template <typename Op>
+__device__ __forceinline__ unsigned long long wave_fold(unsigned long long v, Op op) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = op(v, (unsigned long long)__shfl_xor((long long)v, o, 64));
+  return v;
+}
+struct OpMinU { __device__ unsigned long long operator()(unsigned long long a, unsigned long long b) const { return a < b ? a : b; } };
+struct OpMaxU { __device__ unsigned long long operator()(unsigned long long a, unsigned long long b) const { return a > b ? a : b; } };
+
+// Start of a step, one wavefront per job: the previous step's dt1, dt2 and max v
+// folded from the copies into JobRed (the integration in k_map reads them), then this
+// step's accumulators cleared.
+__global__ __launch_bounds__(64) void k_reset(Work W, int njobs, int par) {
+  const int j = blockIdx.x, l = threadIdx.x;
+  if (j == 0 && l == 0) {   // spill queue and the any-job-rebuilds flag of this step
+    W.ctr[0] = 0ull;
+    W.ctr[1] = 0ull;
   }
   if (j >= njobs) return;
-  JobRed& R = red[j];
-  R.step += 1;   // re-seeding RNG stream of this step (graph replays advance it too)
-  R.drift = 0ull;
-  R.rebuild = 0;
-  R.dt1[par] = 0x7FF0000000000000ull;
-  R.dt2[par] = 0x7FF0000000000000ull;
-  R.maxv[par] = 0ull;
-  R.res[par ^ 1] = 0ull;
-  R.maxdisp[par ^ 1] = 0ull;
-  R.rsum[par ^ 1] = 0ull;
-  R.nremap[par] = 0;
+  JobRed& R = W.red[j];
+  JobAcc* A = acc_of(W, j);
+  const bool c = l < kAccCopies;
+  const unsigned long long d1 = wave_fold(c ? A[l].dt1[par ^ 1] : ~0ull, OpMinU());
+  const unsigned long long d2 = wave_fold(c ? A[l].dt2[par ^ 1] : ~0ull, OpMinU());
+  const unsigned long long mv = wave_fold(c ? A[l].maxv[par ^ 1] : 0ull, OpMaxU());
+  if (c) {
+    A[l].dt1[par] = 0x7FF0000000000000ull;
+    A[l].dt2[par] = 0x7FF0000000000000ull;
+    A[l].maxv[par] = 0ull;
+    A[l].res[par ^ 1] = 0ull;
+    A[l].maxdisp[par ^ 1] = 0ull;
+    A[l].rsum[par ^ 1] = 0ull;
+    A[l].drift = 0ull;
+  }
+  if (l == 0) {
+    R.step += 1;   // re-seeding RNG stream of this step (graph replays advance it too)
+    R.rebuild = 0;
+    R.dt1[par ^ 1] = d1;
+    R.dt2[par ^ 1] = d2;
+    R.maxv[par ^ 1] = mv;
+    R.nremap[par] = 0;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -446,7 +484,7 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
   const int64_t loc = slot - J.p0;
   double res = 0., md = 0., drift = 0.;
   unsigned long long rq = 0ull;
-  int nre = 0, bad = 0, nsurf = 0;
+  int nre = 0, bad = 0;
   if (loc < J.n) {
     double x[3] = {0., 0., 0.};
 #pragma unroll
@@ -551,7 +589,6 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
     }
     S.h[slot] = h;
     S.type[slot] = type;
-    nsurf = type == SMG_SURFACE;
     float hp[3] = {0.f, 0.f, 0.f}, lp[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int a = 0; a < D; a++) {
@@ -599,21 +636,20 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
   rq = block_reduce(rq, OpAddU(), lds_u);
   nre = block_reduce(nre, OpAdd(), lds_i);
   bad = block_reduce(bad, OpMaxI(), lds_i);
-  nsurf = block_reduce(nsurf, OpAdd(), lds_i);
   if (threadIdx.x == 0) {
     JobRed& R = W.red[j];
     if (mode == kModeStep) {   // residual of the previous step, whose motion this pass applied
-      atomicMax(&R.drift, dbits(drift));
-      atomicMax(&R.res[par ^ 1], dbits(res));
-      atomicMax(&R.maxdisp[par ^ 1], dbits(md));
+      JobAcc& A = acc_mine(W, j);
+      atomicMax(&A.drift, dbits(drift));
+      atomicMax(&A.res[par ^ 1], dbits(res));
+      atomicMax(&A.maxdisp[par ^ 1], dbits(md));
       if (rq) {
-        atomicAdd(&R.rsum[par ^ 1], rq);
-        atomicAdd(&R.rcum, rq);
+        atomicAdd(&A.rsum[par ^ 1], rq);
+        atomicAdd(&A.rcum, rq);
       }
     }
     if (nre) atomicAdd(&R.nremap[par], nre);
     if (bad) atomicOr(&R.fail, 1);
-    if (nsurf) atomicAdd(&R.nsurf, nsurf);
   }
 }
 
@@ -633,7 +669,9 @@ __global__ __launch_bounds__(64) void k_decide(Work W, int njobs, int force) {
     if (l == 0) R.rebuild = 0;
     return;
   }
-  const int rb = force || !R.built || R.jover || bitsd(R.drift) >= double(W.jp[j].skin_lim);
+  const JobAcc* A = acc_of(W, j);
+  const unsigned long long drift = wave_fold(l < kAccCopies ? A[l].drift : 0ull, OpMaxU());
+  const int rb = force || !R.built || R.jover || bitsd(drift) >= double(W.jp[j].skin_lim);
   const int njump = R.njump;
   __syncthreads();   // every lane has read R before lane 0 writes it
   if (rb) {
@@ -1464,8 +1502,9 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
   dt = block_reduce(dt, OpMin(), lds_d);
   const int mx = block_reduce(ntr, OpMaxI(), lds_i);
   if (threadIdx.x == 0) {
-    if (dt < 1.0e300) atomicMin(&W.red[j].dt1[par], dbits(dt));
-    if (mx > 0) atomicMax(&W.red[j].maxnb, mx);
+    JobAcc& A = acc_mine(W, j);
+    if (dt < 1.0e300) atomicMin(&A.dt1[par], dbits(dt));
+    if (mx > 0) atomicMax(&A.maxnb, mx);
   }
 }
 
@@ -1488,7 +1527,7 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
   }
   double dt = 1.0e300, U = 0.;
   if (live) {
-    const double dt1 = bitsd(W.red[j].dt1[par]);
+    const double dt1 = acc_dt(W, j, par, false);
     const float hdt = float(0.5 * dt1);
     const float4 r0 = W.rec[4 * int64_t(slot) + 0], r1 = W.rec[4 * int64_t(slot) + 1], r2 = W.rec[4 * int64_t(slot) + 2];
     const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
@@ -1669,8 +1708,9 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
   dt = block_reduce(dt, OpMin(), lds_d);
   U = block_reduce(U, OpMax(), lds_d);
   if (threadIdx.x == 0 && dt < 1.0e300) {
-    atomicMin(&W.red[j].dt2[par], dbits(dt));
-    atomicMax(&W.red[j].maxv[par], dbits(U));
+    JobAcc& A = acc_mine(W, j);
+    atomicMin(&A.dt2[par], dbits(dt));
+    atomicMax(&A.maxv[par], dbits(U));
   }
 }
 
@@ -1801,7 +1841,7 @@ __global__ __launch_bounds__(kTile) void k_residual(Work W, PState S, int par) {
   double res = 0., md = 0.;
   unsigned long long rq = 0ull;
   if (slot - J.p0 < J.n) {
-    const double dt1 = bitsd(W.red[j].dt1[par]), dt2 = bitsd(W.red[j].dt2[par]);
+    const double dt1 = acc_dt(W, j, par, false), dt2 = acc_dt(W, j, par, true);
     double U = 0.;
 #pragma unroll
     for (int a = 0; a < D; a++) {
@@ -1819,18 +1859,19 @@ __global__ __launch_bounds__(kTile) void k_residual(Work W, PState S, int par) {
   md = block_reduce(md, OpMax(), lds_d);
   rq = block_reduce(rq, OpAddU(), lds_u);
   if (threadIdx.x == 0) {
-    atomicMax(&W.red[j].res[par], dbits(res));
-    atomicMax(&W.red[j].maxdisp[par], dbits(md));
-    if (rq) atomicAdd(&W.red[j].rsum[par], rq);
+    JobAcc& A = acc_mine(W, j);
+    atomicMax(&A.res[par], dbits(res));
+    atomicMax(&A.maxdisp[par], dbits(md));
+    if (rq) atomicAdd(&A.rsum[par], rq);
   }
 }
 
-__global__ void k_clear_res(JobRed* red, int njobs, int par) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < njobs) {
-    red[j].res[par] = 0ull;
-    red[j].maxdisp[par] = 0ull;
-    red[j].rsum[par] = 0ull;
+__global__ void k_clear_res(Work W, int njobs, int par) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;   // job * kAccCopies + copy
+  if (i < int64_t(njobs) * kAccCopies) {
+    W.acc[i].res[par] = 0ull;
+    W.acc[i].maxdisp[par] = 0ull;
+    W.acc[i].rsum[par] = 0ull;
   }
 }
 
@@ -1875,7 +1916,7 @@ void launch_jump_lists(hipStream_t st, int D, const Work& W, const PState& S, in
   else k_jump_lists<3><<<g, 64, 0, st>>>(W, S);
 }
 void launch_reset(hipStream_t st, const Work& W, int njobs, int par) {
-  k_reset<<<(njobs + 255) / 256, 256, 0, st>>>(W.red, njobs, par, W.ctr);
+  if (njobs > 0) k_reset<<<njobs, 64, 0, st>>>(W, njobs, par);
 }
 void launch_seed(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S) {
   if (D == 2) k_seed<2><<<ntiles, kTile, 0, st>>>(W, S);
@@ -1922,7 +1963,7 @@ void launch_force2(hipStream_t st, int D, int64_t ntiles, const Work& W, const P
   else k_force2<3><<<ntiles, kTile, 0, st>>>(W, S, par);
 }
 void launch_residual(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int njobs, int par) {
-  k_clear_res<<<(njobs + 255) / 256, 256, 0, st>>>(W.red, njobs, par);
+  k_clear_res<<<(njobs * kAccCopies + 255) / 256, 256, 0, st>>>(W, njobs, par);
   if (D == 2) k_residual<2><<<ntiles, kTile, 0, st>>>(W, S, par);
   else k_residual<3><<<ntiles, kTile, 0, st>>>(W, S, par);
 }
