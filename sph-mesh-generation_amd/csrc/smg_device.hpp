@@ -20,13 +20,10 @@ constexpr int kJumpMax = 64;     // particles per job that may outrun their list
 constexpr int kJumpList = 512;   // candidate entries of one jumper's own list
 constexpr int kJumpBit = 1 << 30;   // flag in the type bits of a record: the slot is a jumper
 
-// one level-set cell: phi (FP64, used for projection), raw normal, interface tag
-struct alignas(8) LsCell {
-  double phi;
-  float n[3];
-  int32_t tag;
-};
-static_assert(sizeof(LsCell) == 24, "LsCell layout");
+// Level-set cells live in two arrays of the same (bricked) cell order: phi in FP64
+// (used for projection) and float4 (raw normal xyz, interface tag as int bits).  The
+// interior particles' samples need phi only and read 8 bytes per corner.
+constexpr int64_t kLsCellBytes = 8 + 16;
 
 // Level-set cells are stored in bricks of 4 x 4 (x 4 in 3D) cells, so that the 2^D
 // corners of a sample, and the samples of nearby particles, share few cache lines.
@@ -127,7 +124,8 @@ struct Work {
   const int32_t* sblk_job;   // job owning each 4096-entry block of the cell tables (-1: none)
   JobRed* red;
   JobAcc* acc;      // kAccCopies per job, [job * kAccCopies + copy]
-  const LsCell* ls;
+  const double* lsp;   // level-set phi per cell
+  const float4* lsn;   // level-set (normal, tag bits) per cell
   float* nu_a;   // nu read by the stage-2 pass (previous step's value)
   float* nu_b;   // nu written by the stage-2 pass
   float* a1[3];
@@ -135,7 +133,9 @@ struct Work {
   float* sum;
   uint32_t* key;
   uint32_t* skey;
-  int32_t* perm;
+  int32_t* perm;     // sort: source slot of each sorted position (cell order)
+  int32_t* sperm;    // the same with each cell in particle-id order
+  int32_t* sid;      // particle id at each sorted position (k_scatter)
   int32_t* counts;
   int32_t* starts;
   int32_t* bsum;

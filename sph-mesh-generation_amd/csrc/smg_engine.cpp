@@ -175,7 +175,8 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   d_sblk_job_ = alloc_<int32_t>((tbl + kScanItems - 1) / kScanItems + 1);
   d_red_ = alloc_<JobRed>(njobs);
   d_acc_ = alloc_<JobAcc>(int64_t(njobs) * kAccCopies);
-  d_ls_ = alloc_<LsCell>(n_ls_);
+  d_lsp_ = alloc_<double>(n_ls_);
+  d_lsn_ = alloc_<float4>(n_ls_);
   PState* S2[2] = {&A_, &B_};
   for (PState* S : S2) {
     for (int a = 0; a < 3; a++) {
@@ -198,6 +199,8 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   W_.key = alloc_<uint32_t>(cap_total_);
   W_.skey = alloc_<uint32_t>(cap_total_);
   W_.perm = alloc_<int32_t>(cap_total_);
+  W_.sperm = alloc_<int32_t>(cap_total_);
+  W_.sid = alloc_<int32_t>(cap_total_);
   W_.counts = alloc_<int32_t>(n_tbl_);
   W_.starts = alloc_<int32_t>(n_tbl_ + 1);
   const int64_t nb = (n_tbl_ + kScanItems - 1) / kScanItems;
@@ -227,7 +230,8 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   W_.sblk_job = d_sblk_job_;
   W_.red = d_red_;
   W_.acc = d_acc_;
-  W_.ls = d_ls_;
+  W_.lsp = d_lsp_;
+  W_.lsn = d_lsn_;
   W_.cap_total = cap_total_;
   W_.n_tbl = n_tbl_;
   W_.K = K_;
@@ -296,19 +300,21 @@ int Engine::upload_job_(int j, const JobInput& in) {
   jp_[j].spill_cap = jp_[j].cap * sps_;
   const JobParams& J = jp_[j];
   const int64_t m = job_lset_cells(J);
-  std::vector<LsCell> hls(m, LsCell{0., {0.f, 0.f, 0.f}, 0});   // the padding stays zero
+  std::vector<double> hp(m, 0.);   // the padding stays zero
+  std::vector<float4> hn(m, make_float4(0.f, 0.f, 0.f, 0.f));
   int64_t q = 0;   // the caller's grid: x fastest
   for (int c = 0; c < J.ng[2]; c++)
     for (int b = 0; b < J.ng[1]; b++)
       for (int a = 0; a < J.ng[0]; a++, q++) {
-        LsCell& cc = hls[ls_cell_offset(J.D, J.ng, a, b, c)];
-        cc.phi = in.phi[q];
-        cc.n[0] = in.normal[3 * q + 0];
-        cc.n[1] = in.normal[3 * q + 1];
-        cc.n[2] = in.normal[3 * q + 2];
-        cc.tag = in.tag[q];
+        const int64_t o = ls_cell_offset(J.D, J.ng, a, b, c);
+        hp[o] = in.phi[q];
+        int32_t tag = in.tag[q];
+        float tf;
+        std::memcpy(&tf, &tag, sizeof(tf));
+        hn[o] = make_float4(in.normal[3 * q + 0], in.normal[3 * q + 1], in.normal[3 * q + 2], tf);
       }
-  HIP_TRY(hipMemcpyAsync(d_ls_ + J.lset_off, hls.data(), sizeof(LsCell) * m, hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipMemcpyAsync(d_lsp_ + J.lset_off, hp.data(), sizeof(double) * m, hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipMemcpyAsync(d_lsn_ + J.lset_off, hn.data(), sizeof(float4) * m, hipMemcpyHostToDevice, st_));
   HIP_TRY(hipMemcpyAsync(d_jp_ + j, &J, sizeof(JobParams), hipMemcpyHostToDevice, st_));
   const int64_t t0 = J.p0 / kTile, nt = J.cap / kTile;
   for (int64_t t = t0; t < t0 + nt; t++) tile_job_h_[t] = int32_t(j);
@@ -501,7 +507,7 @@ int Engine::sort_job_(int j) {
   Wt.starts = W_.starts + J.tbl_base;
   Wt.n_tbl = J.tbl_size;
   launch_scan(st_, Wt, false, false);
-  launch_scatter(st_, nt, Ws);
+  launch_scatter(st_, nt, Ws, A_);
   launch_segsort(st_, nt, Ws, A_);
   launch_gather(st_, D_, nt, Ws, A_, B_);
   launch_copyback(st_, D_, nt, Ws, A_, B_);
@@ -569,7 +575,7 @@ int Engine::sort_particles() {
   launch_decide(st_, W_, int(jp_.size()), 1);
   launch_key(st_, D_, ntiles_, W_, A_);
   launch_scan(st_, W_, false, true);
-  launch_scatter(st_, ntiles_, W_);
+  launch_scatter(st_, ntiles_, W_, A_);
   launch_segsort(st_, ntiles_, W_, A_);
   launch_gather(st_, D_, ntiles_, W_, A_, B_);
   launch_copyback(st_, D_, ntiles_, W_, A_, B_);
@@ -682,7 +688,7 @@ int Engine::step(int64_t nsteps) {
     launch_scan(st_, W_, true, true);
     end_();
     begin_("k_scatter");
-    launch_scatter(st_, ntiles_, W_);
+    launch_scatter(st_, ntiles_, W_, A_);
     end_();
     begin_("k_segsort");
     launch_segsort(st_, ntiles_, W_, A_);

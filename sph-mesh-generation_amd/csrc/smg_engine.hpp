@@ -146,7 +146,8 @@ class Engine {
   std::vector<int32_t> sblk_job_h_;
   JobRed* d_red_ = nullptr;
   JobAcc* d_acc_ = nullptr;
-  LsCell* d_ls_ = nullptr;
+  double* d_lsp_ = nullptr;
+  float4* d_lsn_ = nullptr;
   PState A_{}, B_{};
   Work W_{};
   std::vector<void*> allocs_;

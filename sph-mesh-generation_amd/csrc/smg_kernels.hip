@@ -23,6 +23,7 @@
 // FMA changes an FP64 rounding.
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 #include "smg_device.hpp"
 #include "smg_kernels.hpp"
@@ -150,8 +151,8 @@ __device__ __forceinline__ void ls_locate(const JobParams& J, const double* x, d
 
 // Calculate_particle_infor (particle_mesh_generation.cpp:175-265): bi/trilinear phi
 // and raw normal; the normal is left unnormalised (quirk Q3)
-template <int D>
-__device__ void ls_sample(const JobParams& J, const LsCell* __restrict__ ls, const double* x, double& phi, double* n) {
+template <int D, bool kNormal>
+__device__ void ls_sample(const JobParams& J, const Work& W, const double* x, double& phi, double* n) {
   double cs[3];
   int pk[3];
   ls_locate<D>(J, x, cs, pk);
@@ -166,27 +167,30 @@ __device__ void ls_sample(const JobParams& J, const LsCell* __restrict__ ls, con
     f[a] = rr - double(m[a]);
   }
   phi = 0.;
-  n[0] = n[1] = n[2] = 0.;
+  if (kNormal) n[0] = n[1] = n[2] = 0.;
 #pragma unroll
   for (int r = 0; r < 2; r++)
 #pragma unroll
     for (int s = 0; s < (D > 1 ? 2 : 1); s++)
 #pragma unroll
       for (int t = 0; t < (D > 2 ? 2 : 1); t++) {
-        const LsCell c = ls[ls_index(J, D, pk[0] * 4 + m[0] + r, pk[1] * 4 + m[1] + s, pk[2] * 4 + m[2] + t)];
+        const int64_t c = ls_index(J, D, pk[0] * 4 + m[0] + r, pk[1] * 4 + m[1] + s, pk[2] * 4 + m[2] + t);
         double coef = fabs(f[0] - (1.0 - r));
         if (D > 1) coef = coef * fabs(f[1] - (1.0 - s));
         if (D > 2) coef = coef * fabs(f[2] - (1.0 - t));
-        phi += c.phi * coef;
-        n[0] += double(c.n[0]) * coef;
-        n[1] += double(c.n[1]) * coef;
-        n[2] += double(c.n[2]) * coef;
+        phi += W.lsp[c] * coef;
+        if (kNormal) {
+          const float4 q = W.lsn[c];
+          n[0] += double(q.x) * coef;
+          n[1] += double(q.y) * coef;
+          n[2] += double(q.z) * coef;
+        }
       }
 }
 
 // Levelset_levelinfo::Get_id_pkg_cell (lset_level_infor.cpp:350-366) -> tag_interface
 template <int D>
-__device__ int ls_tag(const JobParams& J, const LsCell* __restrict__ ls, const double* x) {
+__device__ int ls_tag(const JobParams& J, const Work& W, const double* x) {
   double cs[3];
   int pk[3];
   ls_locate<D>(J, x, cs, pk);
@@ -198,7 +202,7 @@ __device__ int ls_tag(const JobParams& J, const LsCell* __restrict__ ls, const d
     c = (0 >= c) ? 0 : c;
     g[a] = pk[a] * 4 + c;
   }
-  return ls[ls_index(J, D, g[0], g[1], g[2])].tag;
+  return __float_as_int(W.lsn[ls_index(J, D, g[0], g[1], g[2])].w);
 }
 
 // Levelset::Get_scale (level_set.cpp:311-351) and the linear-profile extension
@@ -420,7 +424,7 @@ __global__ __launch_bounds__(kTile) void k_seed(Work W, PState S) {
   int bad = 0;
   if (loc < J.n) {
     const uint64_t key = rng_key(J.seed, 0xFFFFFFFFull, uint64_t(loc));
-    double x[3] = {0., 0., 0.}, phi = 0., n[3], h = 0.;
+    double x[3] = {0., 0., 0.}, phi = 0., n[3] = {0., 0., 0.}, h = 0.;
     const double pad = J.dpkg[0];
     bool ok = false;
     for (int att = 0; att < J.remap_max && !ok; att++) {
@@ -438,7 +442,7 @@ __global__ __launch_bounds__(kTile) void k_seed(Work W, PState S) {
         v = (llo >= v) ? llo : v;
         x[a] = v;
       }
-      ls_sample<D>(J, W.ls, x, phi, n);
+      ls_sample<D, false>(J, W, x, phi, n);
       h = h_of_phi(J, phi);
       ok = phi > h;
     }
@@ -515,15 +519,18 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
         for (int a = 0; a < D; a++) x[a] = x[a] + v[a] * dt2;
       }
     }
-    double phi, n[3];
-    ls_sample<D>(J, W.ls, x, phi, n);
+    // phi only: the normal matters to a particle in a cut cell (projection) and to a
+    // surface particle, and is sampled there; an interior particle's stays 0, unread
+    double phi, n[3] = {0., 0., 0.};
+    ls_sample<D, false>(J, W, x, phi, n);
     if (mode == kModeInit) {
       h = h_of_phi(J, phi);
     } else {
       if (mode == kModePrerun) h = h_of_phi(J, phi);
       // Find_and_map_particle (particle_mesh_generation.cpp:408-467)
-      const int tag = ls_tag<D>(J, W.ls, x);
+      const int tag = ls_tag<D>(J, W, x);
       if (tag == 1 && (type == SMG_REAL || type == SMG_SURFACE)) {
+        ls_sample<D, true>(J, W, x, phi, n);   // the same phi, and the normal
         type = SMG_SURFACE;
         const double dist = fabs(phi);
         double nn = 0.;
@@ -551,7 +558,7 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
         }
       }
       // sample again where the particle moved (a REAL particle keeps x, phi and n)
-      if (type == SMG_SURFACE) ls_sample<D>(J, W.ls, x, phi, n);
+      if (type == SMG_SURFACE) ls_sample<D, true>(J, W, x, phi, n);
       if (phi < 0. && type == SMG_REAL) {
         // Remap_this_particle_to_positive_phase (sph_mesh_generation.cpp:1885-1928)
         nre = 1;
@@ -570,7 +577,7 @@ __global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int p
             v = (lo >= v) ? lo : v;
             x[a] = v;
           }
-          ls_sample<D>(J, W.ls, x, phi, n);
+          ls_sample<D, false>(J, W, x, phi, n);
           ok = phi > h;
         }
         if (ok) {
@@ -807,7 +814,7 @@ __global__ __launch_bounds__(kTile) void k_scan_down(const int32_t* __restrict__
 
 // A job's cell starts are taken relative to the start of its table segment, so the
 // slot ranges of the jobs may sit anywhere (gaps included) and still sort in place.
-__global__ __launch_bounds__(kTile) void k_scatter(Work W) {
+__global__ __launch_bounds__(kTile) void k_scatter(Work W, PState S) {
   const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
@@ -817,30 +824,27 @@ __global__ __launch_bounds__(kTile) void k_scatter(Work W) {
   const int64_t d = J.p0 + int64_t(W.starts[k] - W.starts[J.tbl_base]) + r;
   W.perm[d] = int32_t(p);
   W.skey[d] = k;
+  W.sid[d] = S.id[p];
 }
-
-__global__ __launch_bounds__(kTile) void k_segsort(Work W, PState S) {
+// Order inside each cell by particle id (deterministic whatever order the scatter's
+// atomics took), a rank sort: slot d of cell [c0, c1) goes to c0 + #{a : sid[a] <
+// sid[d]}; the ids of a job are distinct.  Dead slots (the pad cell) keep their place.
+__global__ __launch_bounds__(kTile) void k_segsort(Work W) {
   const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
   const int64_t d = tile_id(W) * kTile + threadIdx.x;
   const uint32_t k = W.skey[d];
-  if (k == uint32_t(J.tbl_base + J.tbl_size - 1)) return;   // dead slots: no order needed
-  if (d > J.p0 && W.skey[d - 1] == k) return;                 // not the head of its cell
-  int64_t e = d + 1;
-  const int64_t end = J.p0 + J.cap;
-  while (e < end && W.skey[e] == k) e++;
-  // insertion sort of perm[d, e) by particle id
-  for (int64_t a = d + 1; a < e; a++) {
-    const int32_t pa = W.perm[a];
-    const int32_t ia = S.id[pa];
-    int64_t b = a - 1;
-    while (b >= d && S.id[W.perm[b]] > ia) {
-      W.perm[b + 1] = W.perm[b];
-      b--;
-    }
-    W.perm[b + 1] = pa;
+  if (k == uint32_t(J.tbl_base + J.tbl_size - 1)) {
+    W.sperm[d] = W.perm[d];
+    return;
   }
+  const int64_t base = J.p0 - int64_t(W.starts[J.tbl_base]);
+  const int64_t c0 = base + W.starts[k], c1 = base + W.starts[k + 1];
+  const int32_t me = W.sid[d];
+  int rank = 0;
+  for (int64_t a = c0; a < c1; a++) rank += W.sid[a] < me ? 1 : 0;
+  W.sperm[c0 + rank] = W.perm[d];
 }
 
 // new order into B (the scratch set); k_copyback then moves it into A, so A is the
@@ -850,7 +854,7 @@ __global__ __launch_bounds__(kTile) void k_gather(Work W, PState A, PState B) {
   const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
   const int64_t d = tile_id(W) * kTile + threadIdx.x;
-  const int64_t p = W.perm[d];
+  const int64_t p = W.sperm[d];
 #pragma unroll
   for (int a = 0; a < D; a++) {
     B.xh[a][d] = A.xh[a][p];
@@ -932,53 +936,59 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
     }
   }
   const float cutf = float(J.cut_v);
+  // one image: every level and stencil cell; the plain image (code 0, the first for
+  // every lane) tests xh_i - xh_q directly, a mirror image A + sg xh_q
+  auto image = [&](uint32_t code, auto plain) {
+    double y[3] = {0., 0., 0.};
+    image_of<D>(J, code, xi, y);
+    // FP32 relative position: dr_a = A_a + s_a * xh_q
+    float A[3] = {0.f, 0.f, 0.f}, sg[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      const uint32_t m = (code >> (2 * a)) & 3u;
+      A[a] = m == 0 ? xh[a] : (m == 1 ? xh[a] - 2.f * J.wl_h[a] : xh[a] - 2.f * J.wh_h[a]);
+      sg[a] = m == 0 ? -1.f : 1.f;
+    }
+    for (int L = 0; L < J.nlev; L++) {
+      const int s = L < li ? L : li;
+      const int shift = D * (L - s);
+      int cy[3] = {0, 0, 0};
+      cell_at<D>(J, s, y, cy);
+      const int64_t base = J.tbl_base + J.lev_off[L];
+      for (int dz = (D > 2 ? -1 : 0); dz <= (D > 2 ? 1 : 0); dz++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dx = -1; dx <= 1; dx++) {
+            int c[3] = {cy[0] + dx, cy[1] + dy, cy[2] + dz};
+            if (c[0] < 0 || c[0] >= J.lev_n[s][0] || c[1] < 0 || c[1] >= J.lev_n[s][1]) continue;
+            if (D > 2 && (c[2] < 0 || c[2] >= J.lev_n[s][2])) continue;
+            const uint64_t m = morton<D>(c);
+            const int32_t lo = W.starts[base + int64_t(m << shift)] + soff;
+            const int32_t hi2 = W.starts[base + int64_t((m + 1) << shift)] + soff;
+            for (int32_t q = lo; q < hi2; q++) {
+              const float4 qr = W.recb[q];   // == rec[0] at a build, 16-byte stride
+              const float qv[3] = {qr.x, qr.y, qr.z};
+              float d2 = 0.f;
+#pragma unroll
+              for (int a = 0; a < D; a++) {
+                const float t = decltype(plain)::value ? xh[a] - qv[a] : A[a] + sg[a] * qv[a];
+                d2 += t * t;
+              }
+              const float r = fmaxf(rfi, qr.w * cutf);
+              const float rhi = r + (J.pf_tol + 1.e-6f * r);
+              if (d2 <= rhi * rhi) {
+                emit(cnt, int32_t(uint32_t(q) | (code << kJBits)));
+                cnt++;
+              }
+            }
+          }
+    }
+  };
+  image(0u, std::true_type{});
   for (int o0 = 0; o0 < nopt[0]; o0++)
     for (int o1 = 0; o1 < nopt[1]; o1++)
       for (int o2 = 0; o2 < nopt[2]; o2++) {
-        const uint32_t code = uint32_t(opt[0][o0] | (opt[1][o1] << 2) | (opt[2][o2] << 4));
-        double y[3] = {0., 0., 0.};
-        image_of<D>(J, code, xi, y);
-        // FP32 relative position: dr_a = A_a + s_a * xh_q
-        float A[3] = {0.f, 0.f, 0.f}, sg[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-        for (int a = 0; a < D; a++) {
-          const uint32_t m = (code >> (2 * a)) & 3u;
-          A[a] = m == 0 ? xh[a] : (m == 1 ? xh[a] - 2.f * J.wl_h[a] : xh[a] - 2.f * J.wh_h[a]);
-          sg[a] = m == 0 ? -1.f : 1.f;
-        }
-        for (int L = 0; L < J.nlev; L++) {
-          const int s = L < li ? L : li;
-          const int shift = D * (L - s);
-          int cy[3] = {0, 0, 0};
-          cell_at<D>(J, s, y, cy);
-          const int64_t base = J.tbl_base + J.lev_off[L];
-          for (int dz = (D > 2 ? -1 : 0); dz <= (D > 2 ? 1 : 0); dz++)
-            for (int dy = -1; dy <= 1; dy++)
-              for (int dx = -1; dx <= 1; dx++) {
-                int c[3] = {cy[0] + dx, cy[1] + dy, cy[2] + dz};
-                if (c[0] < 0 || c[0] >= J.lev_n[s][0] || c[1] < 0 || c[1] >= J.lev_n[s][1]) continue;
-                if (D > 2 && (c[2] < 0 || c[2] >= J.lev_n[s][2])) continue;
-                const uint64_t m = morton<D>(c);
-                const int32_t lo = W.starts[base + int64_t(m << shift)] + soff;
-                const int32_t hi2 = W.starts[base + int64_t((m + 1) << shift)] + soff;
-                for (int32_t q = lo; q < hi2; q++) {
-                  const float4 qr = W.recb[q];   // == rec[0] at a build, 16-byte stride
-                  const float qv[3] = {qr.x, qr.y, qr.z};
-                  float d2 = 0.f;
-#pragma unroll
-                  for (int a = 0; a < D; a++) {
-                    const float t = A[a] + sg[a] * qv[a];
-                    d2 += t * t;
-                  }
-                  const float r = fmaxf(rfi, qr.w * cutf);
-                  const float rhi = r + (J.pf_tol + 1.e-6f * r);
-                  if (d2 <= rhi * rhi) {
-                    emit(cnt, int32_t(uint32_t(q) | (code << kJBits)));
-                    cnt++;
-                  }
-                }
-              }
-        }
+        if (o0 + o1 + o2 == 0) continue;   // the plain image, done above
+        image(uint32_t(opt[0][o0] | (opt[1][o1] << 2) | (opt[2][o2] << 4)), std::false_type{});
       }
   return cnt;
 }
@@ -1419,6 +1429,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
     // kC<D> ih^(D+1) with one factor ih dropped.
     const float Ck = surf ? Cs : Cw;
     const float kd = surf ? kC<D - 1>() : kC<D>();
+    const bool wave_surf = __ballot(surf) != 0ull;   // wave-uniform: skip the projection
     for_each_neighbor<D>(W, S, J, slot, jt, [&](int k, int64_t q, uint32_t code, const float4& qh, const float4& ql,
                                                  const float* dr0, float d2) {
       ntr++;
@@ -1442,15 +1453,18 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
         for (int a = 0; a < D; a++) ca += ni[a] * nq[a];
         use = fabsf(ca) > J.cos_thres;
       }
-      float dn = 0.f;
+      float dr[3] = {dr0[0], dr0[1], dr0[2]};
+      float rt = d2;
+      if (wave_surf) {
+        float dn = 0.f;
 #pragma unroll
-      for (int a = 0; a < D; a++) dn += dr0[a] * ni[a];
-      float dr[3] = {0.f, 0.f, 0.f};
-      float rt = 0.f;
+        for (int a = 0; a < D; a++) dn += dr0[a] * ni[a];
+        rt = 0.f;
 #pragma unroll
-      for (int a = 0; a < D; a++) {
-        dr[a] = dr0[a] - ni[a] * dn;
-        rt += dr[a] * dr[a];
+        for (int a = 0; a < D; a++) {
+          dr[a] = dr0[a] - ni[a] * dn;
+          rt += dr[a] * dr[a];
+        }
       }
       rt = fsqrt(rt);
       const bool counts = use && (!surf || tq == SMG_SURFACE);   // the kernel summation
@@ -1561,6 +1575,7 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
     // one straight-line body for both particle types, as in the stage-1 pass: with
     // ni = 0 a REAL particle's projected rt^2 is its rr bit for bit
     const float kd = surf ? kC<D - 1>() : kC<D>();
+    const bool wave_surf = __ballot(surf) != 0ull;   // wave-uniform: skip the projection
     auto pair_rec = [&](int64_t q, uint32_t code, bool checked, const float4& qh, const float4& ql,
                         const float4& qv) {
       const int tq = __float_as_int(ql.w) & ~kJumpBit;
@@ -1586,14 +1601,17 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
         for (int a = 0; a < D; a++) ca += ni[a] * nq[a];
         use = use && fabsf(ca) > J.cos_thres;
       }
-      float dn = 0.f;
+      float rt = rr;
+      if (wave_surf) {
+        float dn = 0.f;
 #pragma unroll
-      for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
-      float rt = 0.f;
+        for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
+        rt = 0.f;
 #pragma unroll
-      for (int a = 0; a < D; a++) {
-        const float t = dr[a] - ni[a] * dn;
-        rt += t * t;
+        for (int a = 0; a < D; a++) {
+          const float t = dr[a] - ni[a] * dn;
+          rt += t * t;
+        }
       }
       rt = fsqrt(rt);
       const float ihm = frcp(0.5f * (h + hq));
@@ -1824,7 +1842,7 @@ __global__ void k_sample_phi(Work W, int job, int64_t n, const double* pts, doub
   const JobParams& J = W.jp[job];
   const double x[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
   double p, nn[3];
-  ls_sample<D>(J, W.ls, x, p, nn);
+  ls_sample<D, false>(J, W, x, p, nn);
   phi[i] = p;
 }
 
@@ -1935,11 +1953,12 @@ void launch_scan(hipStream_t st, const Work& W, bool gated, bool per_job) {
   k_scan_top<<<1, 1024, 0, st>>>(W.bsum, nb, gate);
   k_scan_down<<<nb, kTile, 0, st>>>(W.counts, W.n_tbl, W.bsum, nb, W.starts, gate, bj, W.red);
 }
-void launch_scatter(hipStream_t st, int64_t ntiles, const Work& W) {
-  k_scatter<<<ntiles, kTile, 0, st>>>(W);
+void launch_scatter(hipStream_t st, int64_t ntiles, const Work& W, const PState& S) {
+  k_scatter<<<ntiles, kTile, 0, st>>>(W, S);
 }
 void launch_segsort(hipStream_t st, int64_t ntiles, const Work& W, const PState& S) {
-  k_segsort<<<ntiles, kTile, 0, st>>>(W, S);
+  (void)S;
+  k_segsort<<<ntiles, kTile, 0, st>>>(W);
 }
 void launch_gather(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& A, const PState& B) {
   if (D == 2) k_gather<2><<<ntiles, kTile, 0, st>>>(W, A, B);

@@ -20,7 +20,7 @@ void launch_nb_out(hipStream_t st, int D, int64_t ntiles, const Work& W, const P
 void launch_seed(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S);
 void launch_map(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int mode, int par);
 void launch_scan(hipStream_t st, const Work& W, bool gated, bool per_job);
-void launch_scatter(hipStream_t st, int64_t ntiles, const Work& W);
+void launch_scatter(hipStream_t st, int64_t ntiles, const Work& W, const PState& S);
 void launch_segsort(hipStream_t st, int64_t ntiles, const Work& W, const PState& S);
 void launch_gather(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& A, const PState& B);
 void launch_nb(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S);

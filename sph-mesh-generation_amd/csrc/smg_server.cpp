@@ -40,7 +40,7 @@ Sizes sizes_for(int K, int sps) {
   const int64_t state = 9 * 4 + 8 + 4 + 4;   // xh, xl, nrm (3 each), h, type, id
   z.per_slot = 2 * state + 2 * 4 + 6 * 4 + 4 + 4 * 16 + 3 * 4 + int64_t(K) * 4 + 4 * 4 + 16 + int64_t(sps) * 4 + 1;
   z.per_table = 2 * 4 + 1;
-  z.per_cell = int64_t(sizeof(smg::LsCell));
+  z.per_cell = smg::kLsCellBytes;
   z.per_job = int64_t(sizeof(smg::JobParams) + sizeof(smg::JobRed) + smg::kAccCopies * sizeof(smg::JobAcc));
   return z;
 }
