@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define SMG_ABI_VERSION 3
+#define SMG_ABI_VERSION 4
 
 enum {
   SMG_OK = 0,
@@ -134,6 +134,31 @@ int smg_destroy(smg_job* job);
 /* Level set on the one-cell-halo grid of smg_lset_build (copied to the device);
  * replaces Load_rst_case / Levelset::Load_lset_restart_file (:323-422). */
 int smg_set_levelset(smg_job* job, const double* phi, const float* normal, const int8_t* tag);
+
+/* Multi-resolution extension; no counterpart in the reference, whose level set is
+ * single-level (level_set.cpp:53; Lmin = Lmax = 0, sph_mesh_generation.cpp:308-310).
+ * Refined levels over the base level set of smg_set_levelset: level l = 1..nlev has
+ * num_pkg << l packages per axis.  Its cells are given on its own one-cell-halo grid,
+ * as smg_lset_build produces them for a config with num_pkg << l, and active (one byte
+ * per package of the level, x fastest, nonzero = kept) selects the packages stored on
+ * the device, typically a band near the surface.  A sample (Calculate_particle_infor)
+ * uses the finest level that stores all 2^D corner cells, else the next coarser, down
+ * to the base grid; the cut-cell tag the finest level that stores the cell.  With
+ * every package active the result equals a single-level run on the finest grid.  The
+ * spacing profile's level-set cell and maximum_phi come from the finest level.  At
+ * most 3 levels; call before seeding; not available in a server pool. */
+typedef struct smg_lset_level {
+  const double* phi;
+  const float* normal;
+  const int8_t* tag;
+  const uint8_t* active;
+} smg_lset_level;
+int smg_set_levelset_levels(smg_job* job, int32_t nlev, const smg_lset_level* levels);
+
+/* The job's level-set phi at n points (n*3 doubles) as the relaxation samples it
+ * (Calculate_particle_infor's interpolation, particle_mesh_generation.cpp:175-265, on
+ * the finest stored level); the job must be seeded or initialised. */
+int smg_sample_phi(smg_job* job, int64_t n, const double* pts, double* phi);
 
 /* Particles at given positions (n*3 doubles, unused axes ignored); type may be NULL
  * (all REAL).  h is set from the level set (Set_particle_info). */

@@ -29,16 +29,39 @@ struct smg_job {
   std::vector<int8_t> tag;
   double max_phi = 0.;
   bool has_lset = false;
+  struct Level {
+    std::vector<double> phi;
+    std::vector<float> normal;
+    std::vector<int8_t> tag;
+    std::vector<uint8_t> active;
+  };
+  std::vector<Level> levels;   // refined levels 1, 2, ... (smg_set_levelset_levels)
   std::unique_ptr<smg::Engine> eng;
   bool profile = false;
 };
 
 namespace {
 
-int64_t halo_cells(const smg_config& c) {
+int64_t halo_cells(const smg_config& c, int shift = 0) {
   int64_t m = 1;
-  for (int a = 0; a < c.dim; a++) m *= 4 * int64_t(c.num_pkg[a]) + 2;
+  for (int a = 0; a < c.dim; a++) m *= 4 * (int64_t(c.num_pkg[a]) << shift) + 2;
   return m;
+}
+
+// maximum_phi over the interior cells of a one-cell-halo grid with num_pkg << shift
+// packages, as Parallel_get_global_scale (parallelfunc_mesh.cpp:46-54)
+double interior_max_phi(const smg_config& c, const double* phi, int shift) {
+  const int nx = 4 * (c.num_pkg[0] << shift), ny = c.dim > 1 ? 4 * (c.num_pkg[1] << shift) : 1;
+  const int nz = c.dim > 2 ? 4 * (c.num_pkg[2] << shift) : 1;
+  const int gx = nx + 2, gy = c.dim > 1 ? ny + 2 : 1;
+  double mp = 0.;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        const int64_t o = ((int64_t(c.dim > 2 ? k + 1 : 0) * gy) + (c.dim > 1 ? j + 1 : 0)) * gx + i + 1;
+        if (phi[o] > 0.) mp = std::max(mp, std::fabs(phi[o]));
+      }
+  return mp;
 }
 
 int check_config(const smg_config* c) {
@@ -71,6 +94,14 @@ int build_engine(smg_job* job, int64_t n) {
   in.phi = job->phi.data();
   in.normal = job->normal.data();
   in.tag = job->tag.data();
+  for (auto& L : job->levels) {
+    smg::LevelInput li;
+    li.phi = L.phi.data();
+    li.normal = L.normal.data();
+    li.tag = L.tag.data();
+    li.active = L.active.data();
+    in.levels.push_back(li);
+  }
   job->eng.reset(new smg::Engine());
   job->eng->profile_enable(job->profile);
   const int rc = job->eng->setup({in}, job->cfg.max_neighbors, job->cfg.spill_per_slot, nullptr);
@@ -146,19 +177,37 @@ int smg_set_levelset(smg_job* job, const double* phi, const float* normal, const
   job->phi.assign(phi, phi + m);
   job->normal.assign(normal, normal + 3 * m);
   job->tag.assign(tag, tag + m);
-  // maximum_phi over interior cells, as Parallel_get_global_scale (parallelfunc_mesh.cpp:46-54)
-  const smg_config& c = job->cfg;
-  const int nx = 4 * c.num_pkg[0], ny = c.dim > 1 ? 4 * c.num_pkg[1] : 1, nz = c.dim > 2 ? 4 * c.num_pkg[2] : 1;
-  const int gx = nx + 2, gy = c.dim > 1 ? ny + 2 : 1;
-  double mp = 0.;
-  for (int k = 0; k < nz; k++)
-    for (int j = 0; j < ny; j++)
-      for (int i = 0; i < nx; i++) {
-        const int64_t o = ((int64_t(c.dim > 2 ? k + 1 : 0) * gy) + (c.dim > 1 ? j + 1 : 0)) * gx + i + 1;
-        if (job->phi[o] > 0.) mp = std::max(mp, std::fabs(job->phi[o]));
-      }
-  job->max_phi = mp;
+  job->levels.clear();
+  job->max_phi = interior_max_phi(job->cfg, job->phi.data(), 0);
   job->has_lset = true;
+  return SMG_OK;
+}
+
+int smg_set_levelset_levels(smg_job* job, int32_t nlev, const smg_lset_level* levels) {
+  if (!job || nlev < 0 || nlev > smg::kMaxLsLevels || (nlev > 0 && !levels)) {
+    set_error("smg_set_levelset_levels: bad argument (at most 3 levels)");
+    return SMG_ERR_ARG;
+  }
+  if (!job->has_lset) { set_error("smg_set_levelset_levels: set the base level set first"); return SMG_ERR_STATE; }
+  if (job->eng) { set_error("smg_set_levelset_levels: call before seeding"); return SMG_ERR_STATE; }
+  std::vector<smg_job::Level> lv(nlev);
+  for (int l = 0; l < nlev; l++) {
+    const smg_lset_level& L = levels[l];
+    if (!L.phi || !L.normal || !L.tag || !L.active) {
+      set_error("smg_set_levelset_levels: NULL array in level " + std::to_string(l + 1));
+      return SMG_ERR_ARG;
+    }
+    const int64_t m = halo_cells(job->cfg, l + 1);
+    int64_t np = 1;
+    for (int a = 0; a < job->cfg.dim; a++) np *= int64_t(job->cfg.num_pkg[a]) << (l + 1);
+    lv[l].phi.assign(L.phi, L.phi + m);
+    lv[l].normal.assign(L.normal, L.normal + 3 * m);
+    lv[l].tag.assign(L.tag, L.tag + m);
+    lv[l].active.assign(L.active, L.active + np);
+  }
+  job->levels = std::move(lv);
+  job->max_phi = nlev > 0 ? interior_max_phi(job->cfg, job->levels.back().phi.data(), nlev)
+                          : interior_max_phi(job->cfg, job->phi.data(), 0);
   return SMG_OK;
 }
 
@@ -219,6 +268,12 @@ int smg_get_neighbors(smg_job* job, int64_t* offs, int32_t* ids, int32_t* codes,
   if (!job || !job->eng || !offs) { set_error("smg_get_neighbors: bad argument"); return SMG_ERR_STATE; }
   if (ids && !codes) { set_error("smg_get_neighbors: codes is NULL"); return SMG_ERR_ARG; }
   return job->eng->get_neighbors(0, offs, ids, codes, capacity);
+}
+
+int smg_sample_phi(smg_job* job, int64_t n, const double* pts, double* phi) {
+  if (!job || n < 0 || (n > 0 && (!pts || !phi))) { set_error("smg_sample_phi: bad argument"); return SMG_ERR_ARG; }
+  if (!job->eng) { set_error("smg_sample_phi: seed or initialise the job first"); return SMG_ERR_STATE; }
+  return job->eng->sample_phi(0, n, pts, phi);
 }
 
 int smg_profile_enable(smg_job* job, int enable) {

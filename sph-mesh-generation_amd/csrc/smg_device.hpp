@@ -25,6 +25,14 @@ constexpr int kJumpBit = 1 << 30;   // flag in the type bits of a record: the sl
 // interior particles' samples need phi only and read 8 bytes per corner.
 constexpr int64_t kLsCellBytes = 8 + 16;
 
+// Refined level-set levels above a job's base grid (a multi-resolution extension: the
+// reference's level set is single-level).  Level l has num_pkg << l packages per
+// axis; only the packages of a band near the surface are stored, as bricks of 4^D
+// cells in package order (cell c of package p is global cell 4 p + c), reached
+// through a package map of (packages + 2)^D entries (one halo package on each side):
+// brick index, or -1 where the level holds no data.
+constexpr int kMaxLsLevels = 3;
+
 // Level-set cells are stored in bricks of 4 x 4 (x 4 in 3D) cells, so that the 2^D
 // corners of a sample, and the samples of nearby particles, share few cache lines.
 // Cell (a, b, c) of a job's one-cell-halo grid of ng[0] x ng[1] x ng[2] cells sits at
@@ -59,6 +67,14 @@ struct JobParams {
   int64_t spill_base, spill_cap;              // this job's part of the spill pool
   uint64_t seed;
   int64_t lset_off;                           // first cell of this job's halo grid
+  int32_t ls_nl;                              // refined level-set levels (0: single-level)
+  int32_t lnp[kMaxLsLevels][3];               // packages per axis of refined level l + 1
+  double ldpkg[kMaxLsLevels][3], ldcell[kMaxLsLevels][3];
+  int64_t lmap_off[kMaxLsLevels];             // the level's package map in Work::lpm
+  int64_t lcell_off[kMaxLsLevels];            // first cell of the level's bricks in lsp / lsn
+  int64_t lbricks[kMaxLsLevels];              // bricks stored for the level
+  int64_t lmap_base;                          // this job's package maps in Work::lpm
+  double seed_pad;                            // seeding margin: one package of the finest level
   int64_t tbl_base, tbl_size;                 // cell-table segment (last entry = pad cell)
   int64_t lev_off[kMaxLevels];
   int64_t p0, n, cap;                         // slot range
@@ -126,6 +142,7 @@ struct Work {
   JobAcc* acc;      // kAccCopies per job, [job * kAccCopies + copy]
   const double* lsp;   // level-set phi per cell
   const float4* lsn;   // level-set (normal, tag bits) per cell
+  const int32_t* lpm;  // refined level-set package maps (brick index or -1)
   float* nu_a;   // nu read by the stage-2 pass (previous step's value)
   float* nu_b;   // nu written by the stage-2 pass
   float* a1[3];

@@ -65,6 +65,14 @@ static int ceil_log2(int64_t v) {
 }
 
 // One job's parameters: everything except its slot, table and level-set ranges.
+// package (p0, p1, p2) of a refined level's map (index 0 is the halo package before
+// the first): stored where the interior package it is, or borders, is active
+static bool level_pkg_active(const JobParams& J, int l, const LevelInput& L, int p0, int p1, int p2) {
+  int i[3] = {p0 - 1, p1 - 1, p2 - 1};
+  for (int a = 0; a < 3; a++) i[a] = a < J.D ? std::min(std::max(i[a], 0), J.lnp[l][a] - 1) : 0;
+  return L.active[(int64_t(i[2]) * J.lnp[l][1] + i[1]) * J.lnp[l][0] + i[0]] != 0;
+}
+
 int make_job_params(const JobInput& in, int D, JobParams& J) {
   const smg_config& c = in.cfg;
   std::memset(&J, 0, sizeof(J));
@@ -93,7 +101,36 @@ int make_job_params(const JobInput& in, int D, JobParams& J) {
     J.wl_h[a] = lh; J.wl_l[a] = float(J.box_l[a] - double(lh));
     J.wh_h[a] = rh; J.wh_l[a] = float(J.box_r[a] - double(rh));
   }
-  J.dl = dl;
+  // refined level-set levels: geometry, stored packages (a halo package is stored
+  // where the interior package next to it is) and the ranges they take
+  const int nl = int(in.levels.size());
+  if (nl > kMaxLsLevels) { set_error("setup: at most 3 refined level-set levels"); return SMG_ERR_ARG; }
+  J.ls_nl = nl;
+  int64_t cells = ls_cells_padded(D, J.ng), maps = 0;
+  for (int l = 0; l < nl; l++) {
+    const LevelInput& L = in.levels[l];
+    if (!L.phi || !L.normal || !L.tag || !L.active) { set_error("setup: refined level incomplete"); return SMG_ERR_ARG; }
+    int np[3] = {1, 1, 1};
+    for (int a = 0; a < 3; a++) {
+      const bool on = a < D;
+      J.lnp[l][a] = on ? (c.num_pkg[a] << (l + 1)) : 1;
+      J.ldpkg[l][a] = on ? c.domain[a] / double(J.lnp[l][a]) : 1.;
+      J.ldcell[l][a] = on ? J.ldpkg[l][a] / 4.0 : 1.;
+      if (on) dl = std::min(dl, J.ldcell[l][a]);
+      np[a] = on ? J.lnp[l][a] + 2 : 1;
+    }
+    int64_t nb = 0;
+    for (int p2 = 0; p2 < np[2]; p2++)
+      for (int p1 = 0; p1 < np[1]; p1++)
+        for (int p0 = 0; p0 < np[0]; p0++) nb += level_pkg_active(J, l, L, p0, p1, p2) ? 1 : 0;
+    J.lbricks[l] = nb;
+    J.lcell_off[l] = cells;
+    J.lmap_off[l] = maps;
+    cells += nb * (D > 2 ? 64 : 16);
+    maps += int64_t(np[0]) * np[1] * np[2];
+  }
+  J.dl = dl;   // the finest level-set cell (Get_scale's |phi| < dl)
+  J.seed_pad = nl > 0 ? J.ldpkg[nl - 1][0] : J.dpkg[0];
   J.cell_ratio = c.cell_ratio;
   J.cut_off = c.cut_off;
   J.nu_coeff = c.nu_coeff;
@@ -150,7 +187,20 @@ int make_job_params(const JobInput& in, int D, JobParams& J) {
   return SMG_OK;
 }
 
-int64_t job_lset_cells(const JobParams& J) { return ls_cells_padded(J.D, J.ng); }
+int64_t job_lset_cells(const JobParams& J) {
+  int64_t m = ls_cells_padded(J.D, J.ng);
+  for (int l = 0; l < J.ls_nl; l++) m += J.lbricks[l] * (J.D > 2 ? 64 : 16);
+  return m;
+}
+int64_t job_lmap_entries(const JobParams& J) {
+  int64_t m = 0;
+  for (int l = 0; l < J.ls_nl; l++) {
+    int64_t e = 1;
+    for (int a = 0; a < J.D; a++) e *= J.lnp[l][a] + 2;
+    m += e;
+  }
+  return m;
+}
 
 // 4096-entry scan blocks of a job's table segment (+ one entry for a sub-scan's end
 // sentinel); segments start on block boundaries so that no scan block spans two jobs
@@ -164,7 +214,7 @@ int Engine::set_table_owner_(const JobParams& J, int32_t owner) {
   return SMG_OK;
 }
 
-int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, int spill_per_slot) {
+int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, int spill_per_slot, int64_t maps) {
   if (tbl >= (int64_t(1) << 31) - 2) { set_error("setup: cell tables exceed 2^31 entries"); return SMG_ERR_CAPACITY; }
   if (slots >= (int64_t(1) << kJBits)) { set_error("setup: more than 2^26 particle slots"); return SMG_ERR_CAPACITY; }
   cap_total_ = slots;
@@ -177,6 +227,7 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   d_acc_ = alloc_<JobAcc>(int64_t(njobs) * kAccCopies);
   d_lsp_ = alloc_<double>(n_ls_);
   d_lsn_ = alloc_<float4>(n_ls_);
+  d_lpm_ = alloc_<int32_t>(std::max<int64_t>(maps, 1));
   PState* S2[2] = {&A_, &B_};
   for (PState* S : S2) {
     for (int a = 0; a < 3; a++) {
@@ -232,6 +283,7 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   W_.acc = d_acc_;
   W_.lsp = d_lsp_;
   W_.lsn = d_lsn_;
+  W_.lpm = d_lpm_;
   W_.cap_total = cap_total_;
   W_.n_tbl = n_tbl_;
   W_.K = K_;
@@ -313,8 +365,48 @@ int Engine::upload_job_(int j, const JobInput& in) {
         std::memcpy(&tf, &tag, sizeof(tf));
         hn[o] = make_float4(in.normal[3 * q + 0], in.normal[3 * q + 1], in.normal[3 * q + 2], tf);
       }
+  // refined levels: package maps, and the bricks of the stored packages in map order
+  // (cells outside the level's one-cell-halo grid stay zero; they are never sampled)
+  std::vector<int32_t> hm(job_lmap_entries(J), -1);
+  for (int l = 0; l < J.ls_nl; l++) {
+    const LevelInput& L = in.levels[l];
+    int np[3] = {1, 1, 1}, ngl[3] = {1, 1, 1};
+    for (int a = 0; a < J.D; a++) {
+      np[a] = J.lnp[l][a] + 2;
+      ngl[a] = 4 * J.lnp[l][a] + 2;
+    }
+    const int cpb = J.D > 2 ? 64 : 16;
+    int64_t b = 0, e = J.lmap_off[l];
+    for (int p2 = 0; p2 < np[2]; p2++)
+      for (int p1 = 0; p1 < np[1]; p1++)
+        for (int p0 = 0; p0 < np[0]; p0++, e++) {
+          if (!level_pkg_active(J, l, L, p0, p1, p2)) continue;
+          hm[e] = int32_t(b);
+          const int64_t base = J.lcell_off[l] + b * cpb;
+          for (int c2 = 0; c2 < (J.D > 2 ? 4 : 1); c2++)
+            for (int c1 = 0; c1 < 4; c1++)
+              for (int c0 = 0; c0 < 4; c0++) {
+                // global cell 4 (p - 1) + c, at index + 1 on the one-cell-halo grid
+                const int g[3] = {4 * (p0 - 1) + c0 + 1, 4 * (p1 - 1) + c1 + 1, J.D > 2 ? 4 * (p2 - 1) + c2 + 1 : 0};
+                bool in_grid = true;
+                for (int a = 0; a < J.D; a++) in_grid = in_grid && g[a] >= 0 && g[a] < ngl[a];
+                if (!in_grid) continue;
+                const int64_t qq = (int64_t(g[2]) * ngl[1] + g[1]) * ngl[0] + g[0];
+                const int64_t o = base + (c2 * 4 + c1) * 4 + c0;
+                hp[o] = L.phi[qq];
+                int32_t tag = L.tag[qq];
+                float tf;
+                std::memcpy(&tf, &tag, sizeof(tf));
+                hn[o] = make_float4(L.normal[3 * qq + 0], L.normal[3 * qq + 1], L.normal[3 * qq + 2], tf);
+              }
+          b++;
+        }
+  }
   HIP_TRY(hipMemcpyAsync(d_lsp_ + J.lset_off, hp.data(), sizeof(double) * m, hipMemcpyHostToDevice, st_));
   HIP_TRY(hipMemcpyAsync(d_lsn_ + J.lset_off, hn.data(), sizeof(float4) * m, hipMemcpyHostToDevice, st_));
+  if (!hm.empty())
+    HIP_TRY(hipMemcpyAsync(d_lpm_ + J.lmap_base, hm.data(), sizeof(int32_t) * hm.size(), hipMemcpyHostToDevice, st_));
+  HIP_TRY(hipStreamSynchronize(st_));   // the host buffers go out of scope
   HIP_TRY(hipMemcpyAsync(d_jp_ + j, &J, sizeof(JobParams), hipMemcpyHostToDevice, st_));
   const int64_t t0 = J.p0 / kTile, nt = J.cap / kTile;
   for (int64_t t = t0; t < t0 + nt; t++) tile_job_h_[t] = int32_t(j);
@@ -342,7 +434,7 @@ int Engine::setup(const std::vector<JobInput>& jobs, int max_neighbors, int spil
   SMG_TRY(begin_setup_(jobs[0].cfg.dim, max_neighbors, stream));
   jp_.assign(jobs.size(), JobParams{});
   active_.assign(jobs.size(), 1);
-  int64_t p0 = 0, tbl = 0, ls = 0;
+  int64_t p0 = 0, tbl = 0, ls = 0, maps = 0;
   for (size_t j = 0; j < jobs.size(); j++) {
     JobParams& J = jp_[j];
     SMG_TRY(make_job_params(jobs[j], D_, J));
@@ -350,11 +442,13 @@ int Engine::setup(const std::vector<JobInput>& jobs, int max_neighbors, int spil
     tbl += table_blocks(J) * kScanItems;
     J.lset_off = ls;
     ls += job_lset_cells(J);
+    J.lmap_base = maps;
+    maps += job_lmap_entries(J);
     J.p0 = p0;
     p0 += J.cap;
   }
   tile_job_h_.assign(p0 / kTile, -1);
-  SMG_TRY(alloc_buffers_(p0, tbl, ls, int(jobs.size()), spill_per_slot));
+  SMG_TRY(alloc_buffers_(p0, tbl, ls, int(jobs.size()), spill_per_slot, maps));
   for (size_t j = 0; j < jobs.size(); j++) SMG_TRY(upload_job_(int(j), jobs[j]));
   ntiles_ = p0 / kTile;
   HIP_TRY(hipStreamSynchronize(st_));
@@ -435,6 +529,7 @@ int Engine::admit(const JobInput& in, int* job) {
   for (int q = 0; q < int(active_.size()); q++)
     if (!active_[q]) { j = q; break; }
   if (j < 0) { set_error("admit: all job slots in use"); return SMG_ERR_CAPACITY; }
+  if (!in.levels.empty()) { set_error("admit: refined level sets are not supported in a pool"); return SMG_ERR_ARG; }
   JobParams J;
   SMG_TRY(make_job_params(in, D_, J));
   const int64_t nt = J.cap / kTile, nls = job_lset_cells(J);
@@ -545,6 +640,24 @@ int Engine::red_of(int j, JobRed* r) {
 }
 
 int Engine::download_reds() { return download_red_(); }
+
+int Engine::sample_phi(int job, int64_t n, const double* pts, double* phi) {
+  if (n == 0) return SMG_OK;
+  double *d_pts = nullptr, *d_phi = nullptr;
+  hipError_t e = hipMalloc(&d_pts, sizeof(double) * 3 * n);
+  if (e == hipSuccess) e = hipMalloc(&d_phi, sizeof(double) * n);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_pts, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st_);
+  if (e == hipSuccess) {
+    launch_sample_phi(st_, D_, W_, job, n, d_pts, d_phi);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(phi, d_phi, sizeof(double) * n, hipMemcpyDeviceToHost, st_);
+  if (e == hipSuccess) e = hipStreamSynchronize(st_);
+  (void)hipFree(d_pts);
+  (void)hipFree(d_phi);
+  if (e != hipSuccess) { set_error(std::string("sample_phi: ") + hipGetErrorString(e)); return SMG_ERR_HIP; }
+  return SMG_OK;
+}
 
 void Engine::begin_(const char* name) {
   if (!profiling_) return;

@@ -16,6 +16,15 @@
 
 namespace smg {
 
+// a refined level-set level: its one-cell-halo grid (4 (num_pkg << l) + 2 cells per
+// axis) and one byte per package, nonzero where the level is stored
+struct LevelInput {
+  const double* phi = nullptr;
+  const float* normal = nullptr;
+  const int8_t* tag = nullptr;
+  const uint8_t* active = nullptr;
+};
+
 struct JobInput {
   smg_config cfg;
   int64_t n = 0;
@@ -23,6 +32,7 @@ struct JobInput {
   const double* phi = nullptr;    // one-cell-halo level-set grid
   const float* normal = nullptr;
   const int8_t* tag = nullptr;
+  std::vector<LevelInput> levels;   // refined levels 1, 2, ... (may be empty)
 };
 
 // first-fit allocator of [0, total) ranges with coalescing on release
@@ -50,6 +60,7 @@ class RangeSet {
 
 int make_job_params(const JobInput& in, int D, JobParams& J);
 int64_t job_lset_cells(const JobParams& J);
+int64_t job_lmap_entries(const JobParams& J);
 int64_t table_blocks(const JobParams& J);
 
 class Engine {
@@ -70,6 +81,7 @@ class Engine {
   bool active(int job) const { return job >= 0 && job < int(active_.size()) && active_[job]; }
   int red_of(int job, JobRed* r);
   int download_reds();
+  int sample_phi(int job, int64_t n, const double* pts, double* phi);
   const JobRed& red_host(int job) const { return red_h_[job]; }
   int64_t used_tiles() const { return is_pool_ ? tiles_free_.total() - tiles_free_.free_total() : ntiles_; }
 
@@ -114,7 +126,7 @@ class Engine {
   int sort_particles();
   int sort_job_(int job);
   int begin_setup_(int D, int max_neighbors, hipStream_t stream);
-  int alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, int spill_per_slot);
+  int alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, int spill_per_slot, int64_t maps = 0);
   int upload_job_(int job, const JobInput& in);
   void update_extent_();
   int set_table_owner_(const JobParams& J, int32_t owner);
@@ -148,6 +160,7 @@ class Engine {
   JobAcc* d_acc_ = nullptr;
   double* d_lsp_ = nullptr;
   float4* d_lsn_ = nullptr;
+  int32_t* d_lpm_ = nullptr;
   PState A_{}, B_{};
   Work W_{};
   std::vector<void*> allocs_;

@@ -152,7 +152,7 @@ __device__ __forceinline__ void ls_locate(const JobParams& J, const double* x, d
 // Calculate_particle_infor (particle_mesh_generation.cpp:175-265): bi/trilinear phi
 // and raw normal; the normal is left unnormalised (quirk Q3)
 template <int D, bool kNormal>
-__device__ void ls_sample(const JobParams& J, const Work& W, const double* x, double& phi, double* n) {
+__device__ void ls_sample_base(const JobParams& J, const Work& W, const double* x, double& phi, double* n) {
   double cs[3];
   int pk[3];
   ls_locate<D>(J, x, cs, pk);
@@ -188,9 +188,95 @@ __device__ void ls_sample(const JobParams& J, const Work& W, const double* x, do
       }
 }
 
-// Levelset_levelinfo::Get_id_pkg_cell (lset_level_infor.cpp:350-366) -> tag_interface
+// Refined level l (0-based: refinement l + 1) of a multi-resolution level set.  The
+// cell with global indices g (-1 .. 4 lnp per axis) in the level's bricks, or -1 where
+// the level does not store its package.
 template <int D>
-__device__ int ls_tag(const JobParams& J, const Work& W, const double* x) {
+__device__ __forceinline__ int64_t lvl_cell(const JobParams& J, const Work& W, int l, const int* g) {
+  int p[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    p[a] = (g[a] + 4) / 4 - 1;   // floor(g / 4) for g >= -4
+    c[a] = g[a] - 4 * p[a];
+  }
+  int64_t e = p[0] + 1;
+  if (D > 1) e += int64_t(p[1] + 1) * (J.lnp[l][0] + 2);
+  if (D > 2) e += int64_t(p[2] + 1) * (J.lnp[l][0] + 2) * (J.lnp[l][1] + 2);
+  const int32_t b = W.lpm[J.lmap_base + J.lmap_off[l] + e];
+  if (b < 0) return -1;
+  return J.lset_off + J.lcell_off[l] + int64_t(b) * (D > 2 ? 64 : 16) + ((c[2] * 4 + c[1]) * 4 + c[0]);
+}
+
+// Calculate_particle_infor's interpolation on refined level l: the same arithmetic as
+// the base grid with the level's package and cell sizes.  False (nothing written)
+// when a corner's package is not stored.
+template <int D, bool kNormal>
+__device__ bool ls_sample_level(const JobParams& J, const Work& W, int l, const double* x, double& phi, double* n) {
+  int pk[3] = {0, 0, 0}, m[3] = {0, 0, 0};
+  double f[3] = {0., 0., 0.};
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    double t = x[a] - J.box_l[a];
+    t = (t >= 0.) ? t : 0.;
+    const double hi = J.domain[a] - 1.e-8;
+    const double cs = (t <= hi) ? t : hi;
+    int p = int(floor(cs / J.ldpkg[l][a]));
+    p = (p <= J.lnp[l][a]) ? p : J.lnp[l][a];
+    p = (-1 >= p) ? -1 : p;
+    pk[a] = p;
+    double rr = (cs - J.ldpkg[l][a] * p) / J.ldcell[l][a] - 0.5;
+    rr = (rr <= 3.5) ? rr : 3.5;
+    rr = (-0.5 >= rr) ? -0.5 : rr;
+    m[a] = int(floor(rr));
+    f[a] = rr - double(m[a]);
+  }
+  int64_t cell[8];
+  bool ok = true;
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int s = 0; s < (D > 1 ? 2 : 1); s++)
+#pragma unroll
+      for (int t = 0; t < (D > 2 ? 2 : 1); t++) {
+        const int g[3] = {pk[0] * 4 + m[0] + r, pk[1] * 4 + m[1] + s, pk[2] * 4 + m[2] + t};
+        const int64_t c = lvl_cell<D>(J, W, l, g);
+        cell[(r * 2 + s) * 2 + t] = c;
+        ok = ok && c >= 0;
+      }
+  if (!ok) return false;
+  phi = 0.;
+  if (kNormal) n[0] = n[1] = n[2] = 0.;
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int s = 0; s < (D > 1 ? 2 : 1); s++)
+#pragma unroll
+      for (int t = 0; t < (D > 2 ? 2 : 1); t++) {
+        const int64_t c = cell[(r * 2 + s) * 2 + t];
+        double coef = fabs(f[0] - (1.0 - r));
+        if (D > 1) coef = coef * fabs(f[1] - (1.0 - s));
+        if (D > 2) coef = coef * fabs(f[2] - (1.0 - t));
+        phi += W.lsp[c] * coef;
+        if (kNormal) {
+          const float4 q = W.lsn[c];
+          n[0] += double(q.x) * coef;
+          n[1] += double(q.y) * coef;
+          n[2] += double(q.z) * coef;
+        }
+      }
+  return true;
+}
+
+// the finest level that stores the sample's corners, else the base grid
+template <int D, bool kNormal>
+__device__ void ls_sample(const JobParams& J, const Work& W, const double* x, double& phi, double* n) {
+  for (int l = J.ls_nl - 1; l >= 0; l--)
+    if (ls_sample_level<D, kNormal>(J, W, l, x, phi, n)) return;
+  ls_sample_base<D, kNormal>(J, W, x, phi, n);
+}
+
+template <int D>
+__device__ int ls_tag_base(const JobParams& J, const Work& W, const double* x) {
   double cs[3];
   int pk[3];
   ls_locate<D>(J, x, cs, pk);
@@ -203,6 +289,32 @@ __device__ int ls_tag(const JobParams& J, const Work& W, const double* x) {
     g[a] = pk[a] * 4 + c;
   }
   return __float_as_int(W.lsn[ls_index(J, D, g[0], g[1], g[2])].w);
+}
+
+// Levelset_levelinfo::Get_id_pkg_cell (lset_level_infor.cpp:350-366) -> tag_interface,
+// from the finest level that stores the cell
+template <int D>
+__device__ int ls_tag(const JobParams& J, const Work& W, const double* x) {
+  for (int l = J.ls_nl - 1; l >= 0; l--) {
+    int g[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      double t = x[a] - J.box_l[a];
+      t = (t >= 0.) ? t : 0.;
+      const double hi = J.domain[a] - 1.e-8;
+      const double cs = (t <= hi) ? t : hi;
+      int p = int(floor(cs / J.ldpkg[l][a]));
+      p = (p <= J.lnp[l][a]) ? p : J.lnp[l][a];
+      p = (-1 >= p) ? -1 : p;
+      int c = int(floor((cs - J.ldpkg[l][a] * p) / J.ldcell[l][a]));
+      c = (c <= 3) ? c : 3;
+      c = (0 >= c) ? 0 : c;
+      g[a] = p * 4 + c;
+    }
+    const int64_t c = lvl_cell<D>(J, W, l, g);
+    if (c >= 0) return __float_as_int(W.lsn[c].w);
+  }
+  return ls_tag_base<D>(J, W, x);
 }
 
 // Levelset::Get_scale (level_set.cpp:311-351) and the linear-profile extension
@@ -425,7 +537,7 @@ __global__ __launch_bounds__(kTile) void k_seed(Work W, PState S) {
   if (loc < J.n) {
     const uint64_t key = rng_key(J.seed, 0xFFFFFFFFull, uint64_t(loc));
     double x[3] = {0., 0., 0.}, phi = 0., n[3] = {0., 0., 0.}, h = 0.;
-    const double pad = J.dpkg[0];
+    const double pad = J.seed_pad;   // one package (of the finest level-set level)
     bool ok = false;
     for (int att = 0; att < J.remap_max && !ok; att++) {
 #pragma unroll

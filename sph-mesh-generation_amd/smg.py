@@ -46,6 +46,10 @@ class SmgLsetStats(C.Structure):
     ]
 
 
+class SmgLsetLevel(C.Structure):
+    _fields_ = [("phi", C.c_void_p), ("normal", C.c_void_p), ("tag", C.c_void_p), ("active", C.c_void_p)]
+
+
 class SmgStats(C.Structure):
     _fields_ = [
         ("step", C.c_int64), ("n_particle", C.c_int64), ("n_real", C.c_int64), ("n_surface", C.c_int64),
@@ -77,6 +81,8 @@ def lib():
         L.smg_create.argtypes = [P(SmgConfig), P(C.c_void_p)]
         L.smg_destroy.argtypes = [C.c_void_p]
         L.smg_set_levelset.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.smg_set_levelset_levels.argtypes = [C.c_void_p, C.c_int32, P(SmgLsetLevel)]
+        L.smg_sample_phi.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         L.smg_init_particles.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         L.smg_seed_particles.argtypes = [C.c_void_p, C.c_int64]
         L.smg_prerun.argtypes = [C.c_void_p]
@@ -196,6 +202,111 @@ def make_levelset(cfg, sdf, zero_margin=True):
     return lset_build(cfg, sdf(X, Y, Z), zero_margin)
 
 
+def config_refined(cfg, level):
+    """The config of refined level-set level `level` (num_pkg << level packages)."""
+    c = config_copy(cfg)
+    for a in range(cfg.dim):
+        c.num_pkg[a] = cfg.num_pkg[a] << level
+    return c
+
+
+def make_levelset_levels(cfg, sdf, nlev, band=4.0, zero_margin=True):
+    """Refined levels 1..nlev over the base level set of `cfg` (the multi-resolution
+    extension, smg_set_levelset_levels): level l is built like a single-level set with
+    num_pkg << l packages, and keeps the packages that hold a cell with |phi| < band
+    times the level's package size; band = None keeps every package."""
+    D = cfg.dim
+    out = []
+    for l in range(1, nlev + 1):
+        cl = config_refined(cfg, l)
+        ls = make_levelset(cl, sdf, zero_margin)
+        np_ = [cl.num_pkg[a] if a < D else 1 for a in range(3)]
+        halo = [4 * np_[a] + 2 if a < D else 1 for a in range(3)]
+        phi = ls["phi"].reshape(halo[2], halo[1], halo[0])
+        inner = phi[1:-1, 1:-1, 1:-1] if D > 2 else phi[:, 1:-1, 1:-1]
+        if band is None:
+            act = np.ones((np_[2], np_[1], np_[0]), np.uint8)
+        else:
+            dpkg = cl.domain[0] / float(cl.num_pkg[0])
+            near = np.abs(inner) < band * dpkg
+            if D > 2:
+                act = near.reshape(np_[2], 4, np_[1], 4, np_[0], 4).any(axis=(1, 3, 5))
+            else:
+                act = near.reshape(1, np_[1], 4, np_[0], 4).any(axis=(2, 4))
+            act = act.astype(np.uint8)
+        ls["active"] = np.ascontiguousarray(act.ravel())
+        ls["cfg"] = cl
+        out.append(ls)
+    return out
+
+
+def sdf_ellipsoid_union(n=12, seed=20261203, lo=0.2, hi=0.8, amin=0.05, amax=0.2, exact_band=0.04):
+    """phi > 0 inside the union of n seeded ellipsoids (centres uniform in [lo, hi]^3,
+    semi-axes uniform in [amin, amax], rotations from uniform unit quaternions): the
+    signed distance to each ellipsoid, exact (the closest-point equation solved by
+    bisection on its Lagrange multiplier) within exact_band of the surface and the
+    algebraic estimate (|p/a| - 1) min(a) further out, combined by max."""
+    rng = np.random.default_rng(seed)
+    cs = rng.uniform(lo, hi, (n, 3))
+    ax = rng.uniform(amin, amax, (n, 3))
+    # Shoemake: uniform unit quaternions
+    u = rng.uniform(0., 1., (n, 3))
+    q = np.stack([np.sqrt(1 - u[:, 0]) * np.sin(2 * np.pi * u[:, 1]), np.sqrt(1 - u[:, 0]) * np.cos(2 * np.pi * u[:, 1]),
+                  np.sqrt(u[:, 0]) * np.sin(2 * np.pi * u[:, 2]), np.sqrt(u[:, 0]) * np.cos(2 * np.pi * u[:, 2])], 1)
+    Rs = []
+    for x, y, z, w in q:
+        Rs.append(np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                            [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                            [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]]))
+
+    def dist(P, a):   # signed distance to the axis-aligned ellipsoid (positive outside)
+        P = np.abs(P)
+        g = np.sqrt(((P / a) ** 2).sum(1))
+        out = (g - 1.0) * a.min()
+        sel = np.abs(out) < exact_band
+        if sel.any():
+            p = np.maximum(P[sel], 1e-300)
+            a2 = a * a
+            # F(t) = sum (a_i p_i / (t + a_i^2))^2 - 1 decreases in t > -min a^2; its root
+            # gives the closest point x_i = a_i^2 p_i / (t + a_i^2)
+            t_lo = np.full(len(p), -a2.min() + 1e-14)
+            t_hi = np.full(len(p), np.linalg.norm(p, axis=1) * a.max())
+            for _ in range(48):   # the bracket starts below |p| max(a) <= 0.2: 2^-48 of it
+                t = 0.5 * (t_lo + t_hi)
+                F = ((a * p / (t[:, None] + a2)) ** 2).sum(1) - 1.0
+                t_lo = np.where(F > 0, t, t_lo)
+                t_hi = np.where(F > 0, t_hi, t)
+            t = 0.5 * (t_lo + t_hi)
+            xc = a2 * p / (t[:, None] + a2)
+            d = np.linalg.norm(p - xc, axis=1)
+            out[sel] = np.where(g[sel] >= 1.0, d, -d)
+        return out
+
+    def f(X, Y, Z):
+        P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+        best = None
+        for c, a, R in zip(cs, ax, Rs):
+            v = -dist((P - c) @ R, a)
+            best = v if best is None else np.maximum(best, v)
+        return best.reshape(X.shape)
+    return f
+
+
+def config_c3(num_pkg=16, dx0=0.0015, ratio=8.0, band=0.1, seed=20261203):
+    """3D multi-resolution case: inside the union of 12 seeded ellipsoids in the unit
+    cube; dx linear in phi from dx0 at the surface to ratio * dx0 at phi >= band; the
+    base level set (4 num_pkg)^3 cells, refined twice (make_levelset_levels)."""
+    cfg = default_config(3)
+    for a in range(3):
+        cfg.num_pkg[a] = num_pkg
+    cfg.profile = PROFILE_LINEAR
+    cfg.lin_dx0 = dx0
+    cfg.lin_dx1 = dx0 * ratio
+    cfg.lin_band = band
+    cfg.seed = seed
+    return cfg, sdf_ellipsoid_union(seed=seed)
+
+
 def config_c1(h_over_dx=1.0, num_pkg=32, dx=0.01, seed=20261203):
     """2D: [0,1]^2 minus a circle r = 0.2 at the centre, uniform dx, lset 128^2 cells."""
     cfg = default_config(2)
@@ -229,7 +340,7 @@ def config_c2(scale=0.564, num_pkg=64, seed=20261203):
 class Job:
     """One relaxation job on the GPU (SPH_mesh_generation driven through the C ABI)."""
 
-    def __init__(self, cfg, lset, profile=False):
+    def __init__(self, cfg, lset, profile=False, levels=None):
         self.cfg = config_copy(cfg)
         self.D = cfg.dim
         h = C.c_void_p()
@@ -238,6 +349,12 @@ class Job:
         self.lset = lset
         _check(lib().smg_set_levelset(self.h, _ptr(lset["phi"]), _ptr(lset["normal"]), _ptr(lset["tag"])),
                "smg_set_levelset")
+        self.levels = levels or []
+        if self.levels:
+            arr = (SmgLsetLevel * len(self.levels))()
+            for i, L in enumerate(self.levels):
+                arr[i] = SmgLsetLevel(_ptr(L["phi"]), _ptr(L["normal"]), _ptr(L["tag"]), _ptr(L["active"]))
+            _check(lib().smg_set_levelset_levels(self.h, len(self.levels), arr), "smg_set_levelset_levels")
         if profile:
             _check(lib().smg_profile_enable(self.h, 1), "smg_profile_enable")
         self.n = 0
@@ -310,6 +427,14 @@ class Job:
         ids = np.zeros(max(m, 1), np.int32); codes = np.zeros(max(m, 1), np.int32)
         _check(lib().smg_get_neighbors(self.h, _ptr(offs), _ptr(ids), _ptr(codes), m), "smg_get_neighbors")
         return offs, ids[:m], codes[:m]
+
+    def sample_phi(self, pts):
+        """Level-set phi at points (n x 3) as the relaxation samples it."""
+        p = np.zeros((len(pts), 3))
+        p[:, :np.asarray(pts).shape[1]] = pts
+        out = np.zeros(len(p))
+        _check(lib().smg_sample_phi(self.h, len(p), _ptr(p), _ptr(out)), "smg_sample_phi")
+        return out
 
     def stats(self):
         st = SmgStats()
