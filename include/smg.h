@@ -280,6 +280,61 @@ int smg_tri_edges(smg_job* job, int32_t* edges, int64_t capacity, int64_t* n_edg
 int smg_tri_cliques(int64_t n_vertices, const int32_t* edges, int64_t n_edges, int32_t* tris, int64_t capacity,
                     int64_t* n_tri);
 
+/* ---- 3D tetrahedralisation (Graphmeshcls::Reconstruct_tets, graph_mesh.cpp:981-1963) ----
+ * Delaunay tetrahedra of the particle positions (the reference calls TetGen "JCnQ",
+ * graph_mesh.cpp:1021) from each particle's Voronoi cell, clipped on the GPU in FP64.
+ * A tetrahedron is produced when its circumsphere is certified empty by the cell
+ * search (every particle within twice its circumradius of a vertex was clipped); the
+ * convex-hull slivers and cavity-spanning tetrahedra of a global tetrahedralisation,
+ * whose circumspheres are far larger than the spacing, are not produced (n_open counts
+ * the particles whose cells stayed open).  Then the reference's deletion rule
+ * (Check_whether_this_tet_should_be_delete, graph_mesh.cpp:675-979): a tetrahedron of
+ * four non-REAL particles whose aspect ratio (longest edge / smallest height) exceeds
+ * delete_thres goes (the phi tests there are computed but do not decide).  Boundary
+ * triangles: the face of three non-REAL particles of a kept tetrahedron with exactly one
+ * REAL particle (graph_mesh.cpp:1153-1176).  Quality statistics follow
+ * Tet_quality_statistics (graph_mesh.cpp:1967-2460).  TetGen's flip / optimisation pass
+ * ("JrO/1o/100Q", graph_mesh.cpp:1401) is TetGen-internal and not reproduced. */
+typedef struct smg_tet_stats {
+  int64_t n_tet;                   /* tetrahedra after the deletion rule                  */
+  int64_t n_delaunay;              /* certified Delaunay tetrahedra of the GPU stage       */
+  int64_t n_deleted;               /* four non-REAL particles and aspect > delete_thres    */
+  int64_t n_surface_tri;           /* boundary triangles                                   */
+  int64_t n_open;                  /* particles whose cell was not closed by the search    */
+  int64_t n_cell_overflow;         /* particles whose cell could not be built             */
+  int64_t n_duplicate;             /* particles on the point of a smaller id (in no tet)   */
+  double min_edge, max_edge;
+  double min_volume, max_volume, total_volume;
+  double min_aspect, max_aspect, avg_aspect;                     /* longest edge / min height */
+  double min_radius_ratio, max_radius_ratio, avg_radius_ratio;   /* 3 r_in / r_circ           */
+  double min_dihedral, max_dihedral, avg_min_dihedral;           /* degrees                   */
+  int64_t n_min_dihedral_below[4]; /* tetrahedra with min dihedral < 10, 20, 30, 40 degrees */
+  int64_t dihedral_hist[45];       /* dihedral angles in 4-degree bins                     */
+  int64_t aspect_hist[12];         /* bins bounded by 1.5 2 2.5 3 4 6 10 15 25 50 100      */
+  int64_t radius_hist[56];         /* radius ratio in 0.02 bins, the last one open         */
+  double cell_ms;                  /* GPU time of the Voronoi-cell pass                    */
+  double total_ms;                 /* the whole call, downloads and host stage included    */
+} smg_tet_stats;
+
+/* Tetrahedralises a 3D job's current particles.  tets: 4 particle ids per tetrahedron
+ * (ascending), sorted; may be NULL to query the count; capacity in tetrahedra.
+ * delete_thres <= 0 takes the reference's 10 (sph_mesh_generation.cpp:705). */
+int smg_tetrahedralize(smg_job* job, double delete_thres, int32_t* tets, int64_t capacity, int64_t* n_tet,
+                       smg_tet_stats* stats);
+/* Boundary triangles (3 ascending particle ids, sorted) of the last smg_tetrahedralize. */
+int smg_tet_surface(smg_job* job, int32_t* tris, int64_t capacity, int64_t* n_tri);
+/* The GPU stage of the last smg_tetrahedralize: the certified Delaunay tetrahedra
+ * (before deletion, ascending ids, sorted) and, by particle id, the cell status (0
+ * closed, 1 open at the search limit; 2 capacity, 3-5 degenerate cut: not built; 6 on
+ * the point of a smaller id) and the certified radius.  status and
+ * rcert may be NULL; they hold n_particles entries. */
+int smg_tet_delaunay(smg_job* job, int32_t* tets, int64_t capacity, int64_t* n_tet, int8_t* status, double* rcert);
+/* The host stage alone (no GPU): deletion rule, boundary triangles and statistics of
+ * given tetrahedra over positions x (3 per vertex) and particle types. */
+int smg_tet_finish(int64_t n_vertices, const double* x, const int32_t* type, const int32_t* tets, int64_t n_tets,
+                   double delete_thres, int32_t* kept, int64_t kept_capacity, int64_t* n_kept, int32_t* tris,
+                   int64_t tri_capacity, int64_t* n_tri, smg_tet_stats* stats);
+
 /* ---- serving: continuous batching over a device-resident job pool ----
  * No counterpart in the reference.  A server owns one pool of particle slots, cell
  * tables and level-set cells on the device (its HBM footprint, fixed at creation).

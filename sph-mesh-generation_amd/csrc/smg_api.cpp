@@ -496,6 +496,46 @@ int smg_tri_edges(smg_job* job, int32_t* edges, int64_t capacity, int64_t* n_edg
   return SMG_OK;
 }
 
+int smg_tetrahedralize(smg_job* job, double delete_thres, int32_t* tets, int64_t capacity, int64_t* n_tet,
+                       smg_tet_stats* stats) {
+  if (!job || !job->eng || !n_tet) { set_error("smg_tetrahedralize: bad argument"); return SMG_ERR_STATE; }
+  smg_tet_stats st;
+  const int rc = job->eng->tetrahedralize(0, delete_thres, &st);
+  if (rc != SMG_OK) return rc;
+  const auto& t = job->eng->last_tets().kept;
+  *n_tet = int64_t(t.size());
+  if (stats) *stats = st;
+  if (!tets) return SMG_OK;
+  if (int64_t(t.size()) > capacity) { set_error("smg_tetrahedralize: capacity too small"); return SMG_ERR_CAPACITY; }
+  for (size_t q = 0; q < t.size(); q++)
+    for (int a = 0; a < 4; a++) tets[4 * q + a] = t[q][a];
+  return SMG_OK;
+}
+
+int smg_tet_surface(smg_job* job, int32_t* tris, int64_t capacity, int64_t* n_tri) {
+  if (!job || !job->eng || !n_tri) { set_error("smg_tet_surface: bad argument"); return SMG_ERR_STATE; }
+  const auto& t = job->eng->last_tets().surface;
+  *n_tri = int64_t(t.size());
+  if (!tris) return SMG_OK;
+  if (int64_t(t.size()) > capacity) { set_error("smg_tet_surface: capacity too small"); return SMG_ERR_CAPACITY; }
+  for (size_t q = 0; q < t.size(); q++)
+    for (int a = 0; a < 3; a++) tris[3 * q + a] = t[q][a];
+  return SMG_OK;
+}
+
+int smg_tet_delaunay(smg_job* job, int32_t* tets, int64_t capacity, int64_t* n_tet, int8_t* status, double* rcert) {
+  if (!job || !job->eng || !n_tet) { set_error("smg_tet_delaunay: bad argument"); return SMG_ERR_STATE; }
+  const auto& R = job->eng->last_tets();
+  *n_tet = int64_t(R.delaunay.size());
+  if (status) std::copy(R.status.begin(), R.status.end(), status);
+  if (rcert) std::copy(R.rcert.begin(), R.rcert.end(), rcert);
+  if (!tets) return SMG_OK;
+  if (int64_t(R.delaunay.size()) > capacity) { set_error("smg_tet_delaunay: capacity too small"); return SMG_ERR_CAPACITY; }
+  for (size_t q = 0; q < R.delaunay.size(); q++)
+    for (int a = 0; a < 4; a++) tets[4 * q + a] = R.delaunay[q][a];
+  return SMG_OK;
+}
+
 int smg_autotune(const smg_config* cfg, const double* phi, const float* normal, const int8_t* tag, int64_t n,
                  int32_t warmup, int32_t steps, smg_tune_result* res) {
   if (check_config(cfg) != SMG_OK) return SMG_ERR_ARG;

@@ -104,6 +104,15 @@ class Engine {
   int triangulate(int job, std::vector<std::array<int32_t, 3>>& tris, smg_tri_stats* st);   // smg_tri.cpp
   // Voronoi-stage edges (i > j) of the last triangulate call
   const std::vector<std::array<int32_t, 2>>& last_edges() const { return last_edges_; }
+  // 3D tetrahedralisation (smg_tet.cpp) and the results of its last call
+  struct TetResult {
+    std::vector<std::array<int32_t, 4>> delaunay, kept;   // ascending ids, sorted
+    std::vector<std::array<int32_t, 3>> surface;
+    std::vector<int8_t> status;                           // by particle id
+    std::vector<double> rcert;
+  };
+  int tetrahedralize(int job, double delete_thres, smg_tet_stats* st);
+  const TetResult& last_tets() const { return tet_; }
   int check_health();   // remap failures, neighbour overflow
 
   void profile_enable(bool on);
@@ -141,6 +150,7 @@ class Engine {
   bool own_stream_ = false;
   std::vector<JobParams> jp_;
   std::vector<std::array<int32_t, 2>> last_edges_;
+  TetResult tet_;
   std::vector<uint8_t> active_;
   bool is_pool_ = false;
   bool alloc_failed_ = false;

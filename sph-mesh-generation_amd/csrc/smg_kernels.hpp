@@ -31,6 +31,27 @@ void launch_residual(hipStream_t st, int D, int64_t ntiles, const Work& W, const
 void launch_vor2d(hipStream_t st, int64_t ntiles, const Work& W, const PState& S, int32_t* out_nb, int32_t* out_cnt,
                   int KE);
 void launch_sample_phi(hipStream_t st, int D, const Work& W, int job, int64_t n, const double* pts, double* phi);
+
+// 3D Voronoi cells -> Delaunay tetrahedra (smg_vor3d.hip)
+struct V3Grid {
+  const double* x;        // 3 per particle, by particle id
+  const int32_t* start;   // cell starts, g0 g1 g2 + 1 entries (x fastest)
+  const int32_t* order;   // particle ids in cell order
+  int64_t n;
+  int32_t g[3];
+  double lo[3];           // grid origin
+  double s;               // cell size
+  int32_t rmax;           // outermost ring of cells searched
+};
+struct V3Out {
+  int32_t* tets;                // 4 ascending particle ids per tetrahedron
+  unsigned long long* count;    // tetrahedra found (may exceed cap: then none past cap is written)
+  int64_t cap;
+  int8_t* status;               // by id: 0 cell closed, 1 open at rmax; not built: 2 capacity,
+                                //   3 empty cut boundary, 4 non-convex cut, 5 singular vertex
+  double* rcert;                // by id: every point within this radius was clipped
+};
+void launch_vor3d(hipStream_t st, const V3Grid& G, const V3Out& O);
 void launch_fp64_probe(hipStream_t st, int64_t n, const double* a, const double* b, double* out);
 
 }  // namespace smg

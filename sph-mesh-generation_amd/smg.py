@@ -592,6 +592,90 @@ def tri_cliques(n_vertices, edges):
     return out[:m.value]
 
 
+class SmgTetStats(C.Structure):
+    _fields_ = [
+        ("n_tet", C.c_int64), ("n_delaunay", C.c_int64), ("n_deleted", C.c_int64), ("n_surface_tri", C.c_int64),
+        ("n_open", C.c_int64), ("n_cell_overflow", C.c_int64), ("n_duplicate", C.c_int64),
+        ("min_edge", C.c_double), ("max_edge", C.c_double),
+        ("min_volume", C.c_double), ("max_volume", C.c_double), ("total_volume", C.c_double),
+        ("min_aspect", C.c_double), ("max_aspect", C.c_double), ("avg_aspect", C.c_double),
+        ("min_radius_ratio", C.c_double), ("max_radius_ratio", C.c_double), ("avg_radius_ratio", C.c_double),
+        ("min_dihedral", C.c_double), ("max_dihedral", C.c_double), ("avg_min_dihedral", C.c_double),
+        ("n_min_dihedral_below", C.c_int64 * 4), ("dihedral_hist", C.c_int64 * 45),
+        ("aspect_hist", C.c_int64 * 12), ("radius_hist", C.c_int64 * 56),
+        ("cell_ms", C.c_double), ("total_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        for k in ("n_min_dihedral_below", "dihedral_hist", "aspect_hist", "radius_hist"):
+            d[k] = list(d[k])
+        return d
+
+
+def _tet_sigs(L):
+    P = C.POINTER
+    L.smg_tetrahedralize.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_int64, P(C.c_int64), P(SmgTetStats)]
+    L.smg_tet_surface.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, P(C.c_int64)]
+    L.smg_tet_delaunay.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, P(C.c_int64), C.c_void_p, C.c_void_p]
+    L.smg_tet_finish.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_double,
+                                 C.c_void_p, C.c_int64, P(C.c_int64), C.c_void_p, C.c_int64, P(C.c_int64),
+                                 P(SmgTetStats)]
+
+
+def tetrahedralize(job, delete_thres=0.0):
+    """(tets (m x 4 ascending particle ids, sorted), stats) of a 3D job's current particles."""
+    L = lib(); _tet_sigs(L)
+    m = C.c_int64()
+    st = SmgTetStats()
+    # a Delaunay tetrahedralisation has about 6.5 tetrahedra per point
+    tets = np.zeros((max(8 * job.n, 16), 4), np.int32)
+    rc = L.smg_tetrahedralize(job.h, float(delete_thres), _ptr(tets), len(tets), C.byref(m), C.byref(st))
+    if rc == -3 and m.value > len(tets):   # SMG_ERR_CAPACITY: computed, not copied
+        tets = np.zeros((m.value, 4), np.int32)
+        rc = L.smg_tetrahedralize(job.h, float(delete_thres), _ptr(tets), len(tets), C.byref(m), C.byref(st))
+    _check(rc, "smg_tetrahedralize")
+    return tets[:m.value], st.as_dict()
+
+
+def tet_surface(job):
+    """Boundary triangles (ascending ids, sorted) of the job's last tetrahedralize()."""
+    L = lib(); _tet_sigs(L)
+    m = C.c_int64()
+    _check(L.smg_tet_surface(job.h, None, 0, C.byref(m)), "smg_tet_surface")
+    t = np.zeros((max(m.value, 1), 3), np.int32)
+    _check(L.smg_tet_surface(job.h, _ptr(t), m.value, C.byref(m)), "smg_tet_surface")
+    return t[:m.value]
+
+
+def tet_delaunay(job, n):
+    """(certified Delaunay tets, per-particle cell status, certified radius) of the job's
+    last tetrahedralize(); n = the job's particle count."""
+    L = lib(); _tet_sigs(L)
+    m = C.c_int64()
+    _check(L.smg_tet_delaunay(job.h, None, 0, C.byref(m), None, None), "smg_tet_delaunay")
+    t = np.zeros((max(m.value, 1), 4), np.int32)
+    status = np.zeros(n, np.int8)
+    rcert = np.zeros(n)
+    _check(L.smg_tet_delaunay(job.h, _ptr(t), m.value, C.byref(m), _ptr(status), _ptr(rcert)), "smg_tet_delaunay")
+    return t[:m.value], status, rcert
+
+
+def tet_finish(x, types, tets, delete_thres=0.0):
+    """Host stage alone: (kept tets, boundary triangles, stats) of given tetrahedra."""
+    L = lib(); _tet_sigs(L)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    ty = np.ascontiguousarray(types, np.int32)
+    t = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+    kept = np.zeros((max(len(t), 1), 4), np.int32)
+    tris = np.zeros((max(4 * len(t), 1), 3), np.int32)
+    nk, nt = C.c_int64(), C.c_int64()
+    st = SmgTetStats()
+    _check(L.smg_tet_finish(len(x), _ptr(x), _ptr(ty), _ptr(t), len(t), float(delete_thres), _ptr(kept), len(kept),
+                            C.byref(nk), _ptr(tris), len(tris), C.byref(nt), C.byref(st)), "smg_tet_finish")
+    return kept[:nk.value], tris[:nt.value], st.as_dict()
+
+
 class SmgTuneResult(C.Structure):
     _fields_ = [("best_skin", C.c_double), ("best_max_neighbors", C.c_int32), ("n_tried", C.c_int32),
                 ("best_ms_per_step", C.c_double), ("skin", C.c_double * 16), ("max_neighbors", C.c_int32 * 16),
