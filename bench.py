@@ -335,6 +335,9 @@ def serve_once(smg, specs, lam, duration, max_jobs, budget_gb, rng_seed, label, 
     arrivals = arrivals[arrivals < duration]
     queue, arrived_at, done = [], {}, []
     k = 0
+    t_adv = t_adm = 0.0
+    ps_adv = 0   # particle-steps the advances covered
+    live = 0     # particles admitted and not yet finished
     t0 = time.perf_counter()
     while True:
         now = time.perf_counter() - t0
@@ -348,18 +351,25 @@ def serve_once(smg, specs, lam, duration, max_jobs, budget_gb, rng_seed, label, 
             c, ls, n = specs[i % len(specs)]
             cc = smg.SmgConfig.from_buffer_copy(c)
             cc.seed = c.seed + 1000003 * i
+            ta = time.perf_counter()
             t = srv.admit(cc, ls, n)
+            t_adm += time.perf_counter() - ta
             if t is None:
                 break
             arrived_at[t] = (arrivals[i], n)
+            live += n
             queue.pop(0)
         if srv.usage()["n_active"] == 0:
             if k < len(arrivals):
                 time.sleep(max(0.0, min(arrivals[k] - (time.perf_counter() - t0), duration - now)))
             continue
+        ta = time.perf_counter()
         srv.advance()
+        t_adv += time.perf_counter() - ta
+        ps_adv += live * srv.cfg.chunk_steps
         for f in srv.finished():
             ta, n = arrived_at[f["ticket"]]
+            live -= n
             done.append({"latency": time.perf_counter() - t0 - ta, "n": n, "steps": f["steps"], "status": f["status"]})
             srv.result(f["ticket"])
     foot = smg.server_footprint(sc)
@@ -370,7 +380,10 @@ def serve_once(smg, specs, lam, duration, max_jobs, budget_gb, rng_seed, label, 
          "mean_steps": float(np.mean([d["steps"] for d in done])) if done else None,
          "converged": sum(d["status"] == 0 for d in done),
          "particle_steps_per_s": sum(d["n"] * d["steps"] for d in done) / duration,
-         "pool_footprint_GB": foot / 1e9, "max_jobs": jobs_cap}
+         "pool_footprint_GB": foot / 1e9, "max_jobs": jobs_cap,
+         # where the window went: stepping (with retirement downloads) and admission
+         # (level-set upload, seeding, pre-run); the device rate while stepping
+         "advance_s": t_adv, "admit_s": t_adm, "stepping_particle_steps_per_s": ps_adv / max(t_adv, 1e-9)}
     log(f"{label} lam={lam}: {json.dumps(r)}")
     return r
 
