@@ -1414,14 +1414,12 @@ __device__ __forceinline__ void for_each_neighbor(const Work& W, const PState& S
     for (int g = 0; 4 * g < ni; g++) {
       const uint32_t ev[4] = {gn.x, gn.y, gn.z, gn.w};
       float4 qh[4], ql[4];
+      // past the end of the list a lane loads its own record (always valid) and skips it
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        qh[u] = ql[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (4 * g + u < ni) {
-          const int64_t q = ev[u] & kJMask;
-          qh[u] = W.rec[4 * q + 0];
-          ql[u] = W.rec[4 * q + 1];
-        }
+        const int64_t q = 4 * g + u < ni ? int64_t(ev[u] & kJMask) : slot;
+        qh[u] = W.rec[4 * q + 0];
+        ql[u] = W.rec[4 * q + 1];
       }
       if (4 * g + 4 < ni) gn = nb_group(W, slot, g + 1);
 #pragma unroll
