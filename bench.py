@@ -230,6 +230,9 @@ def c3(args, smg):
     |v| dt / dx) is below 1e-3."""
     t0 = time.perf_counter()
     cfg, sdf = smg.config_c3()
+    # the seeded start is uniform in space, so the particles of the coarse interior
+    # (h 8x the surface's) start with lists of up to ~1200 entries: a larger spill pool
+    cfg.spill_per_slot = 512
     base = smg.make_levelset(cfg, sdf)
     lv = smg.make_levelset_levels(cfg, sdf, 2)
     n = int(lv[-1]["stats"].num_particle)

@@ -127,6 +127,43 @@ int smg_default_config(int dim, smg_config* cfg);
 int smg_lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin,
                    double* phi, float* normal, int8_t* tag, smg_lset_stats* stats);
 
+/* The reference's level-set restart file (./rstfile/levelset.rst), the input of its
+ * relaxation run: case header (SPH_mesh_generation::Load_rst_case,
+ * sph_mesh_generation.cpp:323-373), level-set header (Levelset::Load_lset_restart_file,
+ * level_set.cpp:1098-1142), level info and per-cell fields over the interior cells
+ * (Levelset_levelinfo::Load_level_set_rstfile, lset_level_infor.cpp:932-1146). */
+typedef struct smg_rst_info {
+  int32_t resolution[3], Lmin, Lmax, ini_num_cell[3], num_level;
+  int32_t num_singularity, num_segment, glbl_num_interface_cell, reserved;
+  double convergence_error;
+  double fac_maximum_dl, fac_middel_dl, fac_minimum_dl, maximum_curv_artificial;
+  double total_mass, total_mass_surface, total_mass_segment, total_volume, total_area, total_length;
+  double maximum_phi, maximum_psi, maximum_dl, middel_dl, minimum_dl, maximum_curv, minimum_curv;
+  double scale, dl;
+  int64_t n_cells;       /* interior cells (4 num_pkg per active axis)                  */
+  int64_t n_halo_cells;  /* cells of the one-cell-halo grid (phi, normal, tag outputs)   */
+  int64_t n_ext_cells;   /* cells of the full-margin grid (phi_ext output)               */
+} smg_rst_info;
+
+/* Reads a restart file into a config (box, packages, particle box, the file's tanh
+ * spacing profile: minimum_dl, maximum_dl, maximum_phi) and the engine's level-set
+ * arrays on the one-cell-halo grid of smg_lset_build: phi, normal (3 per cell) and
+ * tag (1 = CUT_CELL) from the file, halo cells zero as after the reference's restart
+ * (the margin packages are not in the file).  phi_ext (may be NULL) gets phi on the
+ * full-margin grid with zero margins, the input smg_lset_build(zero_margin = 1) needs
+ * to recompute the particle count as the reference's restart mode does.  With every
+ * array NULL only cfg and info (with the array sizes) are filled.  Files with
+ * singularity or segment cells are rejected (SMG_ERR_ARG). */
+int smg_lset_read_rst(const char* path, smg_config* cfg, smg_rst_info* info, double* phi, float* normal,
+                      int8_t* tag, double* phi_ext);
+/* Writes a level set (one-cell-halo arrays of smg_lset_build and its statistics) in the
+ * same layout, as Levelset::Save_lset_restart_file (level_set.cpp:1022-1094) and
+ * Levelset_levelinfo::Write_level_set_rstfile (lset_level_infor.cpp:695-927) lay it
+ * out for one rank: curv and psi 0, no characteristic cells, scale = the spacing
+ * profile's dx(phi). */
+int smg_lset_write_rst(const char* path, const smg_config* cfg, const double* phi, const float* normal,
+                       const int8_t* tag, const smg_lset_stats* stats);
+
 /* SPH_mesh_generation constructor + Initialize_case (sph_mesh_generation.cpp:8, :594). */
 int smg_create(const smg_config* cfg, smg_job** job);
 int smg_destroy(smg_job* job);

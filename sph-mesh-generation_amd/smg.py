@@ -197,6 +197,59 @@ def sdf_sphere_union(centers, radii, dim):
     return f
 
 
+class SmgRstInfo(C.Structure):
+    _fields_ = [
+        ("resolution", C.c_int32 * 3), ("Lmin", C.c_int32), ("Lmax", C.c_int32), ("ini_num_cell", C.c_int32 * 3),
+        ("num_level", C.c_int32), ("num_singularity", C.c_int32), ("num_segment", C.c_int32),
+        ("glbl_num_interface_cell", C.c_int32), ("reserved", C.c_int32),
+        ("convergence_error", C.c_double),
+        ("fac_maximum_dl", C.c_double), ("fac_middel_dl", C.c_double), ("fac_minimum_dl", C.c_double),
+        ("maximum_curv_artificial", C.c_double),
+        ("total_mass", C.c_double), ("total_mass_surface", C.c_double), ("total_mass_segment", C.c_double),
+        ("total_volume", C.c_double), ("total_area", C.c_double), ("total_length", C.c_double),
+        ("maximum_phi", C.c_double), ("maximum_psi", C.c_double), ("maximum_dl", C.c_double),
+        ("middel_dl", C.c_double), ("minimum_dl", C.c_double), ("maximum_curv", C.c_double),
+        ("minimum_curv", C.c_double), ("scale", C.c_double), ("dl", C.c_double),
+        ("n_cells", C.c_int64), ("n_halo_cells", C.c_int64), ("n_ext_cells", C.c_int64),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        for k in ("resolution", "ini_num_cell"):
+            d[k] = list(d[k])
+        return d
+
+
+def write_levelset_rst(path, cfg, lset):
+    """The reference's levelset.rst layout (smg_lset_write_rst) for a level set of
+    make_levelset / lset_build."""
+    L = lib()
+    L.smg_lset_write_rst.argtypes = [C.c_char_p, C.POINTER(SmgConfig), C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.POINTER(SmgLsetStats)]
+    _check(L.smg_lset_write_rst(os.fsencode(path), C.byref(cfg), _ptr(lset["phi"]), _ptr(lset["normal"]),
+                                _ptr(lset["tag"]), C.byref(lset["stats"])), "smg_lset_write_rst")
+
+
+def read_levelset_rst(path):
+    """(cfg, lset, info, phi_ext) from a levelset.rst (smg_lset_read_rst): lset holds
+    the file's phi / normal / tag on the one-cell-halo grid and, as "stats", the
+    statistics smg_lset_build recomputes from phi with zero margins (the reference's
+    restart mode); info is the file's header."""
+    L = lib()
+    L.smg_lset_read_rst.argtypes = [C.c_char_p, C.POINTER(SmgConfig), C.POINTER(SmgRstInfo), C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p]
+    cfg, info = SmgConfig(), SmgRstInfo()
+    _check(L.smg_lset_read_rst(os.fsencode(path), C.byref(cfg), C.byref(info), None, None, None, None),
+           "smg_lset_read_rst")
+    nh, ne = int(info.n_halo_cells), int(info.n_ext_cells)
+    phi = np.zeros(nh); normal = np.zeros(3 * nh, np.float32); tag = np.zeros(nh, np.int8)
+    phi_ext = np.zeros(ne)
+    _check(L.smg_lset_read_rst(os.fsencode(path), C.byref(cfg), C.byref(info), _ptr(phi), _ptr(normal), _ptr(tag),
+                               _ptr(phi_ext)), "smg_lset_read_rst")
+    stats = lset_build(cfg, phi_ext, zero_margin=True)["stats"]
+    return cfg, {"phi": phi, "normal": normal, "tag": tag, "stats": stats}, info, phi_ext
+
+
 def make_levelset(cfg, sdf, zero_margin=True):
     X, Y, Z = cell_centres_ext(cfg)
     return lset_build(cfg, sdf(X, Y, Z), zero_margin)
