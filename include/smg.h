@@ -156,6 +156,13 @@ typedef struct smg_rst_info {
  * singularity or segment cells are rejected (SMG_ERR_ARG). */
 int smg_lset_read_rst(const char* path, smg_config* cfg, smg_rst_info* info, double* phi, float* normal,
                       int8_t* tag, double* phi_ext);
+/* An SDFGen signed-distance file (text: ni nj nk / origin / dx / ni*nj*nk values, i
+ * fastest), as Levelset_levelinfo::Load_SDF_file (lset_level_infor.cpp:1168-1240)
+ * reads it: 3D only, dx must equal the level-set cell size (within 1e-15), values of
+ * cells beyond the configured grid are skipped and cells the file does not cover stay
+ * zero, as do the margin packages.  Output: phi on the full-margin grid of
+ * smg_lset_build ((4 num_pkg + 8)^3 cells, x fastest), ready for smg_lset_build. */
+int smg_lset_read_sdf(const char* path, const smg_config* cfg, double* phi_ext);
 /* Writes a level set (one-cell-halo arrays of smg_lset_build and its statistics) in the
  * same layout, as Levelset::Save_lset_restart_file (level_set.cpp:1022-1094) and
  * Levelset_levelinfo::Write_level_set_rstfile (lset_level_infor.cpp:695-927) lay it

@@ -199,6 +199,45 @@ int smg_lset_read_rst(const char* path, smg_config* cfg, smg_rst_info* info, dou
   return SMG_OK;
 }
 
+int smg_lset_read_sdf(const char* path, const smg_config* cfg, double* phi_ext) {
+  using namespace smg;
+  if (!path || !cfg || !phi_ext) { set_error("smg_lset_read_sdf: NULL argument"); return SMG_ERR_ARG; }
+  if (cfg->dim != 3) { set_error("smg_lset_read_sdf: SDF files are 3D (Load_SDF_file rejects 2D)"); return SMG_ERR_ARG; }
+  FILE* fp = std::fopen(path, "r");
+  if (!fp) { set_error(std::string("smg_lset_read_sdf: cannot open ") + path); return SMG_ERR_ARG; }
+  FileGuard guard{fp};
+  long long g[3];
+  double origin[3], dx;
+  if (std::fscanf(fp, "%lld %lld %lld", &g[0], &g[1], &g[2]) != 3 ||
+      std::fscanf(fp, "%lf %lf %lf", &origin[0], &origin[1], &origin[2]) != 3 || std::fscanf(fp, "%lf", &dx) != 1) {
+    set_error("smg_lset_read_sdf: bad header (ni nj nk / origin / dx)");
+    return SMG_ERR_ARG;
+  }
+  if (g[0] <= 0 || g[1] <= 0 || g[2] <= 0) { set_error("smg_lset_read_sdf: bad grid size"); return SMG_ERR_ARG; }
+  int nc[3], ext[3];
+  double dl = 1.e20;
+  for (int a = 0; a < 3; a++) {
+    nc[a] = 4 * cfg->num_pkg[a];
+    ext[a] = nc[a] + 8;
+    dl = std::fmin(dl, cfg->domain[a] / double(cfg->num_pkg[a]) / 4.0);
+  }
+  if (std::fabs(dx - dl) > 1.e-15) {   // the reference aborts on this (lset_level_infor.cpp:1196-1199)
+    set_error("smg_lset_read_sdf: the file's dx differs from the level-set cell size");
+    return SMG_ERR_ARG;
+  }
+  // the margin packages start at zero, as Levelset_cell::Initialize leaves them
+  std::memset(phi_ext, 0, sizeof(double) * size_t(ext[0]) * ext[1] * ext[2]);
+  for (long long k = 0; k < g[2]; k++)
+    for (long long j = 0; j < g[1]; j++)
+      for (long long i = 0; i < g[0]; i++) {
+        double v;
+        if (std::fscanf(fp, "%lf", &v) != 1) { set_error("smg_lset_read_sdf: truncated values"); return SMG_ERR_ARG; }
+        if (i < nc[0] && j < nc[1] && k < nc[2])
+          phi_ext[((k + 4) * ext[1] + (j + 4)) * int64_t(ext[0]) + (i + 4)] = v;
+      }
+  return SMG_OK;
+}
+
 int smg_lset_write_rst(const char* path, const smg_config* cfg, const double* phi, const float* normal,
                        const int8_t* tag, const smg_lset_stats* st) {
   using namespace smg;

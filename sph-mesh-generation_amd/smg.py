@@ -230,6 +230,16 @@ def write_levelset_rst(path, cfg, lset):
                                 _ptr(lset["tag"]), C.byref(lset["stats"])), "smg_lset_write_rst")
 
 
+def read_levelset_sdf(path, cfg):
+    """phi on the full-margin grid from an SDFGen .sdf file (smg_lset_read_sdf)."""
+    L = lib()
+    L.smg_lset_read_sdf.argtypes = [C.c_char_p, C.POINTER(SmgConfig), C.c_void_p]
+    ext = [4 * cfg.num_pkg[a] + 8 if a < cfg.dim else 1 for a in range(3)]
+    phi_ext = np.zeros(ext[0] * ext[1] * ext[2])
+    _check(L.smg_lset_read_sdf(os.fsencode(path), C.byref(cfg), _ptr(phi_ext)), "smg_lset_read_sdf")
+    return phi_ext
+
+
 def read_levelset_rst(path):
     """(cfg, lset, info, phi_ext) from a levelset.rst (smg_lset_read_rst): lset holds
     the file's phi / normal / tag on the one-cell-halo grid and, as "stats", the

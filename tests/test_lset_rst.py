@@ -165,3 +165,53 @@ def test_bad_files_are_rejected(smg_mod, tmp_path):
         smg_mod.read_levelset_rst(str(trunc))
     with pytest.raises(smg_mod.SmgError):
         smg_mod.read_levelset_rst(str(tmp_path / "missing.rst"))
+
+
+def _write_sdf(path, phi, origin, dx):
+    """SDFGen's text layout: ni nj nk / origin / dx / values, i fastest."""
+    nk, nj, ni = phi.shape
+    with open(path, "w") as f:
+        f.write(f"{ni} {nj} {nk}\n{origin[0]} {origin[1]} {origin[2]}\n{dx!r}\n")
+        for v in phi.ravel():
+            f.write(f"{float(v)!r}\n")
+
+
+def test_sdf_file_fills_the_interior(smg_mod, tmp_path):
+    cfg, sdf = smg_mod.config_c2(scale=3.0, num_pkg=8)
+    nc = 4 * cfg.num_pkg[0]
+    dl = cfg.domain[0] / cfg.num_pkg[0] / 4.0
+    X, Y, Z = smg_mod.cell_centres_ext(cfg)
+    full = sdf(X, Y, Z)
+    interior = full[4:-4, 4:-4, 4:-4]
+    path = tmp_path / "case.sdf"
+    _write_sdf(path, interior, (0.0, 0.0, 0.0), dl)
+    phi_ext = smg_mod.read_levelset_sdf(str(path), cfg).reshape(full.shape)
+    assert np.array_equal(phi_ext[4:-4, 4:-4, 4:-4], interior)
+    margin = np.ones(full.shape, bool)
+    margin[4:-4, 4:-4, 4:-4] = False
+    assert np.all(phi_ext[margin] == 0.0)
+    # a file larger than the grid: the extra cells are skipped; a smaller one leaves zeros
+    big = np.pad(interior, ((0, 2), (0, 3), (0, 1)), constant_values=9.0)
+    _write_sdf(path, big, (0.0, 0.0, 0.0), dl)
+    assert np.array_equal(smg_mod.read_levelset_sdf(str(path), cfg).reshape(full.shape)[4:-4, 4:-4, 4:-4], interior)
+    _write_sdf(path, interior[:, :, : nc - 4], (0.0, 0.0, 0.0), dl)
+    part = smg_mod.read_levelset_sdf(str(path), cfg).reshape(full.shape)[4:-4, 4:-4, 4:-4]
+    assert np.array_equal(part[:, :, : nc - 4], interior[:, :, : nc - 4]) and np.all(part[:, :, nc - 4:] == 0)
+    # the level set built from it: the sphere hole's volume to the grid's accuracy
+    ls = smg_mod.lset_build(cfg, phi_ext, zero_margin=True)
+    assert ls["stats"].total_volume == pytest.approx(1.0 - 4.0 / 3.0 * np.pi * 0.3 ** 3, rel=0.05)
+
+
+def test_sdf_file_errors(smg_mod, tmp_path):
+    cfg, sdf = smg_mod.config_c2(scale=3.0, num_pkg=8)
+    path = tmp_path / "case.sdf"
+    _write_sdf(path, np.zeros((4, 4, 4)), (0.0, 0.0, 0.0), 0.5)   # dx differs from the cell size
+    with pytest.raises(smg_mod.SmgError):
+        smg_mod.read_levelset_sdf(str(path), cfg)
+    cfg2, _ = smg_mod.config_c1()
+    with pytest.raises(smg_mod.SmgError):
+        smg_mod.read_levelset_sdf(str(path), cfg2)   # 2D
+    dl = cfg.domain[0] / cfg.num_pkg[0] / 4.0
+    path.write_text(f"4 4 4\n0 0 0\n{dl!r}\n1.0\n2.0\n")   # truncated values
+    with pytest.raises(smg_mod.SmgError):
+        smg_mod.read_levelset_sdf(str(path), cfg)
