@@ -1636,7 +1636,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
 // P4: stage-2 (viscous) force, stage-2 dt and nu
 
 template <int D>
-__global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_force2(Work W, PState S, int par) {
   __shared__ double lds_d[4];
   __shared__ JumpTile jt;
   const int j = W.tile_job[tile_id(W)];
@@ -1800,7 +1800,28 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
           for (int u = 0; u < 4; u++)
             if ((nib >> u) & 1u) pair_rec(ev[u] & kJMask, ev[u] >> kJBits, true, qh[u], ql[u], qv[u]);
         }
-        for (int k = 64 * kMaskWords; k < cnt; k++) {
+        // entries past the masks: the inline ones four at a time (one 16-byte load per
+        // group, the records gathered together), then the spilled tail
+        const int kin = cnt < W.K ? cnt : W.K;
+        for (int g = 16 * kMaskWords; 4 * g < kin; g++) {
+          const uint4 gv = nb_group(W, slot, g);
+          const uint32_t ev[4] = {gv.x, gv.y, gv.z, gv.w};
+          float4 qh[4], ql[4], qv[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            // past the end a lane loads its own record and skips it
+            const int64_t q = 4 * g + u < kin ? int64_t(ev[u] & kJMask) : slot;
+            qh[u] = W.rec[4 * q + 0];
+            ql[u] = W.rec[4 * q + 1];
+            qv[u] = W.rec[4 * q + 2];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            if (4 * g + u >= kin || (__float_as_int(ql[u].w) & kJumpBit)) continue;
+            pair_rec(ev[u] & kJMask, ev[u] >> kJBits, false, qh[u], ql[u], qv[u]);
+          }
+        }
+        for (int k = kin > 64 * kMaskWords ? kin : 64 * kMaskWords; k < cnt; k++) {
           const uint32_t e = nb_entry(W, slot, k, off);
           if (__float_as_int(W.rec[4 * int64_t(e & kJMask) + 1].w) & kJumpBit) continue;
           pair(e & kJMask, e >> kJBits, false);
