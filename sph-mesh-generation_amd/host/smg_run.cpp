@@ -3,10 +3,13 @@
 // the unit square (2D) or cube (3D) minus a circle / sphere at the centre.
 //
 //   smg_run [--dim 2|3] [--outputs N] [--output-dt STEPS] [--dx DX] [--num-pkg P]
-//           [--radius R] [--csv FILE]
+//           [--radius R] [--csv FILE] [--mesh FILE]
 //
-// Prints one line per output round (steps, dt, residual, particle counts) and, with
-// --csv, writes the final positions and types by particle id.
+// Prints one line per output round (steps, dt, residual, particle counts); with
+// --csv, writes the final positions and types by particle id; with --mesh, runs the
+// mesh stage of Post_processing and writes its cells (triangles in 2D, tetrahedra in
+// 3D, one line of particle ids each) and, in 3D, the boundary triangles after a line
+// "surface N".
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -58,6 +61,7 @@ int main(int argc, char** argv) {
   int dim = 2, outputs = 5, num_pkg = 32;
   double output_dt = 100., dx = 0.01, radius = 0.2;
   const char* csv = nullptr;
+  const char* mesh = nullptr;
   for (int i = 1; i < argc; i++) {
     auto next = [&](const char* name) -> const char* {
       if (i + 1 >= argc) {
@@ -73,6 +77,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(argv[i], "--num-pkg")) num_pkg = std::atoi(next("--num-pkg"));
     else if (!std::strcmp(argv[i], "--radius")) radius = std::atof(next("--radius"));
     else if (!std::strcmp(argv[i], "--csv")) csv = next("--csv");
+    else if (!std::strcmp(argv[i], "--mesh")) mesh = next("--mesh");
     else {
       std::fprintf(stderr, "unknown argument %s\n", argv[i]);
       return 2;
@@ -109,6 +114,27 @@ int main(int argc, char** argv) {
     for (size_t i = 0; i < type.size(); i++)
       std::fprintf(f, "%zu,%.17g,%.17g,%.17g,%d\n", i, x[3 * i], x[3 * i + 1], x[3 * i + 2], type[i]);
     std::fclose(f);
+  }
+  if (mesh) {
+    const SPH_mesh_generation::Mesh m = solver.current_solver.Mesh_generation();
+    const size_t per = m.dim == 2 ? 3 : 4;
+    FILE* f = std::fopen(mesh, "w");
+    if (!f) {
+      std::fprintf(stderr, "cannot write %s\n", mesh);
+      return 1;
+    }
+    std::fprintf(f, "cells %zu\n", m.cells.size() / per);
+    for (size_t q = 0; q < m.cells.size(); q += per) {
+      for (size_t u = 0; u < per; u++) std::fprintf(f, u ? " %d" : "%d", m.cells[q + u]);
+      std::fprintf(f, "\n");
+    }
+    if (m.dim == 3) {
+      std::fprintf(f, "surface %zu\n", m.surface.size() / 3);
+      for (size_t q = 0; q < m.surface.size(); q += 3)
+        std::fprintf(f, "%d %d %d\n", m.surface[q], m.surface[q + 1], m.surface[q + 2]);
+    }
+    std::fclose(f);
+    std::printf("mesh: %zu cells, %zu boundary triangles\n", m.cells.size() / per, m.surface.size() / 3);
   }
   return 0;
 }

@@ -99,6 +99,46 @@ class SPH_mesh_generation {
     type.assign(total_num_particle, 0);
     check(smg_get_state(job_, x.data(), nullptr, type.data(), nullptr, nullptr), "smg_get_state");
   }
+  // The mesh stage of Post_processing (sph_mesh_generation.cpp:2409-2436) on the
+  // current particles.  2D: the triangles of Local_voronoi_diagram +
+  // Reconstruct_tris (3 ids per triangle, i > j > k).  3D: the tetrahedra of
+  // Reconstruct_tets after the deletion rule (4 ascending ids) and their boundary
+  // triangles.  delete_thres <= 0 takes the reference's tet_delete_thres (10).
+  struct Mesh {
+    int dim = 0;
+    std::vector<int32_t> cells;     // 3 (2D) or 4 (3D) particle ids per cell
+    std::vector<int32_t> surface;   // 3D boundary triangles, 3 ids each
+  };
+  Mesh Mesh_generation(double delete_thres = 0.) {
+    Mesh m;
+    m.dim = cfg_.dim;
+    int64_t nc = 0;
+    if (cfg_.dim == 2) {
+      smg_tri_stats ts{};
+      check(smg_triangulate(job_, nullptr, 0, &nc, &ts), "smg_triangulate");
+      m.cells.assign(3 * static_cast<size_t>(nc), 0);
+      check(smg_triangulate(job_, m.cells.data(), nc, &nc, &ts), "smg_triangulate");
+      return m;
+    }
+    smg_tet_stats ts{};
+    // a Delaunay tetrahedralisation has about 6.5 tetrahedra per point; a short
+    // buffer comes back as SMG_ERR_CAPACITY with the count, and the copy is redone
+    int64_t cap = 8 * total_num_particle + 16;
+    m.cells.assign(4 * static_cast<size_t>(cap), 0);
+    int rc = smg_tetrahedralize(job_, delete_thres, m.cells.data(), cap, &nc, &ts);
+    if (rc == SMG_ERR_CAPACITY && nc > cap) {
+      cap = nc;
+      m.cells.assign(4 * static_cast<size_t>(cap), 0);
+      rc = smg_tetrahedralize(job_, delete_thres, m.cells.data(), cap, &nc, &ts);
+    }
+    check(rc, "smg_tetrahedralize");
+    m.cells.resize(4 * static_cast<size_t>(nc));
+    int64_t ns = 0;
+    check(smg_tet_surface(job_, nullptr, 0, &ns), "smg_tet_surface");
+    m.surface.assign(3 * static_cast<size_t>(ns), 0);
+    check(smg_tet_surface(job_, m.surface.data(), ns, &ns), "smg_tet_surface");
+    return m;
+  }
   // Output_simulation_infor: step count, dt, residual, particle counts
   smg_stats Output_simulation_infor() {
     smg_stats st{};
