@@ -126,6 +126,14 @@ int smg_default_config(int dim, smg_config* cfg);
  * phi (double), normal (float, 3 per cell), tag (int8, 1 = CUT_CELL). */
 int smg_lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin,
                    double* phi, float* normal, int8_t* tag, smg_lset_stats* stats);
+/* smg_lset_build on the GPU (2D and 3D): the same arguments and outputs, computed by
+ * the same operation sequences, so phi, normal, tag, maximum_phi, volume, area and, for
+ * the uniform and linear spacing profiles, masses and particle counts are bit-identical
+ * to the host builder's; with the tanh profile the per-cell spacing uses the device tanh
+ * (within 2 ulp).  Replaces the same reference routines as smg_lset_build.  kernel_ms
+ * (may be NULL): device time of the build, transfers excluded.  No GPU: SMG_ERR_HIP. */
+int smg_lset_build_gpu(const smg_config* cfg, const double* phi_ext, int zero_margin,
+                       double* phi, float* normal, int8_t* tag, smg_lset_stats* stats, double* kernel_ms);
 
 /* The reference's level-set restart file (./rstfile/levelset.rst), the input of its
  * relaxation run: case header (SPH_mesh_generation::Load_rst_case,

@@ -139,9 +139,11 @@ double square_len(const double v[4], const double px[4], const double py[4]) {
       ex[ne] = px[a] + t * (px[b] - px[a]); ey[ne] = py[a] + t * (py[b] - py[a]); ne++;
     }
   }
+  // lengths as sqrt of the sum of squares (geometry_calculator.cpp:340-342)
+  auto len = [](double dx, double dy) { return std::sqrt(dx * dx + dy * dy); };
   double L = 0.;
-  if (ne == 2) L = std::hypot(ex[1] - ex[0], ey[1] - ey[0]);
-  else if (ne == 4) L = std::hypot(ex[1] - ex[0], ey[1] - ey[0]) + std::hypot(ex[3] - ex[2], ey[3] - ey[2]);
+  if (ne == 2) L = len(ex[1] - ex[0], ey[1] - ey[0]);
+  else if (ne == 4) L = len(ex[1] - ex[0], ey[1] - ey[0]) + len(ex[3] - ex[2], ey[3] - ey[2]);
   return L;
 }
 

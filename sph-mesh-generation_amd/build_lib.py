@@ -8,7 +8,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib", "libsmg.so")
 
-SOURCES = ["smg_kernels.hip", "smg_vor3d.hip", "smg_engine.cpp", "smg_api.cpp", "smg_lset.cpp", "smg_server.cpp",
+SOURCES = ["smg_kernels.hip", "smg_vor3d.hip", "smg_lset_dev.hip", "smg_engine.cpp", "smg_api.cpp", "smg_lset.cpp", "smg_server.cpp",
            "smg_tri.cpp", "smg_tet.cpp", "smg_rst.cpp"]
 
 HIPCC_FLAGS = [

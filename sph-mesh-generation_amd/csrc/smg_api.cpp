@@ -155,6 +155,13 @@ int smg_lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin
   return smg::lset_build(cfg, phi_ext, zero_margin, phi, normal, tag, stats);
 }
 
+int smg_lset_build_gpu(const smg_config* cfg, const double* phi_ext, int zero_margin, double* phi, float* normal,
+                       int8_t* tag, smg_lset_stats* stats, double* kernel_ms) {
+  if (!cfg) { smg::set_error("smg_lset_build_gpu: NULL config"); return SMG_ERR_ARG; }
+  if (!phi_ext || !phi || !normal || !tag) { smg::set_error("smg_lset_build_gpu: NULL buffer"); return SMG_ERR_ARG; }
+  return smg::lset_build_device(cfg, phi_ext, zero_margin, phi, normal, tag, stats, kernel_ms);
+}
+
 int smg_create(const smg_config* cfg, smg_job** job) {
   if (!job) { set_error("smg_create: job is NULL"); return SMG_ERR_ARG; }
   *job = nullptr;

@@ -35,6 +35,10 @@ struct SpacingProfile {
 
 int lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin, double* phi, float* normal,
                int8_t* tag, smg_lset_stats* st);
+// the same outputs computed on the current GPU (csrc/smg_lset_dev.hip); kernel_ms (may
+// be null) gets the device time from the first kernel to the last
+int lset_build_device(const smg_config* cfg, const double* phi_ext, int zero_margin, double* phi, float* normal,
+                      int8_t* tag, smg_lset_stats* st, double* kernel_ms);
 
 void set_error(const std::string& msg);
 

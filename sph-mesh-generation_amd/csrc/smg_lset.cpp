@@ -179,7 +179,12 @@ double isoline_length(const double v[4], const double p[4][2]) {
     if ((v[a] < 0.0) != (v[b] < 0.0)) lerp_zero(p[a], p[b], v[a], v[b], 2, e[m++]);
   }
   double L = 0.0;
-  for (int q = 0; q + 1 < m; q += 2) L += std::hypot(e[q + 1][0] - e[q][0], e[q + 1][1] - e[q][1]);
+  // segment length as the reference measures lengths (sqrt of the sum of squares,
+  // geometry_calculator.cpp:340-342), also what the GPU builder evaluates
+  for (int q = 0; q + 1 < m; q += 2) {
+    const double dx = e[q + 1][0] - e[q][0], dy = e[q + 1][1] - e[q][1];
+    L += std::sqrt(dx * dx + dy * dy);
+  }
   return L;
 }
 

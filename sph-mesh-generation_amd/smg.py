@@ -168,6 +168,26 @@ def lset_build(cfg, phi_ext, zero_margin=True):
     return {"phi": phi, "normal": normal, "tag": tag, "stats": st}
 
 
+def lset_build_gpu(cfg, phi_ext, zero_margin=True):
+    """lset_build on the GPU (smg_lset_build_gpu): the same dict, plus "kernel_ms",
+    the device time of the build (transfers excluded)."""
+    D = cfg.dim
+    halo = [4 * cfg.num_pkg[a] + 2 if a < D else 1 for a in range(3)]
+    ext_n = [4 * cfg.num_pkg[a] + 8 if a < D else 1 for a in range(3)]
+    m = halo[0] * halo[1] * halo[2]
+    ext = np.ascontiguousarray(phi_ext, np.float64).ravel()
+    if ext.size != ext_n[0] * ext_n[1] * ext_n[2]:
+        raise ValueError(f"phi_ext has {ext.size} values, the grid {ext_n} needs {ext_n[0] * ext_n[1] * ext_n[2]}")
+    phi = np.zeros(m, np.float64)
+    normal = np.zeros(3 * m, np.float32)
+    tag = np.zeros(m, np.int8)
+    st = SmgLsetStats()
+    ms = C.c_double()
+    _check(lib().smg_lset_build_gpu(C.byref(cfg), _ptr(ext), int(zero_margin), _ptr(phi), _ptr(normal), _ptr(tag),
+                                    C.byref(st), C.byref(ms)), "smg_lset_build_gpu")
+    return {"phi": phi, "normal": normal, "tag": tag, "stats": st, "kernel_ms": ms.value}
+
+
 def sdf_circle_hole(center, radius):
     """phi > 0 outside the circle / sphere (the fluid), SPH-mesh-generation's sign convention."""
     c = np.asarray(center, float)

@@ -58,10 +58,14 @@ def test_no_cpu_fallback_without_gpu(smg_mod, c1):
         "import sys; sys.path.insert(0, %r); import smg; cfg, sdf = smg.config_c1(num_pkg=8, dx=0.05); "
         "L = smg.make_levelset(cfg, sdf); j = smg.Job(cfg, L)\n"
         "try:\n    j.seed(50)\nexcept smg.SmgError as e:\n    print('RC', e.rc)\nelse:\n    print('RC', 0)\n"
+        "X, Y, Z = smg.cell_centres_ext(cfg)\n"
+        "try:\n    smg.lset_build_gpu(cfg, sdf(X, Y, Z))\nexcept smg.SmgError as e:\n    print('LS', e.rc)\n"
+        "else:\n    print('LS', 0)\n"
     ) % os.path.join(ROOT, "sph-mesh-generation_amd")
     env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1", CUDA_VISIBLE_DEVICES="-1")
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
     assert "RC -2" in out.stdout, out.stdout + out.stderr
+    assert "LS -2" in out.stdout, out.stdout + out.stderr
 
 
 def test_server_sizing_is_host_only(smg_mod):
@@ -94,6 +98,7 @@ def test_server_without_gpu_fails_loudly(smg_mod):
     env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1", CUDA_VISIBLE_DEVICES="-1")
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
     assert "RC -2" in out.stdout, out.stdout + out.stderr
+    assert "LS -2" in out.stdout, out.stdout + out.stderr
 
 
 def test_autotune_without_gpu_fails_loudly(smg_mod):
@@ -107,3 +112,4 @@ def test_autotune_without_gpu_fails_loudly(smg_mod):
     env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1", CUDA_VISIBLE_DEVICES="-1")
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
     assert "RC -2" in out.stdout, out.stdout + out.stderr
+    assert "LS -2" in out.stdout, out.stdout + out.stderr
