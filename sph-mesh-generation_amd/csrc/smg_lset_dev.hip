@@ -271,10 +271,32 @@ __device__ __forceinline__ void select_child(const double r[3][3][3], int ch, do
       for (int t = 0; t < 2; t++) out[4 * p + 2 * q + t] = ce ? ty[p][q][1 + t] : ty[p][q][t];
 }
 
+// corners of child ch of a cube (ch may differ between lanes): the same midpoints as
+// refine3, formed only for the child's quadrant.  Per x-plane of the parent: the z
+// midpoints of both rows, the y midpoints of both columns and the centre; the quadrant
+// (cb, ce) picks four of the nine; the x midpoints of the two planes' picks complete it.
 __device__ __forceinline__ void child3(const double* c, int ch, double* out) {
-  double r[3][3][3];
-  refine3(c, r);
-  select_child(r, ch, out);
+  const bool ca = ch & 4, cb = ch & 2, ce = ch & 1;
+  double v[2][2][2];   // [x plane][q][t]
+#pragma unroll
+  for (int x = 0; x < 2; x++) {
+    const double p00 = c[4 * x], p01 = c[4 * x + 1], p10 = c[4 * x + 2], p11 = c[4 * x + 3];
+    const double zm0 = 0.5 * (p00 + p01), zm1 = 0.5 * (p10 + p11);
+    const double ym0 = 0.5 * (p00 + p10), ym1 = 0.5 * (p01 + p11);
+    const double yzm = 0.5 * (zm0 + zm1);
+    v[x][0][0] = cb ? (ce ? yzm : ym0) : (ce ? zm0 : p00);
+    v[x][0][1] = cb ? (ce ? ym1 : yzm) : (ce ? p01 : zm0);
+    v[x][1][0] = cb ? (ce ? zm1 : p10) : (ce ? yzm : ym0);
+    v[x][1][1] = cb ? (ce ? p11 : zm1) : (ce ? ym1 : yzm);
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++)
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const double xm = 0.5 * (v[0][q][t] + v[1][q][t]);
+      out[2 * q + t] = ca ? xm : v[0][q][t];
+      out[4 + 2 * q + t] = ca ? v[1][q][t] : xm;
+    }
 }
 
 // value of a node with no further split: full, empty, or the leaf rule at depth > 5
@@ -315,56 +337,6 @@ __device__ __forceinline__ double leaves6(const double* c) {
     acc += v;
   }
   return acc;
-}
-
-// volume fraction of a split lev-2 node: depth first over levels 3..5 with the path
-// (ch3, ch4, ch5) and the open nodes c3, c4 in registers; every lane advances through
-// its own subtree.  The result is exact (dyadic terms).
-__device__ double subtree_from2(const double* c2) {
-  double c3[8], c4[8];
-  int L = 3, ch3 = 0, ch4 = 0, ch5 = 0;
-  double acc = 0.0;
-  while (true) {
-    double par[8], n[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) par[q] = L == 3 ? c2[q] : (L == 4 ? c3[q] : c4[q]);
-    const int ch = L == 3 ? ch3 : (L == 4 ? ch4 : ch5);
-    child3(par, ch, n);
-    const double w = L == 3 ? 0x1p-12 : (L == 4 ? 0x1p-15 : 0x1p-18);
-    double v;
-    if (node3(n, L, w, v)) {
-      acc += v;
-    } else if (L == 5) {
-      acc += leaves6(n);
-    } else {   // open the node
-      if (L == 3) {
-#pragma unroll
-        for (int q = 0; q < 8; q++) c3[q] = n[q];
-        ch4 = 0;
-      } else {
-#pragma unroll
-        for (int q = 0; q < 8; q++) c4[q] = n[q];
-        ch5 = 0;
-      }
-      L++;
-      continue;
-    }
-    // next sibling, closing finished nodes
-    bool fin = false;
-    while (true) {
-      if (L == 5) {
-        if (++ch5 < 8) break;
-        L = 4;
-      } else if (L == 4) {
-        if (++ch4 < 8) break;
-        L = 3;
-      } else {
-        fin = ++ch3 >= 8;
-        break;
-      }
-    }
-    if (fin) return acc;
-  }
 }
 
 // the lev-2 node of a cut cell at path = 64 sub-cube + 8 lev-1 child + lev-2 child
@@ -505,18 +477,110 @@ __global__ __launch_bounds__(64 * kCutWaves) void k_lset_cut3a(LGrid g, const in
   }
 }
 
-// 3D, second pass: one lane per queued lev-2 node; its subtree's volume fraction is
-// added to the cell's (exact, so the order of the additions does not matter)
+// 3D, second pass: one lane per queued lev-2 node, walking its subtree depth first over
+// levels 3..5 with the path (ch3, ch4, ch5) and the open nodes c3, c4 in registers.
+// Split lev-5 nodes are not expanded where they are found: they go to a per-wavefront
+// buffer in LDS, and whenever it holds 64 the whole wavefront expands 64 of them at once
+// (their 8 leaves each), adding each result to the owning lane's sum with an LDS
+// atomic.  All terms are exact dyadic numbers, so the order of the additions does not
+// matter; each lane's total goes to its cell with one global atomic.
+constexpr int kLeafBuf = 128;
+
 __global__ __launch_bounds__(256) void k_lset_cut3b(LGrid g, const Q2* __restrict__ queue, int64_t nq,
                                                     double* __restrict__ hsum) {
+  __shared__ double buf[4][kLeafBuf][8];
+  __shared__ int32_t own[4][kLeafBuf];
+  __shared__ double accl[4][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (int64_t(blockIdx.x) * blockDim.x + 64 * wave >= nq) return;   // whole wavefronts leave together
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= nq) return;
-  const Q2 e = queue[t];
-  const int i = e.cell % g.nc[0], j = (e.cell / g.nc[0]) % g.nc[1], k = e.cell / (g.nc[0] * g.nc[1]);
-  double C[3][3][3], c2[8];
-  cell_samples<3>(g, i, j, k, C);
-  node2_of(C, e.path, c2);
-  atomicAdd(&hsum[e.cell], subtree_from2(c2));
+  const bool live = t < nq;
+  const Q2 e = live ? queue[t] : Q2{0, 0};
+  double c2[8], c3[8], c4[8];
+  {
+    const int i = e.cell % g.nc[0], j = (e.cell / g.nc[0]) % g.nc[1], k = e.cell / (g.nc[0] * g.nc[1]);
+    double C[3][3][3];
+    cell_samples<3>(g, i, j, k, C);
+    node2_of(C, e.path, c2);
+  }
+  accl[wave][lane] = 0.0;
+  double acc = 0.0;
+  int L = 3, ch3 = 0, ch4 = 0, ch5 = 0;
+  bool active = live;
+  int nbuf = 0;   // the same in every lane (formed from ballots)
+  while (true) {
+    bool want = false;
+    double n[8];
+    if (active) {
+      double par[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) par[q] = L == 3 ? c2[q] : (L == 4 ? c3[q] : c4[q]);
+      child3(par, L == 3 ? ch3 : (L == 4 ? ch4 : ch5), n);
+      const double w = L == 3 ? 0x1p-12 : (L == 4 ? 0x1p-15 : 0x1p-18);
+      double v;
+      bool next = true;
+      if (node3(n, L, w, v)) {
+        acc += v;
+      } else if (L == 5) {
+        want = true;
+      } else {   // open the node
+        if (L == 3) {
+#pragma unroll
+          for (int q = 0; q < 8; q++) c3[q] = n[q];
+          ch4 = 0;
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; q++) c4[q] = n[q];
+          ch5 = 0;
+        }
+        L++;
+        next = false;
+      }
+      if (next) {   // next sibling, closing finished nodes
+        while (true) {
+          if (L == 5) {
+            if (++ch5 < 8) break;
+            L = 4;
+          } else if (L == 4) {
+            if (++ch4 < 8) break;
+            L = 3;
+          } else {
+            active = ++ch3 < 8;
+            break;
+          }
+        }
+      }
+    }
+    const unsigned long long m = __ballot(want);
+    if (want) {
+      const int slot = nbuf + __popcll(m & ((1ull << lane) - 1ull));
+#pragma unroll
+      for (int q = 0; q < 8; q++) buf[wave][slot][q] = n[q];
+      own[wave][slot] = lane;
+    }
+    nbuf += __popcll(m);
+    const bool done = __ballot(active) == 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    while (nbuf >= 64 || (done && nbuf > 0)) {
+      const int take = nbuf >= 64 ? 64 : nbuf;
+      const int base = nbuf - take;
+      if (lane < take) {
+        double c[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) c[q] = buf[wave][base + lane][q];
+        atomicAdd(&accl[wave][own[wave][base + lane]], leaves6(c));
+      }
+      nbuf = base;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (done) break;
+  }
+  acc += accl[wave][lane];
+  if (live) atomicAdd(&hsum[e.cell], acc);
 }
 
 // ---- 2D area fraction: sub-square corners c[2 a + b] ----
