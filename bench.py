@@ -321,6 +321,46 @@ def tet(args, smg):
     print(json.dumps(out), flush=True)
 
 
+def lset(args, smg):
+    """Level-set finishing of the 1M-particle workload's 256^3 level set (unit cube minus
+    a sphere): the GPU builder (smg_lset_build_gpu) against the host builder
+    (smg_lset_build, all host threads) on the same input; both give the same bits."""
+    cfg, sdf = smg.config_c2(scale=args.scale, num_pkg=args.num_pkg)
+    X, Y, Z = smg.cell_centres_ext(cfg)
+    ext = sdf(X, Y, Z)
+    del X, Y, Z
+    smg.lset_build_gpu(cfg, ext)   # first call: code-object load
+    ker, wall = [], []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        G = smg.lset_build_gpu(cfg, ext)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        ker.append(G["kernel_ms"])
+    ncell = 1
+    for a in range(3):
+        ncell *= 4 * cfg.num_pkg[a]
+    kms, wms = min(ker), min(wall)
+    st = G["stats"]
+    log(f"cells={ncell} cut={st.n_interface_cell} particles={st.num_particle} kernel_ms={kms:.2f} wall_ms={wms:.1f}")
+    out = {
+        "metric": "level-set cells/s (GPU level-set finishing: normals, cut tags, cut-cell volume and area, "
+                  "particle count)",
+        "value": ncell / (kms * 1e-3), "unit": "cells/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "3d unit cube minus sphere r=0.3, level set 256^3 cells (the 1M-particle case)",
+                   "cells": ncell, "cut_cells": int(st.n_interface_cell), "num_particle": int(st.num_particle)},
+        "kernel_ms": kms, "wall_ms_with_transfers": wms,
+    }
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        P = smg.lset_build(cfg, ext)
+        dt = time.perf_counter() - t0
+        same = all(np.array_equal(P[k], G[k]) for k in ("phi", "normal", "tag")) and all(
+            getattr(P["stats"], f) == getattr(st, f) for f, _ in smg.SmgLsetStats._fields_)
+        out["host_builder"] = {"ms": dt * 1e3, "threads": min(os.cpu_count() or 1, 32), "bit_identical": bool(same)}
+    print(json.dumps(out), flush=True)
+
+
 def serve_once(smg, specs, lam, duration, max_jobs, budget_gb, rng_seed, label, max_active=0):
     """Poisson arrivals at rate lam for `duration` seconds into a server; returns the
     completions inside the window."""
@@ -441,7 +481,7 @@ def copy(args, smg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--mode", choices=("relax", "batch", "serve", "c3", "copy", "tet"), default="relax")
+    ap.add_argument("--mode", choices=("relax", "batch", "serve", "c3", "copy", "tet", "lset"), default="relax")
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--scale", type=float, default=0.564)
@@ -464,7 +504,8 @@ def main():
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
     import smg
-    {"relax": relax, "batch": batch, "serve": serve, "c3": c3, "copy": copy, "tet": tet}[args.mode](args, smg)
+    {"relax": relax, "batch": batch, "serve": serve, "c3": c3, "copy": copy, "tet": tet,
+     "lset": lset}[args.mode](args, smg)
 
 
 if __name__ == "__main__":

@@ -96,10 +96,12 @@ __device__ bool cut_cell(const LGrid& g, int i, int j, int k) {
   const int s0 = sign3(at(g, i, j, k));
   if (s0 != 0) {   // all 3^D cells around of one sign: no corner can change sign
     const int zr = g.D > 2 ? 1 : 0, yr = g.D > 1 ? 1 : 0;
+    // every load issued up front (no short circuit: a chain of dependent loads)
     bool same = true;
     for (int c = -zr; c <= zr; c++)
       for (int b = -yr; b <= yr; b++)
-        for (int a = -1; a <= 1; a++) same = same && sign3(at(g, i + a, j + b, k + c)) == s0;
+#pragma unroll
+        for (int a = -1; a <= 1; a++) same &= sign3(at(g, i + a, j + b, k + c)) == s0;
     if (same) return false;
   }
   // the corner order of Get_phi_at_corners
@@ -113,17 +115,18 @@ __device__ bool cut_cell(const LGrid& g, int i, int j, int k) {
 
 __global__ __launch_bounds__(256) void k_lset_grid(LGrid g, int zero_margin, double* __restrict__ phi,
                                                    float* __restrict__ normal, int8_t* __restrict__ tag,
-                                                   int32_t* __restrict__ list, LCount* __restrict__ cnt) {
-  const int64_t nh = int64_t(g.halo[0]) * g.halo[1] * g.halo[2];
-  const int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool live = o < nh;
+                                                   int32_t* __restrict__ list, LCount* __restrict__ cnt,
+                                                   double* __restrict__ bmax) {
+  // block (64, 4): 64 cells along x, 4 rows of y; blockIdx.z = z
+  const int hx = int(blockIdx.x) * 64 + int(threadIdx.x & 63);
+  const int hy = int(blockIdx.y) * 4 + int(threadIdx.x >> 6);
+  const int hz = int(blockIdx.z);
+  const bool live = hx < g.halo[0] && hy < g.halo[1];
+  const int64_t o = (int64_t(hz) * g.halo[1] + hy) * g.halo[0] + hx;
   double pmax = 0.0;
   bool is_cut = false, listed = false;
   int32_t cell = 0;
   if (live) {
-    const int hx = int(o % g.halo[0]);
-    const int hy = int((o / g.halo[0]) % g.halo[1]);
-    const int hz = int(o / (int64_t(g.halo[0]) * g.halo[1]));
     const int id3[3] = {hx - 1, g.D > 1 ? hy - 1 : 0, g.D > 2 ? hz - 1 : 0};
     const int i = id3[0], j = id3[1], k = id3[2];
     const bool in = i >= 0 && i < g.nc[0] && (g.D < 2 || (j >= 0 && j < g.nc[1])) &&
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(256) void k_lset_grid(LGrid g, int zero_margin, dou
       bool central = true;
       if (!in) {   // margin package: one-sided at its own faces
         const int c = id3[a];
-        const int local = ((c % 4) + 4) % 4;
+        const int local = c & 3;   // place in the package, also for c < 0
         if (local == 0) { hi3[a] = c + 1; central = false; }
         else if (local == 3) { lo3[a] = c - 1; central = false; }
         else { lo3[a] = c - 1; hi3[a] = c + 1; }
@@ -161,19 +164,49 @@ __global__ __launch_bounds__(256) void k_lset_grid(LGrid g, int zero_margin, dou
       cell = int32_t((int64_t(k) * (g.D > 1 ? g.nc[1] : 1) + j) * g.nc[0] + i);
     }
   }
-  // wavefront-aggregated counters
+  // block-aggregated counters: one atomic per block with cut cells, and the block's
+  // maximum phi to bmax[block] (a global maximum over all blocks by atomics on one
+  // address would serialise)
+  __shared__ unsigned int s_cut[4], s_list[4];
+  __shared__ double s_max[4];
+  __shared__ unsigned long long s_base;
   const unsigned long long cut_mask = __ballot(is_cut);
   const unsigned long long list_mask = __ballot(listed);
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int off = 32; off > 0; off >>= 1) pmax = fmax(pmax, __shfl_xor(pmax, off));
-  unsigned long long base = 0;
   if (lane == 0) {
-    if (cut_mask) atomicAdd(&cnt->n_cut, (unsigned long long)__popcll(cut_mask));
-    if (list_mask) base = atomicAdd(&cnt->n_list, (unsigned long long)__popcll(list_mask));
-    if (pmax > 0.) atomicMax(&cnt->max_phi_bits, (unsigned long long)__double_as_longlong(pmax));
+    s_cut[wave] = (unsigned int)__popcll(cut_mask);
+    s_list[wave] = (unsigned int)__popcll(list_mask);
+    s_max[wave] = pmax;
   }
-  base = __shfl(base, 0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int nc = s_cut[0] + s_cut[1] + s_cut[2] + s_cut[3];
+    const unsigned int nl = s_list[0] + s_list[1] + s_list[2] + s_list[3];
+    if (nc) atomicAdd(&cnt->n_cut, (unsigned long long)nc);
+    s_base = nl ? atomicAdd(&cnt->n_list, (unsigned long long)nl) : 0ull;
+    bmax[(int64_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] =
+        fmax(fmax(s_max[0], s_max[1]), fmax(s_max[2], s_max[3]));
+  }
+  __syncthreads();
+  unsigned long long base = s_base;
+  for (int w = 0; w < wave; w++) base += s_list[w];
   if (listed) list[base + __popcll(list_mask & ((1ull << lane) - 1ull))] = cell;
+}
+
+// maximum of the blocks' maxima (one block)
+__global__ __launch_bounds__(1024) void k_lset_max(const double* __restrict__ bmax, int64_t nb,
+                                                   LCount* __restrict__ cnt) {
+  __shared__ double sm[16];
+  double m = 0.0;
+  for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) m = fmax(m, bmax[b]);
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < int(blockDim.x >> 6); w++) m = fmax(m, sm[w]);
+    cnt->max_phi_bits = (unsigned long long)__double_as_longlong(m);
+  }
 }
 
 // corner / edge / face / centre samples of a cut cell from its 3^D neighbourhood
@@ -391,17 +424,19 @@ constexpr int kCutWaves = 4;
 __constant__ int8_t kSubTet[6][4] = {{0, 1, 3, 7}, {0, 3, 2, 7}, {0, 2, 6, 7}, {0, 6, 4, 7}, {0, 4, 5, 7}, {0, 5, 1, 7}};
 
 // 3D, first pass: one wavefront per listed cut cell, a lane per lev-1 node (8 sub-cubes
-// x 8 children).  Levels 0-2 are evaluated here and split lev-2 nodes go to the queue.
-// Writes the partial volume fraction and the isosurface area (plain stores, so the pass
-// can be run again).
+// x 8 children).  Levels 0-2 are evaluated here and split lev-2 nodes go to the queue;
+// the block reserves its queue range with one atomic.  Writes the partial volume
+// fraction and the isosurface area (plain stores, so the pass can be run again).
 __global__ __launch_bounds__(64 * kCutWaves) void k_lset_cut3a(LGrid g, const int32_t* __restrict__ list,
                                                                int64_t nlist, double* __restrict__ hsum,
                                                                double* __restrict__ carea, Q2* __restrict__ queue,
                                                                int64_t qcap, unsigned long long* __restrict__ qn) {
-  const int64_t w = int64_t(blockIdx.x) * kCutWaves + (threadIdx.x >> 6);
-  if (w >= nlist) return;   // whole wavefronts leave together
-  const int lane = threadIdx.x & 63;
-  const int32_t cell = list[w];
+  __shared__ unsigned int s_tot[kCutWaves];
+  __shared__ unsigned long long s_base;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t w = int64_t(blockIdx.x) * kCutWaves + wave;
+  const bool live = w < nlist;   // a wavefront past the list works on cell 0 and writes nothing
+  const int32_t cell = live ? list[w] : 0;
   const int i = cell % g.nc[0], j = (cell / g.nc[0]) % g.nc[1], k = cell / (g.nc[0] * g.nc[1]);
   double C[3][3][3];
   cell_samples<3>(g, i, j, k, C);
@@ -414,9 +449,11 @@ __global__ __launch_bounds__(64 * kCutWaves) void k_lset_cut3a(LGrid g, const in
 #pragma unroll
       for (int e = 0; e < 3; e++) f += (C[a][b][e] > 0.0) - (C[a][b][e] < 0.0);
   double H = f == 27 ? 1.0 : 0.0, area = 0.0;
-  if (f != 27 && f != -27) {   // uniform over the wavefront
-    const int sc = lane >> 3, ch1 = lane & 7;
-    double s0[8], c1[8], v;
+  const int sc = lane >> 3, ch1 = lane & 7;
+  double c1[8];
+  int nopen = 0;
+  if (live && f != 27 && f != -27) {   // uniform over the wavefront
+    double s0[8], v;
     select_child(C, sc, s0);
     double part = 0.0;
     if (node3(s0, 0, 0x1p-3, v)) {
@@ -426,25 +463,12 @@ __global__ __launch_bounds__(64 * kCutWaves) void k_lset_cut3a(LGrid g, const in
       if (node3(c1, 1, 0x1p-6, v)) {
         part = v;
       } else {
-        double r1[3][3][3];
-        refine3(c1, r1);
 #pragma unroll 1
         for (int ch2 = 0; ch2 < 8; ch2++) {
           double c2[8];
-          select_child(r1, ch2, c2);
-          const bool open = !node3(c2, 2, 0x1p-9, v);
-          if (!open) part += v;
-          // wavefront-aggregated queue append
-          const unsigned long long m = __ballot(open);
-          unsigned long long base = 0;
-          if (lane == __ffsll((long long)m) - 1) base = atomicAdd(qn, (unsigned long long)__popcll(m));
-          base = __shfl(base, __ffsll((long long)m) - 1 < 0 ? 0 : __ffsll((long long)m) - 1);
-          if (open) {
-            const unsigned long long slot = base + __popcll(m & ((1ull << lane) - 1ull));
-            // past the capacity the entry is dropped; the host sees the count, grows
-            // the queue and runs this pass again
-            if (slot < (unsigned long long)qcap) queue[slot] = Q2{cell, (sc << 6) | (ch1 << 3) | ch2};
-          }
+          child3(c1, ch2, c2);
+          if (node3(c2, 2, 0x1p-9, v)) part += v;
+          else nopen++;
         }
       }
     }
@@ -471,7 +495,36 @@ __global__ __launch_bounds__(64 * kCutWaves) void k_lset_cut3a(LGrid g, const in
     }
     for (int u = 0; u < 48; u++) area += __shfl(at_l, u);
   }
-  if (lane == 0) {
+  // queue range of the block: inclusive scan of nopen over the wavefront, then the
+  // wavefronts' totals
+  int inc = nopen;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(inc, off);
+    if (lane >= off) inc += y;
+  }
+  if (lane == 63) s_tot[wave] = (unsigned int)inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (int q = 0; q < kCutWaves; q++) tot += s_tot[q];
+    s_base = tot ? atomicAdd(qn, tot) : 0ull;
+  }
+  __syncthreads();
+  if (nopen > 0) {
+    unsigned long long slot = s_base + (unsigned long long)(inc - nopen);
+    for (int q = 0; q < wave; q++) slot += s_tot[q];
+#pragma unroll 1
+    for (int ch2 = 0; ch2 < 8; ch2++) {
+      double c2[8], v;
+      child3(c1, ch2, c2);
+      if (node3(c2, 2, 0x1p-9, v)) continue;
+      // past the capacity the entry is dropped; the host sees the count, grows the
+      // queue and runs this pass again
+      if (slot < (unsigned long long)qcap) queue[slot] = Q2{cell, (sc << 6) | (ch1 << 3) | ch2};
+      slot++;
+    }
+  }
+  if (live && lane == 0) {
     hsum[cell] = H;
     carea[cell] = area;
   }
@@ -580,7 +633,19 @@ __global__ __launch_bounds__(256) void k_lset_cut3b(LGrid g, const Q2* __restric
     if (done) break;
   }
   acc += accl[wave][lane];
-  if (live) atomicAdd(&hsum[e.cell], acc);
+  // one atomic per cell present in the wavefront (a cell's entries are contiguous)
+  bool pending = live;
+  while (true) {
+    const unsigned long long m = __ballot(pending);
+    if (!m) break;
+    const int lead = __ffsll((long long)m) - 1;
+    const int32_t c = __shfl(e.cell, lead);
+    const bool mine = pending && e.cell == c;
+    double v = mine ? acc : 0.0;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == lead) atomicAdd(&hsum[c], v);
+    pending = pending && !mine;
+  }
 }
 
 // ---- 2D area fraction: sub-square corners c[2 a + b] ----
@@ -838,6 +903,9 @@ int lset_build_device(const smg_config* cfg, const double* phi_ext, int zero_mar
   float* d_nrm = nullptr;
   int8_t* d_tag = nullptr;
   int32_t* d_list = nullptr;
+  double* d_bmax = nullptr;
+  const dim3 grid_blocks(unsigned((g.halo[0] + 63) / 64), unsigned((g.halo[1] + 3) / 4), unsigned(g.halo[2]));
+  const int64_t nblk = int64_t(grid_blocks.x) * grid_blocks.y * grid_blocks.z;
   Q2* d_queue = nullptr;
   LCount* d_cnt = nullptr;
   LCount hc{};
@@ -859,12 +927,14 @@ int lset_build_device(const smg_config* cfg, const double* phi_ext, int zero_mar
   HIP_TRY_LS(hipMalloc(&d_carea, sizeof(double) * n_int));
   HIP_TRY_LS(hipMalloc(&d_psum, sizeof(double) * 4 * npkg));
   HIP_TRY_LS(hipMalloc(&d_cnt, sizeof(LCount)));
+  HIP_TRY_LS(hipMalloc(&d_bmax, sizeof(double) * nblk));
   HIP_TRY_LS(hipMemcpyAsync(d_ext, phi_ext, sizeof(double) * n_ext, hipMemcpyHostToDevice, s));
   HIP_TRY_LS(hipMemsetAsync(d_cnt, 0, sizeof(LCount), s));
   g.ext = d_ext;
   HIP_TRY_LS(hipEventRecord(e0, s));
-  k_lset_grid<<<dim3(unsigned((n_halo + 255) / 256)), dim3(256), 0, s>>>(g, zero_margin, d_phi, d_nrm, d_tag, d_list,
-                                                                        d_cnt);
+  k_lset_grid<<<grid_blocks, dim3(256), 0, s>>>(g, zero_margin, d_phi, d_nrm, d_tag, d_list, d_cnt, d_bmax);
+  HIP_TRY_LS(hipGetLastError());
+  k_lset_max<<<dim3(1), dim3(1024), 0, s>>>(d_bmax, nblk, d_cnt);
   HIP_TRY_LS(hipGetLastError());
   HIP_TRY_LS(hipMemcpyAsync(&hc, d_cnt, sizeof(LCount), hipMemcpyDeviceToHost, s));
   HIP_TRY_LS(hipStreamSynchronize(s));
@@ -936,7 +1006,7 @@ int lset_build_device(const smg_config* cfg, const double* phi_ext, int zero_mar
 done:
   if (s) (void)hipStreamSynchronize(s);
   for (void* ptr : {(void*)d_ext, (void*)d_phi, (void*)d_nrm, (void*)d_tag, (void*)d_list, (void*)d_cvol,
-                    (void*)d_carea, (void*)d_psum, (void*)d_cnt, (void*)d_queue})
+                    (void*)d_carea, (void*)d_psum, (void*)d_cnt, (void*)d_queue, (void*)d_bmax})
     if (ptr) (void)hipFree(ptr);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
