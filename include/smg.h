@@ -238,6 +238,10 @@ int smg_get_pair_terms(smg_job* job, float* sum, float* a1, float* a2);
  * (Level_info::Construct_symmetric_BC_particles, level_infor.cpp:786-856).
  * With ids == NULL only offs is filled (to size the buffers). */
 int smg_get_neighbors(smg_job* job, int64_t* offs, int32_t* ids, int32_t* codes, int64_t capacity);
+/* The level-set normal interpolated at every particle's current position (3 per
+ * particle, by id; not renormalised): the particle's norm that the reference's writers
+ * print (particle_binary_writer.cpp, "norm"). */
+int smg_get_normals(smg_job* job, float* normal);
 
 /* Kernel timing: average device time in ms of each kernel over the launches since
  * profiling was last enabled (enabling starts a new window), measured with HIP events
@@ -453,6 +457,44 @@ int smg_server_usage(smg_server* server, int64_t* used_slots, int64_t* total_slo
  * floor(a/b) and tanh(a) for n inputs (4 outputs per input), to check that the FP64
  * operations the bit-exact paths use round as the host's do. */
 int smg_fp64_probe(int64_t n, const double* a, const double* b, double* out);
+
+/* ---- Output files (host only), single rank --------------------------------------- */
+
+/* Points of an output file: positions and the point data the reference writes. */
+typedef struct smg_vtu_points {
+  int64_t n;
+  const double* x;        /* 3 per point (required when n > 0)        */
+  const int32_t* type;    /* per point; NULL writes zeros              */
+  const double* h;        /* NULL writes zeros                         */
+  const int32_t* nn;      /* neighbour counts; NULL writes zeros       */
+  const int32_t* color;   /* rank colour; NULL writes zeros            */
+  const float* normal;    /* 3 per point; NULL writes zeros            */
+} smg_vtu_points;
+
+/* Particle_binary_writer::Output_particle (particle_binary_writer.cpp:36-401): in dir,
+ * particle_rank_0_step_<n_out>.vtu (VTK XML, raw appended data; point data id, local_id,
+ * type, h, NN, color, norm; one vertex cell per point), particle_step_<n_out>.pvtu, and
+ * particle.pvd, created at n_out = 0 and extended by one DataSet (timestep = time) at
+ * every later n_out. */
+int smg_write_particles_vtu(const char* dir, int n_out, double time, const smg_vtu_points* points);
+/* Mesh_binary_writer::Output_mesh (mesh_binary_writer.cpp:36-650): 2D triangles
+ * (3 ids each) to mesh_tris_rank_0_step_<n_out>.vtu, or 3D tetrahedra (4 ids) to
+ * mesh_tets_... with cell data radius_ratio and mindihedangle; point data type, h,
+ * color; plus the .pvtu and the mesh_tris.pvd / mesh_tets.pvd collection. */
+int smg_write_mesh_vtu(const char* dir, int n_out, double time, int dim, const smg_vtu_points* points,
+                       const int32_t* cells, int64_t n_cells);
+/* SPH_mesh_generation::Save_particle_post_file (sph_mesh_generation.cpp:2624-2667):
+ * dir/step_<step>_rank_0.pst (int32 count, 3 doubles per particle) and a line
+ * "step count" in dir/list.csv, which first != 0 starts with the number of ranks (1). */
+int smg_write_post_file(const char* dir, int64_t step, int64_t n, const double* x, int first);
+/* Load_data_for_post (sph_mesh_generation.cpp:426-590), its list: the number of ranks
+ * and the saved steps with their particle counts (steps/counts may be NULL to query). */
+int smg_read_post_list(const char* dir, int32_t* n_ranks, int64_t* steps, int64_t* counts, int64_t capacity,
+                       int64_t* n_steps);
+/* ... and one step's particles from all rank files, in rank order; color (may be NULL)
+ * gets each particle's rank.  x may be NULL to query the count. */
+int smg_read_post_file(const char* dir, int64_t step, int32_t n_ranks, double* x, int32_t* color, int64_t capacity,
+                       int64_t* n);
 
 #ifdef __cplusplus
 }

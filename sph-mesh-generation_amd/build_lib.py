@@ -9,7 +9,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib", "libsmg.so")
 
 SOURCES = ["smg_kernels.hip", "smg_vor3d.hip", "smg_lset_dev.hip", "smg_engine.cpp", "smg_api.cpp", "smg_lset.cpp", "smg_server.cpp",
-           "smg_tri.cpp", "smg_tet.cpp", "smg_rst.cpp"]
+           "smg_tri.cpp", "smg_tet.cpp", "smg_rst.cpp", "smg_io.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -277,6 +277,12 @@ int smg_get_neighbors(smg_job* job, int64_t* offs, int32_t* ids, int32_t* codes,
   return job->eng->get_neighbors(0, offs, ids, codes, capacity);
 }
 
+int smg_get_normals(smg_job* job, float* normal) {
+  if (!job || !normal) { set_error("smg_get_normals: bad argument"); return SMG_ERR_ARG; }
+  if (!job->eng) { set_error("smg_get_normals: seed or initialise the job first"); return SMG_ERR_STATE; }
+  return job->eng->get_normals(0, normal);
+}
+
 int smg_sample_phi(smg_job* job, int64_t n, const double* pts, double* phi) {
   if (!job || n < 0 || (n > 0 && (!pts || !phi))) { set_error("smg_sample_phi: bad argument"); return SMG_ERR_ARG; }
   if (!job->eng) { set_error("smg_sample_phi: seed or initialise the job first"); return SMG_ERR_STATE; }

@@ -906,6 +906,33 @@ static double bits_to_double(unsigned long long b) {
   return d;
 }
 
+int Engine::get_normals(int job, float* nrm) {
+  // the level-set normal interpolated at every particle's current position (the
+  // particle's norm of the reference's Get_particle_info; the relaxation itself samples
+  // it only in cut cells)
+  const int64_t n = jp_[job].n;
+  if (n == 0) return SMG_OK;
+  std::vector<double> x(3 * size_t(n)), nd(3 * size_t(n));
+  SMG_TRY(get_state(job, x.data(), nullptr, nullptr, nullptr, nullptr));
+  double *d_pts = nullptr, *d_phi = nullptr, *d_n = nullptr;
+  hipError_t e = hipMalloc(&d_pts, sizeof(double) * 3 * n);
+  if (e == hipSuccess) e = hipMalloc(&d_phi, sizeof(double) * n);
+  if (e == hipSuccess) e = hipMalloc(&d_n, sizeof(double) * 3 * n);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_pts, x.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st_);
+  if (e == hipSuccess) {
+    launch_sample_phi(st_, D_, W_, job, n, d_pts, d_phi, d_n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(nd.data(), d_n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st_);
+  if (e == hipSuccess) e = hipStreamSynchronize(st_);
+  (void)hipFree(d_pts);
+  (void)hipFree(d_phi);
+  (void)hipFree(d_n);
+  if (e != hipSuccess) { set_error(std::string("get_normals: ") + hipGetErrorString(e)); return SMG_ERR_HIP; }
+  for (size_t q = 0; q < nd.size(); q++) nrm[q] = float(nd[q]);
+  return SMG_OK;
+}
+
 int Engine::get_state(int job, double* x, double* v, int32_t* type, double* h, float* nu) {
   const JobParams& J = jp_[job];
   const int64_t cap = J.cap, n = J.n;

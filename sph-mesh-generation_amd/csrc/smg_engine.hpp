@@ -98,6 +98,7 @@ class Engine {
   // results (host, by particle id of the job)
   int get_state(int job, double* x, double* v, int32_t* type, double* h, float* nu);
   int get_pair_terms(int job, float* sum, float* a1, float* a2);
+  int get_normals(int job, float* nrm);   // 3 per particle, by id
   int get_neighbors(int job, int64_t* offs, int32_t* ids, int32_t* codes, int64_t cap);
   int get_stats(int job, smg_stats* st);
   int timestep(int job, double* dt);

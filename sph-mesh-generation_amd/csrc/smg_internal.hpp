@@ -40,6 +40,10 @@ int lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin, do
 int lset_build_device(const smg_config* cfg, const double* phi_ext, int zero_margin, double* phi, float* normal,
                       int8_t* tag, smg_lset_stats* st, double* kernel_ms);
 
+// per-tetrahedron radius ratio (3 r_in / r_circ) and smallest dihedral angle in the
+// reference's degrees (csrc/smg_tet.cpp); tets: 4 vertex indices into x (3 per vertex)
+void tet_cell_quality(const double* x, const int32_t* tets, int64_t n, float* radius_ratio, float* min_dihedral);
+
 void set_error(const std::string& msg);
 
 }  // namespace smg

@@ -1967,13 +1967,18 @@ __global__ __launch_bounds__(kTile) void k_vor2d(Work W, PState S, int32_t* out_
 
 // FP64 level-set phi at arbitrary points of one job (Calculate_particle_infor)
 template <int D>
-__global__ void k_sample_phi(Work W, int job, int64_t n, const double* pts, double* phi) {
+__global__ void k_sample_phi(Work W, int job, int64_t n, const double* pts, double* phi, double* nrm) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const JobParams& J = W.jp[job];
   const double x[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
-  double p, nn[3];
-  ls_sample<D, false>(J, W, x, p, nn);
+  double p, nn[3] = {0., 0., 0.};
+  if (nrm) {   // and the interpolated normal (Get_normal), not renormalised
+    ls_sample<D, true>(J, W, x, p, nn);
+    for (int a = 0; a < 3; a++) nrm[3 * i + a] = nn[a];
+  } else {
+    ls_sample<D, false>(J, W, x, p, nn);
+  }
   phi[i] = p;
 }
 
@@ -2121,11 +2126,12 @@ void launch_vor2d(hipStream_t st, int64_t ntiles, const Work& W, const PState& S
                   int KE) {
   k_vor2d<2><<<ntiles, kTile, 0, st>>>(W, S, out_nb, out_cnt, KE);
 }
-void launch_sample_phi(hipStream_t st, int D, const Work& W, int job, int64_t n, const double* pts, double* phi) {
+void launch_sample_phi(hipStream_t st, int D, const Work& W, int job, int64_t n, const double* pts, double* phi,
+                       double* nrm) {
   if (n <= 0) return;
   const int64_t g = (n + 255) / 256;
-  if (D == 2) k_sample_phi<2><<<g, 256, 0, st>>>(W, job, n, pts, phi);
-  else k_sample_phi<3><<<g, 256, 0, st>>>(W, job, n, pts, phi);
+  if (D == 2) k_sample_phi<2><<<g, 256, 0, st>>>(W, job, n, pts, phi, nrm);
+  else k_sample_phi<3><<<g, 256, 0, st>>>(W, job, n, pts, phi, nrm);
 }
 void launch_fp64_probe(hipStream_t st, int64_t n, const double* a, const double* b, double* out) {
   k_fp64_probe<<<(n + 255) / 256, 256, 0, st>>>(n, a, b, out);

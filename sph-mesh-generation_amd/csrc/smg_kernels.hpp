@@ -30,7 +30,9 @@ void launch_force2(hipStream_t st, int D, int64_t ntiles, const Work& W, const P
 void launch_residual(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int njobs, int par);
 void launch_vor2d(hipStream_t st, int64_t ntiles, const Work& W, const PState& S, int32_t* out_nb, int32_t* out_cnt,
                   int KE);
-void launch_sample_phi(hipStream_t st, int D, const Work& W, int job, int64_t n, const double* pts, double* phi);
+// phi (and, when nrm is not null, the interpolated normal, 3 per point) at points
+void launch_sample_phi(hipStream_t st, int D, const Work& W, int job, int64_t n, const double* pts, double* phi,
+                       double* nrm = nullptr);
 
 // 3D Voronoi cells -> Delaunay tetrahedra (smg_vor3d.hip)
 struct V3Grid {

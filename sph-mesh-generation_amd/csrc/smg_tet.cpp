@@ -167,6 +167,18 @@ void quality(const std::vector<Tet>& tets, const double* x, smg_tet_stats* st) {
 }  // namespace
 
 // deletion rule, boundary triangles and statistics of given tetrahedra (host only)
+void tet_cell_quality(const double* x, const int32_t* tets, int64_t n, float* radius_ratio, float* min_dihedral) {
+  for (int64_t q = 0; q < n; q++) {
+    const int32_t* t = tets + 4 * q;
+    const TetGeo g = tet_geometry(&x[3 * size_t(t[0])], &x[3 * size_t(t[1])], &x[3 * size_t(t[2])],
+                                  &x[3 * size_t(t[3])]);
+    double dmin = 180.;
+    for (double d : g.dih) dmin = std::min(dmin, d);
+    radius_ratio[q] = float(g.radius_ratio);
+    min_dihedral[q] = float(dmin);
+  }
+}
+
 int tet_finish_host(int64_t nv, const double* x, const int32_t* type, const std::vector<Tet>& in, double thres,
                     std::vector<Tet>& kept, std::vector<Tri3>& surf, smg_tet_stats* st) {
   if (!(thres > 0.)) thres = kDefaultDeleteThres;

@@ -511,6 +511,12 @@ class Job:
         _check(lib().smg_get_neighbors(self.h, _ptr(offs), _ptr(ids), _ptr(codes), m), "smg_get_neighbors")
         return offs, ids[:m], codes[:m]
 
+    def normals(self):
+        """The particles' interpolated level-set normals (n x 3, float32, by id)."""
+        out = np.zeros((self.n, 3), np.float32)
+        _check(lib().smg_get_normals(self.h, _ptr(out)), "smg_get_normals")
+        return out
+
     def sample_phi(self, pts):
         """Level-set phi at points (n x 3) as the relaxation samples it."""
         p = np.zeros((len(pts), 3))
@@ -935,3 +941,122 @@ def batched_jobs(njobs, n_min, n_max, seed=20261203, dims=(2, 3), max_pkg=None):
         n_t = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
         jobs.append(union_job(dim, n_t, rng, seed + 7919 * (j + 1), max_pkg=max_pkg))
     return jobs
+
+
+# ---- output files (host only) --------------------------------------------------------
+
+class SmgVtuPoints(C.Structure):
+    _fields_ = [("n", C.c_int64), ("x", C.c_void_p), ("type", C.c_void_p), ("h", C.c_void_p), ("nn", C.c_void_p),
+                ("color", C.c_void_p), ("normal", C.c_void_p)]
+
+
+def _vtu_points(x, type=None, h=None, nn=None, color=None, normal=None):
+    keep = []
+
+    def arr(a, dt, per):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dt).reshape(-1)
+        if a.size != per * len(x):
+            raise ValueError("array length does not match the points")
+        keep.append(a)
+        return _ptr(a)
+    x = np.ascontiguousarray(x, np.float64)
+    if x.ndim != 2 or x.shape[1] not in (2, 3):
+        raise ValueError("x must be n x 2 or n x 3")
+    if x.shape[1] == 2:
+        x = np.hstack([x, np.zeros((len(x), 1))])
+    keep.append(x)
+    p = SmgVtuPoints(len(x), _ptr(x), arr(type, np.int32, 1), arr(h, np.float64, 1), arr(nn, np.int32, 1),
+                     arr(color, np.int32, 1), arr(normal, np.float32, 3))
+    return p, keep
+
+
+def write_particles_vtu(dir, n_out, time, x, type=None, h=None, nn=None, color=None, normal=None):
+    """Particle_binary_writer::Output_particle: particle_rank_0_step_<n_out>.vtu, .pvtu, particle.pvd."""
+    L = lib()
+    L.smg_write_particles_vtu.argtypes = [C.c_char_p, C.c_int, C.c_double, C.POINTER(SmgVtuPoints)]
+    p, keep = _vtu_points(x, type, h, nn, color, normal)
+    _check(L.smg_write_particles_vtu(str(dir).encode(), int(n_out), float(time), C.byref(p)), "smg_write_particles_vtu")
+
+
+def write_mesh_vtu(dir, n_out, time, dim, x, cells, type=None, h=None, color=None):
+    """Mesh_binary_writer::Output_mesh: mesh_tris_* (dim 2) or mesh_tets_* (dim 3) .vtu, .pvtu, .pvd."""
+    L = lib()
+    L.smg_write_mesh_vtu.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_int, C.POINTER(SmgVtuPoints), C.c_void_p,
+                                     C.c_int64]
+    p, keep = _vtu_points(x, type, h, None, color, None)
+    c = np.ascontiguousarray(cells, np.int32).reshape(-1, 3 if dim == 2 else 4)
+    _check(L.smg_write_mesh_vtu(str(dir).encode(), int(n_out), float(time), int(dim), C.byref(p), _ptr(c), len(c)),
+           "smg_write_mesh_vtu")
+
+
+def write_job_output(job, dir, n_out, time=None, mesh=None):
+    """The reference's per-output files for a job: particles (positions, type, h, neighbour
+    counts, normals) and, when mesh = (dim, cells) is given, the mesh."""
+    st = job.state()
+    offs, _, _ = job.neighbors()
+    t = float(job.stats()["step"]) if time is None else time
+    write_particles_vtu(dir, n_out, t, st["x"], st["type"], st["h"], np.diff(offs).astype(np.int32), None,
+                        job.normals())
+    if mesh is not None:
+        write_mesh_vtu(dir, n_out, t, mesh[0], st["x"], mesh[1], st["type"], st["h"])
+
+
+def write_post_file(dir, step, x, first):
+    """Save_particle_post_file: step_<step>_rank_0.pst and a list.csv line."""
+    L = lib()
+    L.smg_write_post_file.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int]
+    x3 = np.zeros((len(x), 3))
+    x3[:, :np.asarray(x).shape[1]] = x
+    _check(L.smg_write_post_file(str(dir).encode(), int(step), len(x3), _ptr(x3), int(bool(first))),
+           "smg_write_post_file")
+
+
+def read_post_list(dir):
+    """(number of ranks, [(step, count)]) from list.csv."""
+    L = lib()
+    L.smg_read_post_list.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_int64,
+                                     C.POINTER(C.c_int64)]
+    r, k = C.c_int32(), C.c_int64()
+    _check(L.smg_read_post_list(str(dir).encode(), C.byref(r), None, None, 0, C.byref(k)), "smg_read_post_list")
+    steps = np.zeros(max(k.value, 1), np.int64)
+    counts = np.zeros(max(k.value, 1), np.int64)
+    _check(L.smg_read_post_list(str(dir).encode(), C.byref(r), _ptr(steps), _ptr(counts), len(steps), C.byref(k)),
+           "smg_read_post_list")
+    return r.value, list(zip(steps[:k.value].tolist(), counts[:k.value].tolist()))
+
+
+def read_post_file(dir, step, n_ranks):
+    """(positions n x 3, rank colour per particle) of one saved step, all ranks."""
+    L = lib()
+    L.smg_read_post_file.argtypes = [C.c_char_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
+                                     C.POINTER(C.c_int64)]
+    n = C.c_int64()
+    _check(L.smg_read_post_file(str(dir).encode(), int(step), int(n_ranks), None, None, 0, C.byref(n)),
+           "smg_read_post_file")
+    x = np.zeros((max(n.value, 1), 3))
+    color = np.zeros(max(n.value, 1), np.int32)
+    _check(L.smg_read_post_file(str(dir).encode(), int(step), int(n_ranks), _ptr(x), _ptr(color), n.value,
+                                C.byref(n)), "smg_read_post_file")
+    return x[:n.value], color[:n.value]
+
+
+def read_vtu(path):
+    """A VTK XML file with raw appended data as written here: {array name: values}
+    (shaped n x components) plus "_points" / "_cells" counts."""
+    import re
+    raw = open(path, "rb").read()
+    head, _, rest = raw.partition(b"<AppendedData encoding=\"raw\">")
+    data = rest[rest.index(b"_") + 1:]
+    text = head.decode()
+    m = re.search(r'NumberOfPoints="(\d+)" NumberOfCells="(\d+)"', text)
+    out = {"_points": int(m.group(1)), "_cells": int(m.group(2))}
+    types = {"Int32": np.int32, "Float32": np.float32, "UInt8": np.uint8}
+    for a in re.finditer(r'<DataArray type="(\w+)" Name="(\w+)"(?: NumberOfComponents="(\d+)")? '
+                         r'format="appended" offset="(\d+)"/>', text):
+        dt, name, comps, off = types[a.group(1)], a.group(2), int(a.group(3) or 1), int(a.group(4))
+        nbytes = int(np.frombuffer(data[off:off + 4], np.uint32)[0])
+        v = np.frombuffer(data[off + 4:off + 4 + nbytes], dt)
+        out[name] = v.reshape(-1, comps) if comps > 1 else v
+    return out

@@ -1,0 +1,50 @@
+"""Output files of a relaxed job: the particle VTU carries the job's positions, types,
+h, neighbour counts and interpolated normals; the mesh VTU its triangles (2D) or
+tetrahedra with their quality (3D)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_job_output_2d(smg_mod, c1, tmp_path):
+    cfg, lset = c1
+    job = smg_mod.Job(cfg, lset)
+    job.seed(int(lset["stats"].num_particle))
+    job.step(20)
+    tris, _ = smg_mod.triangulate(job)
+    smg_mod.write_job_output(job, tmp_path, 0, time=20.0, mesh=(2, tris))
+    st = job.state()
+    offs, _, _ = job.neighbors()
+    nrm = job.normals()
+    v = smg_mod.read_vtu(tmp_path / "particle_rank_0_step_0.vtu")
+    assert v["_points"] == job.n
+    assert np.array_equal(v["Position"], st["x"].astype(np.float32))
+    assert np.array_equal(v["type"], st["type"]) and np.array_equal(v["h"], st["h"].astype(np.float32))
+    assert np.array_equal(v["NN"], np.diff(offs)) and np.array_equal(v["norm"], nrm)
+    # particles near the hole (radius 0.2): the interpolated normal points away from its
+    # centre
+    d = np.linalg.norm(st["x"][:, :2] - 0.5, axis=1)
+    near = np.abs(d - 0.2) < 0.02
+    assert near.sum() > 50
+    r = (st["x"][near, :2] - 0.5) / d[near, None]
+    assert np.all(np.sum(nrm[near, :2] * r, axis=1) > 0.9)
+    m = smg_mod.read_vtu(tmp_path / "mesh_tris_rank_0_step_0.vtu")
+    assert m["_cells"] == len(tris) and np.array_equal(m["connectivity"].reshape(-1, 3), tris)
+    job.close()
+
+
+def test_job_mesh_output_3d(smg_mod, tmp_path):
+    cfg, sdf = smg_mod.config_c2(scale=2.0, num_pkg=16)
+    lset = smg_mod.make_levelset(cfg, sdf)
+    job = smg_mod.Job(cfg, lset)
+    job.seed(int(lset["stats"].num_particle))
+    job.step(20)
+    tets, st = smg_mod.tetrahedralize(job)
+    smg_mod.write_job_output(job, tmp_path, 0, mesh=(3, tets))
+    m = smg_mod.read_vtu(tmp_path / "mesh_tets_rank_0_step_0.vtu")
+    assert m["_cells"] == len(tets) == st["n_tet"]
+    assert np.all((m["radius_ratio"] > 0) & (m["radius_ratio"] <= 1.0 + 1e-6))
+    assert np.all((m["mindihedangle"] > 0) & (m["mindihedangle"] < 180))
+    assert float(m["mindihedangle"].min()) == pytest.approx(st["min_dihedral"], rel=1e-5)
+    job.close()
