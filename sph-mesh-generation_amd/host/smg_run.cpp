@@ -3,13 +3,15 @@
 // the unit square (2D) or cube (3D) minus a circle / sphere at the centre.
 //
 //   smg_run [--dim 2|3] [--outputs N] [--output-dt STEPS] [--dx DX] [--num-pkg P]
-//           [--radius R] [--csv FILE] [--mesh FILE]
+//           [--radius R] [--csv FILE] [--mesh FILE] [--outdir DIR]
 //
 // Prints one line per output round (steps, dt, residual, particle counts); with
 // --csv, writes the final positions and types by particle id; with --mesh, runs the
 // mesh stage of Post_processing and writes its cells (triangles in 2D, tetrahedra in
 // 3D, one line of particle ids each) and, in 3D, the boundary triangles after a line
-// "surface N".
+// "surface N"; with --outdir, writes the reference's output files there after every
+// output round (particle VTU/PVTU/PVD, .pst post file and list.csv) and, with --mesh,
+// the mesh VTU of the final state.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -62,6 +64,7 @@ int main(int argc, char** argv) {
   double output_dt = 100., dx = 0.01, radius = 0.2;
   const char* csv = nullptr;
   const char* mesh = nullptr;
+  const char* outdir = nullptr;
   for (int i = 1; i < argc; i++) {
     auto next = [&](const char* name) -> const char* {
       if (i + 1 >= argc) {
@@ -78,6 +81,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(argv[i], "--radius")) radius = std::atof(next("--radius"));
     else if (!std::strcmp(argv[i], "--csv")) csv = next("--csv");
     else if (!std::strcmp(argv[i], "--mesh")) mesh = next("--mesh");
+    else if (!std::strcmp(argv[i], "--outdir")) outdir = next("--outdir");
     else {
       std::fprintf(stderr, "unknown argument %s\n", argv[i]);
       return 2;
@@ -94,7 +98,11 @@ int main(int argc, char** argv) {
   std::printf("start time %g, end time %g, output dt %g, outputs %d\n", Ini.start_time, Ini.end_time, Ini.output_dt,
               Ini.num_output);
   Solver solver(Ini);
-  solver.Load_solver(Ini, c, [](int n, SPH_mesh_generation& s) {
+  solver.Load_solver(Ini, c, [outdir](int n, SPH_mesh_generation& s) {
+    if (outdir) {
+      s.Output_particle(outdir, n);
+      s.Save_particle_post_file(outdir, n);
+    }
     const smg_stats st = s.Output_simulation_infor();
     std::printf("output %d: steps %lld dt %.6e residual %.3e particles %lld real %lld surface %lld rebuilds %lld\n", n,
                 (long long)st.step, st.dt2, st.residual, (long long)st.n_particle, (long long)st.n_real,
@@ -135,6 +143,7 @@ int main(int argc, char** argv) {
     }
     std::fclose(f);
     std::printf("mesh: %zu cells, %zu boundary triangles\n", m.cells.size() / per, m.surface.size() / 3);
+    if (outdir) solver.current_solver.Output_mesh(outdir, 0, m);
   }
   return 0;
 }

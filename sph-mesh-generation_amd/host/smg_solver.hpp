@@ -139,6 +139,37 @@ class SPH_mesh_generation {
     check(smg_tet_surface(job_, m.surface.data(), ns, &ns), "smg_tet_surface");
     return m;
   }
+  // Particle_binary_writer::Output_particle: particle_rank_0_step_<n>.vtu, .pvtu and
+  // particle.pvd in dir (positions, type, h, neighbour counts, normals)
+  void Output_particle(const char* dir, int n) {
+    std::vector<double> x(3 * total_num_particle), h(total_num_particle);
+    std::vector<int32_t> type(total_num_particle), nn(total_num_particle);
+    std::vector<int64_t> offs(total_num_particle + 1);
+    std::vector<float> nrm(3 * total_num_particle);
+    check(smg_get_state(job_, x.data(), nullptr, type.data(), h.data(), nullptr), "smg_get_state");
+    check(smg_get_neighbors(job_, offs.data(), nullptr, nullptr, 0), "smg_get_neighbors");
+    for (int64_t i = 0; i < total_num_particle; i++) nn[i] = int32_t(offs[i + 1] - offs[i]);
+    check(smg_get_normals(job_, nrm.data()), "smg_get_normals");
+    const smg_vtu_points p{total_num_particle, x.data(), type.data(), h.data(), nn.data(), nullptr, nrm.data()};
+    check(smg_write_particles_vtu(dir, n, run_time, &p), "smg_write_particles_vtu");
+  }
+  // Save_particle_post_file (sph_mesh_generation.cpp:2624-2667): step_<run_time>_rank_0.pst
+  // and a list.csv line in dir
+  void Save_particle_post_file(const char* dir, int n) {
+    std::vector<double> x(3 * total_num_particle);
+    check(smg_get_state(job_, x.data(), nullptr, nullptr, nullptr, nullptr), "smg_get_state");
+    check(smg_write_post_file(dir, int64_t(run_time), total_num_particle, x.data(), n == 0), "smg_write_post_file");
+  }
+  // Mesh_binary_writer::Output_mesh of a Mesh_generation result
+  void Output_mesh(const char* dir, int n, const Mesh& m) {
+    std::vector<double> x(3 * total_num_particle), h(total_num_particle);
+    std::vector<int32_t> type(total_num_particle);
+    check(smg_get_state(job_, x.data(), nullptr, type.data(), h.data(), nullptr), "smg_get_state");
+    const smg_vtu_points p{total_num_particle, x.data(), type.data(), h.data(), nullptr, nullptr, nullptr};
+    const int per = m.dim == 2 ? 3 : 4;
+    check(smg_write_mesh_vtu(dir, n, run_time, m.dim, &p, m.cells.data(), int64_t(m.cells.size() / per)),
+          "smg_write_mesh_vtu");
+  }
   // Output_simulation_infor: step count, dt, residual, particle counts
   smg_stats Output_simulation_infor() {
     smg_stats st{};

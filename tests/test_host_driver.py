@@ -48,8 +48,10 @@ def _read_mesh(path, per):
 def test_driver_equals_python_binding(smg_mod, tmp_path):
     csv = tmp_path / "x.csv"
     mesh = tmp_path / "mesh.txt"
-    out = subprocess.run([_driver(), "--outputs", "2", "--output-dt", "10", "--csv", str(csv), "--mesh", str(mesh)],
-                         capture_output=True, text=True, timeout=300)
+    outdir = tmp_path / "outdata"
+    outdir.mkdir()
+    out = subprocess.run([_driver(), "--outputs", "2", "--output-dt", "10", "--csv", str(csv), "--mesh", str(mesh),
+                          "--outdir", str(outdir)], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
     assert sum(line.startswith("output ") for line in out.stdout.splitlines()) == 2
     data = np.loadtxt(csv, delimiter=",", skiprows=1)
@@ -66,6 +68,17 @@ def test_driver_equals_python_binding(smg_mod, tmp_path):
     cells, _ = _read_mesh(mesh, 3)
     assert len(cells) > 0
     assert np.array_equal(cells, tris)
+    # the reference's output files of the two rounds and of the final mesh
+    n = len(g["h"])
+    assert (outdir / "list.csv").read_text() == f"1\n10 {n}\n20 {n}\n"
+    ranks, steps = smg_mod.read_post_list(outdir)
+    x, _ = smg_mod.read_post_file(outdir, steps[-1][0], ranks)
+    assert np.array_equal(x[:, :2], g["x"][:, :2])
+    assert (outdir / "particle.pvd").read_text().count("<DataSet") == 2
+    v = smg_mod.read_vtu(outdir / "particle_rank_0_step_1.vtu")
+    assert np.array_equal(v["Position"], g["x"].astype(np.float32)) and np.array_equal(v["type"], g["type"])
+    m = smg_mod.read_vtu(outdir / "mesh_tris_rank_0_step_0.vtu")
+    assert np.array_equal(m["connectivity"].reshape(-1, 3), tris)
 
 
 @pytest.mark.gpu
