@@ -530,26 +530,124 @@ __global__ __launch_bounds__(64 * kCutWaves) void k_lset_cut3a(LGrid g, const in
   }
 }
 
-// 3D, second pass: one lane per queued lev-2 node, walking its subtree depth first over
-// levels 3..5 with the path (ch3, ch4, ch5) and the open nodes c3, c4 in registers.
-// Split lev-5 nodes are not expanded where they are found: they go to a per-wavefront
-// buffer in LDS, and whenever it holds 64 the whole wavefront expands 64 of them at once
-// (their 8 leaves each), adding each result to the owning lane's sum with an LDS
-// atomic.  All terms are exact dyadic numbers, so the order of the additions does not
-// matter; each lane's total goes to its cell with one global atomic.
-constexpr int kLeafBuf = 128;
+// 3D, second pass: one lane per queued lev-2 node.  Each lane walks levels 3 and 4 of its
+// subtree depth first (its open lev-3 node in registers); split lev-4 nodes go to a
+// per-wavefront LDS buffer, and whenever it holds 64 the whole wavefront expands 64 of
+// them at once (the 8 children of each from one refinement); their split lev-5 children
+// go to a second buffer, expanded 64 at a time into their 8 leaves.  Results are added
+// to the owning lane's sum with LDS atomics: every term is an exact dyadic number, so the
+// order of the additions does not matter.  Each cell gets one global atomic per
+// wavefront.
+constexpr int kBuf = 128;
+
+struct NodeBuf {
+  double node[kBuf][8];
+  int32_t own[kBuf];
+};
+
+template <int CH>
+__device__ __forceinline__ void pick(const double r[3][3][3], double* out) {
+  constexpr int ca = CH >> 2, cb = (CH >> 1) & 1, ce = CH & 1;
+#pragma unroll
+  for (int p = 0; p < 2; p++)
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int t = 0; t < 2; t++) out[4 * p + 2 * q + t] = r[ca + p][cb + q][ce + t];
+}
+
+// child ch of a refined set, ch the same in every lane (a scalar branch, no selects)
+__device__ __forceinline__ void pick_uniform(const double r[3][3][3], int ch, double* out) {
+  switch (ch) {
+    case 0: pick<0>(r, out); break;
+    case 1: pick<1>(r, out); break;
+    case 2: pick<2>(r, out); break;
+    case 3: pick<3>(r, out); break;
+    case 4: pick<4>(r, out); break;
+    case 5: pick<5>(r, out); break;
+    case 6: pick<6>(r, out); break;
+    default: pick<7>(r, out); break;
+  }
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// append the nodes of the lanes that want to the buffer (nb < 64 on entry)
+__device__ __forceinline__ void push_nodes(NodeBuf& B, int& nb, bool want, const double* n, int owner, int lane) {
+  const unsigned long long m = __ballot(want);
+  if (want) {
+    const int slot = nb + __popcll(m & ((1ull << lane) - 1ull));
+#pragma unroll
+    for (int q = 0; q < 8; q++) B.node[slot][q] = n[q];
+    B.own[slot] = owner;
+  }
+  nb += __popcll(m);
+  wave_sync();
+}
+
+// the leaves of the top min(64, n5) lev-5 nodes
+__device__ __forceinline__ void drain5(NodeBuf& B5, int& n5, double (&accl)[64], int lane) {
+  const int take = n5 >= 64 ? 64 : n5;
+  const int base = n5 - take;
+  if (lane < take) {
+    double c[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) c[q] = B5.node[base + lane][q];
+    atomicAdd(&accl[B5.own[base + lane]], leaves6(c));
+  }
+  n5 = base;
+  wave_sync();
+}
+
+// the children of the top min(64, n4) lev-4 nodes; split ones go to B5
+__device__ __forceinline__ void drain4(NodeBuf& B4, int& n4, NodeBuf& B5, int& n5, double (&accl)[64], int lane) {
+  const int take = n4 >= 64 ? 64 : n4;
+  const int base = n4 - take;
+  const bool have = lane < take;
+  double r[3][3][3];
+  int own = 0;
+  {
+    double c[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+    if (have) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) c[q] = B4.node[base + lane][q];
+      own = B4.own[base + lane];
+    }
+    refine3(c, r);
+  }
+  n4 = base;
+  wave_sync();   // the taken slots are free again
+  double acc = 0.0;
+#pragma unroll 1
+  for (int ch = 0; ch < 8; ch++) {
+    double c5[8], v;
+    pick_uniform(r, ch, c5);
+    bool want = false;
+    if (have) {
+      if (node3(c5, 5, 0x1p-18, v)) acc += v;
+      else want = true;
+    }
+    push_nodes(B5, n5, want, c5, own, lane);
+    if (n5 >= 64) drain5(B5, n5, accl, lane);
+  }
+  if (have) atomicAdd(&accl[own], acc);
+  wave_sync();
+}
 
 __global__ __launch_bounds__(256) void k_lset_cut3b(LGrid g, const Q2* __restrict__ queue, int64_t nq,
                                                     double* __restrict__ hsum) {
-  __shared__ double buf[4][kLeafBuf][8];
-  __shared__ int32_t own[4][kLeafBuf];
+  __shared__ NodeBuf b4[4], b5[4];
   __shared__ double accl[4][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (int64_t(blockIdx.x) * blockDim.x + 64 * wave >= nq) return;   // whole wavefronts leave together
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool live = t < nq;
   const Q2 e = live ? queue[t] : Q2{0, 0};
-  double c2[8], c3[8], c4[8];
+  double c2[8], c3[8];
   {
     const int i = e.cell % g.nc[0], j = (e.cell / g.nc[0]) % g.nc[1], k = e.cell / (g.nc[0] * g.nc[1]);
     double C[3][3][3];
@@ -558,79 +656,45 @@ __global__ __launch_bounds__(256) void k_lset_cut3b(LGrid g, const Q2* __restric
   }
   accl[wave][lane] = 0.0;
   double acc = 0.0;
-  int L = 3, ch3 = 0, ch4 = 0, ch5 = 0;
+  int L = 3, ch3 = 0, ch4 = 0;
   bool active = live;
-  int nbuf = 0;   // the same in every lane (formed from ballots)
+  int n4 = 0, n5 = 0;   // the same in every lane (formed from ballots)
   while (true) {
     bool want = false;
     double n[8];
     if (active) {
       double par[8];
 #pragma unroll
-      for (int q = 0; q < 8; q++) par[q] = L == 3 ? c2[q] : (L == 4 ? c3[q] : c4[q]);
-      child3(par, L == 3 ? ch3 : (L == 4 ? ch4 : ch5), n);
-      const double w = L == 3 ? 0x1p-12 : (L == 4 ? 0x1p-15 : 0x1p-18);
+      for (int q = 0; q < 8; q++) par[q] = L == 3 ? c2[q] : c3[q];
+      child3(par, L == 3 ? ch3 : ch4, n);
       double v;
       bool next = true;
-      if (node3(n, L, w, v)) {
+      if (node3(n, L, L == 3 ? 0x1p-12 : 0x1p-15, v)) {
         acc += v;
-      } else if (L == 5) {
+      } else if (L == 4) {
         want = true;
-      } else {   // open the node
-        if (L == 3) {
+      } else {   // open the lev-3 node
 #pragma unroll
-          for (int q = 0; q < 8; q++) c3[q] = n[q];
-          ch4 = 0;
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; q++) c4[q] = n[q];
-          ch5 = 0;
-        }
-        L++;
+        for (int q = 0; q < 8; q++) c3[q] = n[q];
+        ch4 = 0;
+        L = 4;
         next = false;
       }
-      if (next) {   // next sibling, closing finished nodes
-        while (true) {
-          if (L == 5) {
-            if (++ch5 < 8) break;
-            L = 4;
-          } else if (L == 4) {
-            if (++ch4 < 8) break;
-            L = 3;
-          } else {
-            active = ++ch3 < 8;
-            break;
-          }
+      if (next) {
+        if (L == 4 && ++ch4 < 8) {
+        } else {
+          L = 3;
+          active = ++ch3 < 8;
         }
       }
     }
-    const unsigned long long m = __ballot(want);
-    if (want) {
-      const int slot = nbuf + __popcll(m & ((1ull << lane) - 1ull));
-#pragma unroll
-      for (int q = 0; q < 8; q++) buf[wave][slot][q] = n[q];
-      own[wave][slot] = lane;
-    }
-    nbuf += __popcll(m);
+    push_nodes(b4[wave], n4, want, n, lane, lane);
     const bool done = __ballot(active) == 0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    while (nbuf >= 64 || (done && nbuf > 0)) {
-      const int take = nbuf >= 64 ? 64 : nbuf;
-      const int base = nbuf - take;
-      if (lane < take) {
-        double c[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) c[q] = buf[wave][base + lane][q];
-        atomicAdd(&accl[wave][own[wave][base + lane]], leaves6(c));
-      }
-      nbuf = base;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    while (n4 >= 64 || (done && n4 > 0)) drain4(b4[wave], n4, b5[wave], n5, accl[wave], lane);
+    if (done) {
+      while (n5 > 0) drain5(b5[wave], n5, accl[wave], lane);
+      break;
     }
-    if (done) break;
   }
   acc += accl[wave][lane];
   // one atomic per cell present in the wavefront (a cell's entries are contiguous)
