@@ -1060,3 +1060,36 @@ def read_vtu(path):
         v = np.frombuffer(data[off + 4:off + 4 + nbytes], dt)
         out[name] = v.reshape(-1, comps) if comps > 1 else v
     return out
+
+
+def post_process(dir, cfg, lset, outdir=None):
+    """SPH_mesh_generation::Load_data_for_post (sph_mesh_generation.cpp:426-590): for every
+    step in dir/list.csv, the particles of all its rank files are placed on the level set
+    (Get_particle_info, Set_particle_scale and Find_and_map_particle, here smg_prerun,
+    whose force passes move nothing) and meshed: triangles in 2D (where the reference's
+    post mode leaves the mesh empty), tetrahedra after the deletion rule in 3D; with outdir, each step's mesh goes to a mesh VTU (output n_post,
+    time = the step).  Returns [(step, cells)]."""
+    ranks, steps = read_post_list(dir)
+    out = []
+    for n_post, (step, count) in enumerate(steps):
+        x, color = read_post_file(dir, step, ranks)
+        if len(x) != count:
+            raise ValueError(f"step {step}: list.csv says {count} particles, the files hold {len(x)}")
+        job = Job(cfg, lset)
+        try:
+            job.init_particles(x[:, :cfg.dim], np.zeros(len(x), np.int32))
+            job.prerun()
+            if cfg.dim == 2:
+                # the 2D mesh stage reads the neighbour lists of a step; the first step
+                # after Pre_run moves nothing (its velocities are zero) and builds them
+                job.step(1)
+                cells = triangulate(job)[0]
+            else:
+                cells = tetrahedralize(job)[0]
+            if outdir is not None:
+                st = job.state()
+                write_mesh_vtu(outdir, n_post, float(step), cfg.dim, st["x"], cells, st["type"], st["h"], color)
+        finally:
+            job.close()
+        out.append((step, cells))
+    return out

@@ -48,3 +48,26 @@ def test_job_mesh_output_3d(smg_mod, tmp_path):
     assert np.all((m["mindihedangle"] > 0) & (m["mindihedangle"] < 180))
     assert float(m["mindihedangle"].min()) == pytest.approx(st["min_dihedral"], rel=1e-5)
     job.close()
+
+
+def test_post_processing_mode(smg_mod, c1, tmp_path):
+    """Load_data_for_post: particles saved with the post files are placed on the level
+    set again and meshed; with no particle in a cut cell the placement moves nothing, so
+    the triangles equal the relaxed job's own."""
+    cfg, lset = c1
+    job = smg_mod.Job(cfg, lset)
+    job.seed(int(lset["stats"].num_particle))
+    job.step(20)
+    st = job.state()
+    tris, _ = smg_mod.triangulate(job)
+    smg_mod.write_post_file(tmp_path, 20, st["x"][:, :2], first=True)
+    job.close()
+    out = tmp_path / "outdata"
+    out.mkdir()
+    res = smg_mod.post_process(tmp_path, cfg, lset, outdir=out)
+    assert [r[0] for r in res] == [20]
+    assert np.count_nonzero(st["type"] == 1) == 0
+    assert np.array_equal(res[0][1], tris)
+    m = smg_mod.read_vtu(out / "mesh_tris_rank_0_step_0.vtu")
+    assert np.array_equal(m["connectivity"].reshape(-1, 3), tris)
+    assert 'timestep="20"' in (out / "mesh_tris.pvd").read_text()
