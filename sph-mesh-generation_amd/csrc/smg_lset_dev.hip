@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1008,7 +1009,10 @@ int lset_build_device(const smg_config* cfg, const double* phi_ext, int zero_mar
     if (g.D == 3) {
       // about 50 split lev-2 nodes per cut cell on smooth surfaces; a cell that finds
       // the queue full walks its nodes itself
-      qcap = std::max<int64_t>(96 * nlist, int64_t(1) << 16);
+      // SMG_LSET_QUEUE_PER_CELL overrides the 96 (a test reaches the regrow this way)
+      int64_t per_cell = 96;
+      if (const char* env = std::getenv("SMG_LSET_QUEUE_PER_CELL")) per_cell = std::max(1L, std::atol(env));
+      qcap = per_cell == 96 ? std::max<int64_t>(96 * nlist, int64_t(1) << 16) : per_cell * nlist;
       for (int pass = 0; pass < 2; pass++) {   // the second pass only if the queue was short
         HIP_TRY_LS(hipMalloc(&d_queue, sizeof(Q2) * qcap));
         HIP_TRY_LS(hipMemsetAsync(&d_cnt->n_queue, 0, sizeof(unsigned long long), s));

@@ -124,3 +124,20 @@ def test_relaxation_on_gpu_built_levelset(smg_mod):
         job.close()
     assert np.array_equal(out[0]["x"], out[1]["x"])
     assert np.array_equal(out[0]["type"], out[1]["type"])
+
+
+def test_short_queue_is_regrown(smg_mod, monkeypatch):
+    """A level-2 queue too short for the cut cells (1 entry per cell) is counted, grown
+    and the first pass run again: the same bits as with the default size."""
+    cfg, sdf = smg_mod.config_c3(num_pkg=8)
+    X, Y, Z = smg_mod.cell_centres_ext(cfg)
+    ext = sdf(X, Y, Z)
+    ref = smg_mod.lset_build_gpu(cfg, ext)
+    monkeypatch.setenv("SMG_LSET_QUEUE_PER_CELL", "1")
+    G = smg_mod.lset_build_gpu(cfg, ext)
+    for k in ("phi", "normal", "tag"):
+        assert np.array_equal(ref[k], G[k]), k
+    for f, _ in smg_mod.SmgLsetStats._fields_:
+        assert getattr(ref["stats"], f) == getattr(G["stats"], f), f
+    P = smg_mod.lset_build(cfg, ext)
+    assert P["stats"].total_volume == G["stats"].total_volume
