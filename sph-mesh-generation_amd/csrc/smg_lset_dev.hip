@@ -60,8 +60,9 @@ struct LCount {
   unsigned long long n_queue;        // split lev-2 nodes of 3D cut cells
 };
 
+// the full-margin grid holds fewer than 2^31 cells (checked on the host): 32-bit indexing
 __device__ __forceinline__ double at(const LGrid& g, int i, int j, int k) {
-  const int64_t a = i + 4, b = g.D > 1 ? j + 4 : 0, c = g.D > 2 ? k + 4 : 0;
+  const int a = i + 4, b = g.D > 1 ? j + 4 : 0, c = g.D > 2 ? k + 4 : 0;
   return g.ext[(c * g.ext_n[1] + b) * g.ext_n[0] + a];
 }
 
@@ -943,7 +944,7 @@ int lset_build_device(const smg_config* cfg, const double* phi_ext, int zero_mar
     n_int *= g.nc[a];
     npkg *= on ? cfg->num_pkg[a] : 1;
   }
-  if (n_halo > (int64_t(1) << 31) - 1 || n_int > (int64_t(1) << 31) - 1) {
+  if (n_ext > (int64_t(1) << 31) - 1 || n_halo > (int64_t(1) << 31) - 1 || n_int > (int64_t(1) << 31) - 1) {
     set_error("smg_lset_build_gpu: grid larger than 2^31 cells");
     return SMG_ERR_ARG;
   }
