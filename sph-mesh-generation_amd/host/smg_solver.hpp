@@ -47,7 +47,8 @@ class Initialization {
 };
 
 // The level set the case runs on: cell-centred phi with a full margin package per
-// side, from which smg_lset_build derives normals, cut-cell tags and the particle count.
+// side, from which smg_lset_build_gpu derives normals, cut-cell tags and the particle
+// count.
 struct LevelSetCase {
   smg_config cfg{};
   std::vector<double> phi_ext;
@@ -75,8 +76,11 @@ class SPH_mesh_generation {
     phi_.assign(m, 0.);
     normal_.assign(3 * m, 0.f);
     tag_.assign(m, 0);
-    check(smg_lset_build(&cfg_, c.phi_ext.data(), c.zero_margin, phi_.data(), normal_.data(), tag_.data(), &lset_),
-          "smg_lset_build");
+    // the level-set finishing on the GPU (bit-identical to smg_lset_build for the
+    // uniform and linear spacing profiles)
+    check(smg_lset_build_gpu(&cfg_, c.phi_ext.data(), c.zero_margin, phi_.data(), normal_.data(), tag_.data(), &lset_,
+                             nullptr),
+          "smg_lset_build_gpu");
     total_num_particle = lset_.num_particle;
     check(smg_create(&cfg_, &job_), "smg_create");
     check(smg_set_levelset(job_, phi_.data(), normal_.data(), tag_.data()), "smg_set_levelset");

@@ -27,7 +27,7 @@ def test_driver_fails_loudly_without_gpu():
     out = subprocess.run([_driver(), "--outputs", "1", "--output-dt", "2"], env=env, capture_output=True, text=True,
                          timeout=120)
     assert out.returncode == 1
-    assert "ERROR in smg_seed_particles (-2)" in out.stderr
+    assert "ERROR in smg_lset_build_gpu (-2)" in out.stderr
 
 
 @pytest.mark.gpu
