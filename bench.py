@@ -93,7 +93,10 @@ def relax(args, smg):
     log(f"seeded + {args.warmup} warmup steps in {time.perf_counter() - t0:.1f} s")
     rb0 = job.stats()["n_rebuild"]
     G = args.graph_steps
-    steps = (args.steps // G) * G
+    # whole graphs only, and at least one
+    steps = max(G, -(-args.steps // G) * G)
+    if steps != args.steps:
+        log(f"--steps {args.steps} rounded up to {steps} (whole graphs of {G} steps)")
     # timed region: the step loop replayed as a captured hipGraph of G steps
     job.capture(G)
     job.sync()
@@ -175,11 +178,78 @@ def relax(args, smg):
         "kernels_ms": {k: v[0] for k, v in prof.items()},
     }
     if not args.no_cpu:
+        log("accuracy vs the FP64 oracle ...")
+        out["accuracy"] = accuracy(cfg, lset, job)
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(cfg, lset, n, args.cpu_steps)
     else:
+        out["accuracy"] = None
         out["cpu_baseline"] = None
+    job.close()
+    if args.full_run:
+        out["full_run"] = full_run(smg, cfg, lset, n, args.full_run, G)
     print(json.dumps(out), flush=True)
+
+
+def full_run(smg, cfg, lset, n, nsteps, G):
+    """The whole relaxation of the workload (seeding, pre-run and nsteps steps, the
+    step loop as graph replays), timed end to end on the device."""
+    job = smg.Job(cfg, lset, profile=False)
+    t0 = time.perf_counter()
+    job.seed(n)
+    job.capture(G)
+    job.step_async(-(-nsteps // G) * G)
+    job.sync()
+    el = time.perf_counter() - t0
+    st = job.stats()
+    job.close()
+    steps = -(-nsteps // G) * G
+    return {"steps": steps, "seconds": el, "particle_steps_per_s": n * steps / el,
+            "list_rebuilds": st["n_rebuild"], "residual_mean": st["residual"], "residual_max": st["res_max"]}
+
+
+def accuracy(cfg, lset, job):
+    """Single-step error of the timed workload against the FP64 oracle: the oracle
+    takes one step from the engine's current state (positions as the engine stores
+    them), the engine takes the same step; the displacement each sets up for the next
+    step (v dt2) is compared per particle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    oracle.set_position_rounding(True)
+    s = job.state()
+    st = job.stats()
+    ref = oracle.State.from_arrays(s["x"], s["v"], s["h"], s["nu"].astype(np.float64), s["type"],
+                                   st["dt2"] if st["step"] > 0 else 0.0)
+    t0 = time.perf_counter()
+    d = oracle.step(cfg, lset, ref, st["step"] + 1)
+    t_or = time.perf_counter() - t0
+    job.step(1)
+    g = job.state()
+    st1 = job.stats()
+    oracle.set_position_rounding(False)
+    dg = g["v"] * st1["dt2"]
+    do = ref.v * ref.dt
+    ng = np.linalg.norm(dg - do, axis=1)
+    no = np.linalg.norm(do, axis=1)
+    scale = no.max()
+    rel = ng / np.maximum(no, 1e-300)
+    moving = no > 1e-3 * scale
+    return {
+        "what": "one step from the engine's state after the timed window, engine vs FP64 oracle; "
+                "displacement = v * dt2 per particle",
+        "step": int(st["step"] + 1),
+        "positions_bit_exact": bool(np.array_equal(g["x"], ref.x)),
+        "types_equal": bool(np.array_equal(g["type"], ref.type)),
+        "disp_err_max_over_max_disp": float(ng.max() / scale),
+        "disp_rel_err_p50": float(np.median(rel[moving])),
+        "disp_rel_err_p99": float(np.percentile(rel[moving], 99)),
+        "disp_rel_err_max": float(rel[moving].max()),
+        "particles_compared": int(moving.sum()),
+        "dt2_rel_err": float(abs(st1["dt2"] - d["dt2"]) / d["dt2"]),
+        "n_remapped_oracle": d["n_remapped"],
+        "oracle_seconds": t_or,
+    }
 
 
 def batch(args, smg):
@@ -498,6 +568,7 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--max-active", type=int, default=4000000)
     ap.add_argument("--c3-max-steps", type=int, default=6000)
+    ap.add_argument("--full-run", type=int, default=1000, help="relax mode: also time a whole run of this many steps (0: off)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (one GPU per box); "

@@ -848,15 +848,21 @@ int Engine::capture_graph(int64_t steps_per_graph) {
   graph_ = g;
   HIP_TRY(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
   graph_steps_ = steps_per_graph;
+  graph_par_ = int(save & 1);
   return SMG_OK;
 }
 
 // Each replay runs graph_steps_ steps; the parity and the A/B buffer roles repeat
 // every two steps, and the re-seeding RNG stream comes from the device step counter,
-// so a replay is the same computation as the eager steps it stands for.
+// so a replay is the same computation as the eager steps it stands for, provided the
+// step index has the parity it had at capture (the parity picks the dt / max v slots
+// the captured kernels read and write).
 int Engine::run_graph(int64_t launches) {
   if (!graph_exec_) { set_error("run_graph: no graph"); return SMG_ERR_STATE; }
-  if (step_ & 1) { set_error("run_graph: step index must be even"); return SMG_ERR_STATE; }
+  if (int(step_ & 1) != graph_par_) {
+    set_error("run_graph: the step index parity differs from the parity at capture (take one eager step or capture again)");
+    return SMG_ERR_STATE;
+  }
   for (int64_t q = 0; q < launches; q++) {
     HIP_TRY(hipGraphLaunch(graph_exec_, st_));
     step_ += graph_steps_;

@@ -192,6 +192,7 @@ class Engine {
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
   int64_t graph_steps_ = 0;
+  int graph_par_ = 0;   // step index parity at capture; replays need the same parity
 
   template <typename T>
   T* alloc_(int64_t count);
