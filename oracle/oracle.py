@@ -54,6 +54,7 @@ def lib():
         L.or_neighbors.argtypes = [P(SmgConfig), C.c_int64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_int64]
         L.or_pair_terms.argtypes = [P(SmgConfig), C.c_int64] + [C.c_void_p] * 9
+        L.or_stage2.argtypes = [P(SmgConfig), C.c_int64] + [C.c_void_p] * 9
         L.or_lset_build.argtypes = [P(SmgConfig), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                     P(SmgLsetStats)]
         L.or_num_threads.restype = C.c_int
@@ -213,6 +214,20 @@ def pair_terms(cfg, x, nrm, h, nu, types):
     sm = np.zeros(n); a1 = np.zeros((n, 3)); a2 = np.zeros((n, 3)); dts = np.zeros(3)
     lib().or_pair_terms(C.byref(cfg), n, _p(x3), _p(n3), _p(h), _p(nu), _p(t), _p(sm), _p(a1), _p(a2), _p(dts))
     return {"sum": sm, "a1": a1, "a2": a2, "dt1": dts[0], "dt2": dts[1], "max_v": dts[2]}
+
+
+def stage2(cfg, x, nrm, h, nu, v, types):
+    """Stage-2 (viscous) accelerations, nu and dt2 of a given state (no mapping)."""
+    n = len(h)
+    D = x.shape[1]
+    x3 = np.zeros((n, 3)); x3[:, :D] = x
+    n3 = np.zeros((n, 3)); n3[:, :nrm.shape[1]] = nrm
+    v3 = np.zeros((n, 3)); v3[:, :v.shape[1]] = v
+    h = np.ascontiguousarray(h, np.float64); nu = np.ascontiguousarray(nu, np.float64)
+    t = np.ascontiguousarray(types, np.int32)
+    a2 = np.zeros((n, 3)); nu_out = np.zeros(n); dts = np.zeros(2)
+    lib().or_stage2(C.byref(cfg), n, _p(x3), _p(n3), _p(h), _p(nu), _p(v3), _p(t), _p(a2), _p(nu_out), _p(dts))
+    return {"a2": a2, "nu": nu_out, "dt2": dts[0], "max_v": dts[1]}
 
 
 def sample(cfg, lset, x):

@@ -778,6 +778,38 @@ int or_pair_terms(const smg_config* cfg, int64_t n, const double* x, const doubl
   return SMG_OK;
 }
 
+// Stage 2 alone for a given state (x, v, h, nu, type, n), no mapping: the viscous
+// accelerations (Accumulate_particle_force(2)), then Set_timestep(2), which also sets
+// nu = nu_coeff h |v|.  out_a2[n*3], out_nu[n] (may be NULL), dts = {dt2, max |v|}.
+int or_stage2(const smg_config* cfg, int64_t n, const double* x, const double* nrm, const double* h,
+              const double* nu, const double* v, const int32_t* type, double* out_a2, double* out_nu,
+              double* dts) {
+  Ctx c; ctx_init(c, cfg, nullptr, nullptr, nullptr);
+  int D = c.D;
+  std::vector<P> ps(n);
+  for (int64_t i = 0; i < n; i++) {
+    P& p = ps[i]; std::memset(&p, 0, sizeof(P));
+    for (int a = 0; a < 3; a++) {
+      p.x[a] = a < D ? x[3 * i + a] : 0.;
+      p.n[a] = a < D ? nrm[3 * i + a] : 0.;
+      p.v[a] = a < D ? v[3 * i + a] : 0.;
+    }
+    p.h = h[i]; p.nu = nu[i]; p.type = type[i]; p.id = int(i);
+    double dx = p.h * cfg->cut_off / cfg->cell_ratio;
+    p.vol = std::pow(dx, D); p.rho = 1.0 / p.vol;
+  }
+  std::vector<std::vector<NbEntry>> nb;
+  neighbors(c, ps, nb);
+  accumulate_force(c, ps, nb, 2);
+  double dt2, mv2; set_timestep(c, ps, 2, dt2, mv2);
+  for (int64_t i = 0; i < n; i++) {
+    for (int a = 0; a < 3; a++) out_a2[3 * i + a] = ps[i].a[a];
+    if (out_nu) out_nu[i] = ps[i].nu;
+  }
+  if (dts) { dts[0] = dt2; dts[1] = mv2; }
+  return SMG_OK;
+}
+
 int or_num_threads(void) { return omp_get_max_threads(); }
 void or_set_position_rounding(int on) { g_round_positions = on ? 1 : 0; }
 void or_set_threads(int n) { omp_set_num_threads(n); }
