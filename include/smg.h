@@ -211,6 +211,14 @@ int smg_set_levelset_levels(smg_job* job, int32_t nlev, const smg_lset_level* le
  * (Calculate_particle_infor's interpolation, particle_mesh_generation.cpp:175-265, on
  * the finest stored level); the job must be seeded or initialised. */
 int smg_sample_phi(smg_job* job, int64_t n, const double* pts, double* phi);
+/* No counterpart in the reference: smg_sample_phi for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results, bit for bit; the points and results
+ * are staged in a fixed device array of 8 MiB that the library holds on each GPU until
+ * the process ends, so n is at most 262144 (SMG_ERR_CAPACITY beyond).  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (from several threads or
+ * jobs) race and must be serialised by the caller. */
+int smg_sample_phi_static(smg_job* job, int64_t n, const double* pts, double* phi);
 
 /* Particles at given positions (n*3 doubles, unused axes ignored); type may be NULL
  * (all REAL).  h is set from the level set (Set_particle_info). */

@@ -289,6 +289,12 @@ int smg_sample_phi(smg_job* job, int64_t n, const double* pts, double* phi) {
   return job->eng->sample_phi(0, n, pts, phi);
 }
 
+int smg_sample_phi_static(smg_job* job, int64_t n, const double* pts, double* phi) {
+  if (!job || n < 0 || (n > 0 && (!pts || !phi))) { set_error("smg_sample_phi_static: bad argument"); return SMG_ERR_ARG; }
+  if (!job->eng) { set_error("smg_sample_phi_static: seed or initialise the job first"); return SMG_ERR_STATE; }
+  return job->eng->sample_phi_static(0, n, pts, phi);
+}
+
 int smg_profile_enable(smg_job* job, int enable) {
   if (!job) { set_error("smg_profile_enable: job is NULL"); return SMG_ERR_ARG; }
   job->profile = enable != 0;

@@ -659,6 +659,30 @@ int Engine::sample_phi(int job, int64_t n, const double* pts, double* phi) {
   return SMG_OK;
 }
 
+// sample_phi without device allocations: the points and the results pass through the
+// file-scope staging array (calls that overlap on one GPU share it and race)
+int Engine::sample_phi_static(int job, int64_t n, const double* pts, double* phi) {
+  if (n == 0) return SMG_OK;
+  if (n > kStaticSampleMax) {
+    set_error("smg_sample_phi_static: " + std::to_string(n) + " points exceed the staging array's " +
+              std::to_string(kStaticSampleMax) + " (use smg_sample_phi, or sample in parts)");
+    return SMG_ERR_CAPACITY;
+  }
+  double* buf = nullptr;
+  if (static_sample_scratch(&buf) != 0) { set_error("smg_sample_phi_static: no staging array on this device"); return SMG_ERR_HIP; }
+  double* d_pts = buf;
+  double* d_phi = buf + 3 * n;
+  hipError_t e = hipMemcpyAsync(d_pts, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st_);
+  if (e == hipSuccess) {
+    launch_sample_phi(st_, D_, W_, job, n, d_pts, d_phi);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(phi, d_phi, sizeof(double) * n, hipMemcpyDeviceToHost, st_);
+  if (e == hipSuccess) e = hipStreamSynchronize(st_);
+  if (e != hipSuccess) { set_error(std::string("smg_sample_phi_static: ") + hipGetErrorString(e)); return SMG_ERR_HIP; }
+  return SMG_OK;
+}
+
 void Engine::begin_(const char* name) {
   if (!profiling_) return;
   if (ev_pool_.size() < 2) {

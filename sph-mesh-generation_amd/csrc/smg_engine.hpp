@@ -82,6 +82,8 @@ class Engine {
   int red_of(int job, JobRed* r);
   int download_reds();
   int sample_phi(int job, int64_t n, const double* pts, double* phi);
+  // the same with the file-scope staging array in place of device allocations
+  int sample_phi_static(int job, int64_t n, const double* pts, double* phi);
   const JobRed& red_host(int job) const { return red_h_[job]; }
   int64_t used_tiles() const { return is_pool_ ? tiles_free_.total() - tiles_free_.free_total() : ntiles_; }
 

@@ -21,6 +21,7 @@
 // sampling, projection, neighbour predicate) and for the dt / residual reductions;
 // the pair sums run in FP32.  The file is compiled with -ffp-contract=off so that no
 // FMA changes an FP64 rounding.
+#include <mutex>
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <type_traits>
@@ -2135,6 +2136,29 @@ void launch_sample_phi(hipStream_t st, int D, const Work& W, int job, int64_t n,
 }
 void launch_fp64_probe(hipStream_t st, int64_t n, const double* a, const double* b, double* out) {
   k_fp64_probe<<<(n + 255) / 256, 256, 0, st>>>(n, a, b, out);
+}
+
+// Staging area of smg_sample_phi_static: points (3 doubles each) then phi (1 double
+// each), in a file-scope device array so that the entry point allocates nothing.
+// Every GPU holds its own copy, at its own address, until the process ends; in host
+// code the name `scratch` is only a stand-in for the symbol, so the address each
+// device's copy has is looked up once per device and kept.
+static __device__ __align__(256) double scratch[kStaticSampleDoubles];
+
+int static_sample_scratch(double** ptr) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static double* addr[kMaxDevices] = {};
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -1;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+    addr[dev] = static_cast<double*>(p);
+  }
+  *ptr = addr[dev];
+  return 0;
 }
 
 }  // namespace smg

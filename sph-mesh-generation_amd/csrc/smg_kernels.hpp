@@ -56,4 +56,12 @@ struct V3Out {
 void launch_vor3d(hipStream_t st, const V3Grid& G, const V3Out& O);
 void launch_fp64_probe(hipStream_t st, int64_t n, const double* a, const double* b, double* out);
 
+// the no-allocation sampling entry point's staging array (doubles): points then phi,
+// 4 doubles per point
+constexpr int64_t kStaticSampleDoubles = int64_t(1) << 20;   // 8 MiB
+constexpr int64_t kStaticSampleMax = kStaticSampleDoubles / 4;
+// this device's address of the staging array (looked up on the first call on each
+// device, with that device current); 0 on success
+int static_sample_scratch(double** ptr);
+
 }  // namespace smg

@@ -83,6 +83,7 @@ def lib():
         L.smg_set_levelset.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.smg_set_levelset_levels.argtypes = [C.c_void_p, C.c_int32, P(SmgLsetLevel)]
         L.smg_sample_phi.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+        L.smg_sample_phi_static.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         L.smg_init_particles.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         L.smg_seed_particles.argtypes = [C.c_void_p, C.c_int64]
         L.smg_prerun.argtypes = [C.c_void_p]
