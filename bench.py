@@ -83,6 +83,8 @@ def cpu_baseline(cfg, lset, n, steps):
 def relax(args, smg):
     t0 = time.perf_counter()
     cfg, sdf = smg.config_c2(scale=args.scale, num_pkg=args.num_pkg)
+    if args.skin is not None:
+        cfg.skin = args.skin
     lset = smg.make_levelset(cfg, sdf)
     n = int(lset["stats"].num_particle)
     log(f"level set built in {time.perf_counter() - t0:.1f} s, n = {n}")
@@ -568,6 +570,7 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--max-active", type=int, default=4000000)
     ap.add_argument("--c3-max-steps", type=int, default=6000)
+    ap.add_argument("--skin", type=float, default=None, help="relax mode: list skin (default: the config's)")
     ap.add_argument("--full-run", type=int, default=1000, help="relax mode: also time a whole run of this many steps (0: off)")
     args = ap.parse_args()
     if args.gpus != 1:

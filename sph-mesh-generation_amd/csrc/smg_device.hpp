@@ -15,7 +15,19 @@ constexpr int kMaxLevels = 8;    // cell-list levels (cell size c0 / 2^L)
 constexpr int kJBits = 26;       // neighbour entry = slot | (mirror code << 26)
 constexpr uint32_t kJMask = (1u << kJBits) - 1u;
 constexpr int kScanItems = 4096; // table entries per scan workgroup (256 threads x 16)
-constexpr int kMaskWords = 2;    // stage-2 entry masks cover the first 128 list entries
+// Neighbour tiles of the pair passes.  At every list build, each 256-slot tile gets a
+// halo: the distinct (slot, mirror code) pairs its candidate lists name, the tile's own
+// 256 slots first (halo index = slot - tile start), the others after them.  The tile's
+// candidate lists are rewritten as items: 16-bit halo indices, one run per slot, all the
+// tile's runs back to back.  The pair passes stage the halo's records in LDS once per
+// tile and walk the items with the work split evenly over the workgroup's threads.
+constexpr int kHashN = 8192;     // LDS hash slots of the halo build (keys of one pass)
+constexpr int kHL1 = 1024;       // halo records the stage-1 pass stages in LDS
+constexpr int kHL2 = 1024;       // ... and the stage-2 pass
+constexpr int kMaxHalo = 65536;  // halo entries of a tile (16-bit items)
+constexpr uint32_t kHMir = 1u << 29;   // staged type word: the record is a mirror image
+// overflow bits of JobRed::overflow
+constexpr int kOverSpill = 1, kOverItems = 2, kOverHalo = 4;
 constexpr int kJumpMax = 64;     // particles per job that may outrun their lists between builds
 constexpr int kJumpList = 512;   // candidate entries of one jumper's own list
 constexpr int kJumpBit = 1 << 30;   // flag in the type bits of a record: the slot is a jumper
@@ -171,13 +183,25 @@ struct Work {
   int32_t* spill;   // spill pool, spill_cap entries
   int32_t* ovq;     // slots whose lists spill, appended by the list pass
   int32_t* ntrue;   // neighbours within the reference's cutoff (the force passes' count)
-  uint64_t* nbm;    // stage-2 entry masks from the stage-1 pass, [w * cap_total + slot]
+  // neighbour tiles (see kHashN): per tile t, halo entries 256.. at halo[t * ht] as
+  // (slot | code << 26), hcnt[t] = 256 + their number (0: not built); items of slot s at
+  // items[t * 256 * ips + ioff[s] ..] (nbc[s] of them); lane[t * 256 + r] = the tile's
+  // particle of list-length rank r (the pair passes' thread order)
+  uint16_t* items;
+  uint16_t* items2;  // per slot, from the same offset: its stage-2 pairs (nbc2 of them)
+  uint32_t* halo;
+  int32_t* hcnt;
+  int32_t* ioff;
+  int32_t* nbc2;
+  uint8_t* lane;      // lane order of the pair passes
   float4* recb;     // position (hi parts) and h at the last build: (x, y, z, h); h < 0 marks a jumper
   int32_t* jset;    // per job, the jumper slots (kJumpMax each), sorted by slot at each step
   int32_t* jl;      // per job and jumper index, that jumper's candidate list (kJumpList entries)
   int32_t* jcnt;    // its length
   unsigned long long* ctr;   // [0] = ovq length
   int64_t spill_cap;
+  int64_t ips;       // items per slot (a tile holds 256 * ips)
+  int64_t ht;        // halo entries per tile (<= kMaxHalo)
   int64_t tile0;   // first tile of a launch (launches over one job's tiles start there)
   int64_t cap_total;
   int64_t n_tbl;

@@ -261,7 +261,6 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   W_.nbo = alloc_<int32_t>(cap_total_);
   W_.ovq = alloc_<int32_t>(cap_total_);
   W_.ntrue = alloc_<int32_t>(cap_total_);
-  W_.nbm = alloc_<uint64_t>(int64_t(kMaskWords) * cap_total_);
   W_.recb = alloc_<float4>(cap_total_);
   W_.jset = alloc_<int32_t>(int64_t(njobs) * kJumpMax);
   W_.jl = alloc_<int32_t>(int64_t(njobs) * kJumpMax * kJumpList);
@@ -274,6 +273,18 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   if (sps_ < 1) { set_error("setup: too many slots for the spill pool"); return SMG_ERR_CAPACITY; }
   W_.spill_cap = int64_t(sps_) * cap_total_;
   W_.spill = alloc_<int32_t>(W_.spill_cap);
+  // neighbour tiles: a tile holds 256 (K + spill_per_slot) items (the lists' average
+  // capacity) and (K + spill_per_slot) 32 halo entries (at least 4096, at most 65536)
+  W_.ips = int64_t(K_) + sps_;
+  W_.ht = std::min<int64_t>(kMaxHalo, int64_t(kTile) * std::max<int64_t>(16, W_.ips / 8));
+  const int64_t ntl = cap_total_ / kTile;
+  W_.items = alloc_<uint16_t>(W_.ips * cap_total_);
+  W_.items2 = alloc_<uint16_t>(W_.ips * cap_total_);
+  W_.halo = alloc_<uint32_t>(std::max<int64_t>(ntl, 1) * W_.ht);
+  W_.hcnt = alloc_<int32_t>(std::max<int64_t>(ntl, 1));
+  W_.ioff = alloc_<int32_t>(cap_total_);
+  W_.nbc2 = alloc_<int32_t>(cap_total_);
+  W_.lane = alloc_<uint8_t>(cap_total_);
   W_.ctr = alloc_<unsigned long long>(2);
   if (alloc_failed_) { set_error("setup: device allocation failed (hipMalloc)"); return SMG_ERR_HIP; }
   W_.jp = d_jp_;
@@ -300,6 +311,10 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   HIP_TRY(hipMemsetAsync(W_.nbo, 0, sizeof(int32_t) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.ntrue, 0, sizeof(int32_t) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.recb, 0, sizeof(float4) * cap_total_, st_));
+  HIP_TRY(hipMemsetAsync(W_.hcnt, 0, sizeof(int32_t) * std::max<int64_t>(ntl, 1), st_));
+  HIP_TRY(hipMemsetAsync(W_.ioff, 0, sizeof(int32_t) * cap_total_, st_));
+  HIP_TRY(hipMemsetAsync(W_.nbc2, 0, sizeof(int32_t) * cap_total_, st_));
+  HIP_TRY(hipMemsetAsync(W_.lane, 0, cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.nu_a, 0, sizeof(float) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.nu_b, 0, sizeof(float) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.sum, 0, sizeof(float) * cap_total_, st_));
@@ -842,6 +857,9 @@ int Engine::step(int64_t nsteps) {
     begin_("k_nb_spill");
     launch_nb_spill(st_, D_, ntiles_, W_, A_);
     end_();
+    begin_("k_halo");
+    launch_halo(st_, D_, ntiles_, W_);
+    end_();
     begin_("k_force1");
     launch_force1(st_, D_, ntiles_, W_, A_, par);
     end_();
@@ -920,10 +938,21 @@ int Engine::check_health() {
   for (size_t j = 0; j < red_h_.size(); j++) {
     if (!active_[j]) continue;
     if (red_h_[j].fail) { set_error("job " + std::to_string(j) + ": re-seeding failed"); return SMG_ERR_REMAP; }
-    if (red_h_[j].overflow) {
+    const int ov = red_h_[j].overflow;
+    if (ov & kOverSpill) {
       set_error("job " + std::to_string(j) + ": neighbour spill pool exhausted (longest list " +
                 std::to_string(red_h_[j].maxnb) + ", inline " + std::to_string(K_) + ", pool " +
                 std::to_string(W_.spill_cap) + " entries); raise max_neighbors or spill_per_slot");
+      return SMG_ERR_CAPACITY;
+    }
+    if (ov & kOverItems) {
+      set_error("job " + std::to_string(j) + ": a tile's lists exceed its " + std::to_string(W_.ips * kTile) +
+                " pair items (max_neighbors + spill_per_slot per particle); raise max_neighbors or spill_per_slot");
+      return SMG_ERR_CAPACITY;
+    }
+    if (ov) {
+      set_error("job " + std::to_string(j) + ": a tile's neighbours exceed its " + std::to_string(W_.ht) +
+                " halo entries; raise max_neighbors or spill_per_slot");
       return SMG_ERR_CAPACITY;
     }
   }

@@ -1158,7 +1158,7 @@ __global__ __launch_bounds__(kTile) void k_nb_spill(Work W, PState S) {
     const unsigned long long off = atomicAdd(&W.red[j].spill_used, need);
     if (off + need > (unsigned long long)J.spill_cap) {
       W.nbc[slot] = K;
-      atomicOr(&W.red[j].overflow, 1);
+      atomicOr(&W.red[j].overflow, kOverSpill);
       continue;
     }
     const int64_t at = J.spill_base + int64_t(off);
@@ -1493,133 +1493,512 @@ __global__ __launch_bounds__(kTile) void k_nb_out(Work W, PState S, int pass, co
 }
 
 // ---------------------------------------------------------------------------
-// P3: kernel summation + stage-1 force + stage-1 dt, one pass over the list
+// Neighbour tiles of the pair passes: the halo and item build (list-build steps)
+
+static_assert(kHashN == 8192, "halo_hash yields 13 bits");
+__device__ __forceinline__ uint32_t halo_hash(uint32_t e) { return (e * 0x9E3779B1u) >> 19; }
+// the key subset of pass `pass` when a tile's halo takes P passes (P a power of two)
+__device__ __forceinline__ uint32_t halo_part(uint32_t e, int P) { return ((e * 0x85EBCA6Bu) >> 7) & uint32_t(P - 1); }
+
+// exclusive prefix sum over the 256 threads of a workgroup; total = the sum
+__device__ __forceinline__ int block_scan_excl(int v, int* lds4, int& total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    x += lane >= o ? y : 0;
+  }
+  if (lane == 63) lds4[w] = x;
+  __syncthreads();
+  int base = 0, tot = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int s = lds4[q];
+    base += q < w ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return base + x - v;
+}
+
+// the candidate entries of a slot in list order: fn(k, entry)
+template <class Fn>
+__device__ __forceinline__ void for_entries(const Work& W, int64_t slot, int cnt, int32_t off, Fn fn) {
+  const int ni = cnt < W.K ? cnt : W.K;
+  for (int g = 0; 4 * g < ni; g++) {
+    const uint4 v = nb_group(W, slot, g);
+    const uint32_t ev[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (4 * g + u < ni) fn(4 * g + u, ev[u]);
+  }
+  for (int k = ni; k < cnt; k++) fn(k, uint32_t(W.spill[int64_t(off) + (k - W.K)]));
+}
+
+// One workgroup per tile of a rebuilding job.  The tile's halo: its own slots (halo
+// index = slot - tile start), then every other (slot, mirror code) its candidate lists
+// name, each once, deduplicated in an LDS hash table; the candidate lists become items
+// (halo indices) in list order.  A halo too large for one table is built in P passes,
+// each taking the keys of one hash class.  The halo order is that of the table (it may
+// differ between runs); a particle's items keep the list order, and a halo index only
+// says where the record sits, so every pair sum is formed in the same order each run.
+template <int D>
+__global__ __launch_bounds__(kTile) void k_halo(Work W) {
+  __shared__ uint32_t hk[kHashN];
+  __shared__ uint16_t hv[kHashN];
+  __shared__ int s_scan[4];
+  __shared__ int s_len[kTile];
+  __shared__ int s_cnt, s_over;
+  const int64_t tile = tile_id(W);
+  const int j = W.tile_job[tile];
+  if (j < 0 || !W.red[j].rebuild) return;
+  const JobParams& J = W.jp[j];
+  const int64_t ts = tile * kTile, slot = ts + threadIdx.x;
+  const bool live = slot - J.p0 < J.n;
+  const int cnt = live ? W.nbc[slot] : 0;
+  const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
+  // The pair passes' thread order: rank by (list length, slot), so that a wavefront's
+  // threads walk lists of about the same length.  Items are stored by wavefront, eight
+  // to a 16-byte group, lane-interleaved: group g of the particle at wave w, lane l at
+  // woff[w] + (64 g + l) 8, where wave w's part holds as many groups as its longest
+  // list needs; a wavefront's loads of one group are 1 KB contiguous.
+  s_len[threadIdx.x] = cnt;
+  __syncthreads();
+  int rank = 0;
+  for (int q = 0; q < kTile; q++) {
+    const int cq = s_len[q];
+    rank += (cq < cnt || (cq == cnt && q < int(threadIdx.x))) ? 1 : 0;
+  }
+  __syncthreads();
+  // rank r goes to wavefront r % 4, lane r / 4: every wavefront gets a like share of
+  // short and long lists (so the four finish together), and lanes next to each other
+  // lists of about the same length
+  s_len[rank] = cnt;   // sorted: wave w's longest list has rank 252 + w
+  const int rw = rank & 3, rl = rank >> 2;
+  W.lane[ts + rw * 64 + rl] = uint8_t(threadIdx.x);
+  __syncthreads();
+  int M = 0, woff = 0;
+  for (int w = 0; w < kTile / 64; w++) {
+    const int g = (s_len[kTile - 4 + w] + 7) >> 3;
+    if (w < rw) woff += g * 512;
+    M += g * 512;
+  }
+  const int excl = woff + rl * 8;
+  W.ioff[slot] = excl;
+  // A tile that does not fit is marked invalid (hcnt = 0: the pair passes read none of
+  // its items) and its job flagged.
+  auto fail = [&](int bit) {
+    if (threadIdx.x == 0) {
+      atomicOr(&W.red[j].overflow, bit);
+      W.hcnt[tile] = 0;
+    }
+  };
+  if (int64_t(M) > int64_t(kTile) * W.ips) {   // M is the same in every thread
+    fail(kOverItems);
+    return;
+  }
+  uint16_t* it0 = W.items + tile * kTile * W.ips + excl;
+  auto item_at = [&](int k) -> uint16_t& { return it0[(k >> 3) * 512 + (k & 7)]; };
+  uint32_t* hal = W.halo + tile * W.ht;
+  const int cap = int(W.ht < int64_t(kMaxHalo) ? W.ht : int64_t(kMaxHalo)) - kTile;
+  auto own = [&](uint32_t e) {
+    const int64_t q = e & kJMask;
+    return (e >> kJBits) == 0u && q >= ts && q < ts + kTile;
+  };
+  int P = 1, base = 0;
+  for (int pass = 0; pass < P;) {
+    for (int s = threadIdx.x; s < kHashN; s += kTile) hk[s] = 0xFFFFFFFFu;
+    if (threadIdx.x == 0) {
+      s_cnt = 0;
+      s_over = 0;
+    }
+    __syncthreads();
+    // Insert this pass's keys.  Past 3/4 of the table a thread stops after its next new
+    // key, so at most 256 more keys arrive: the table never fills and every probe ends.
+    bool full = false;
+    for_entries(W, slot, cnt, off, [&](int, uint32_t e) {
+      if (full || own(e) || halo_part(e, P) != uint32_t(pass)) return;
+      uint32_t s = halo_hash(e);
+      while (true) {
+        const uint32_t old = atomicCAS(&hk[s], 0xFFFFFFFFu, e);
+        if (old == 0xFFFFFFFFu) {
+          full = atomicAdd(&s_cnt, 1) >= kHashN / 4 * 3;
+          break;
+        }
+        if (old == e) break;
+        s = (s + 1u) & uint32_t(kHashN - 1);
+      }
+    });
+    if (full) s_over = 1;
+    __syncthreads();
+    if (s_over) {   // the same in every thread: start again with twice the passes
+      __syncthreads();
+      if (P >= 64) {
+        fail(kOverHalo);
+        return;
+      }
+      P *= 2;
+      pass = 0;
+      base = 0;
+      continue;
+    }
+    // halo indices for the occupied table slots (thread t takes slots t, t + 256, ...)
+    constexpr int kPer = kHashN / kTile;
+    int c = 0;
+#pragma unroll 8
+    for (int u = 0; u < kPer; u++) c += hk[u * kTile + threadIdx.x] != 0xFFFFFFFFu ? 1 : 0;
+    int U = 0;
+    int idx = base + block_scan_excl(c, s_scan, U);
+    if (base + U > cap) {
+      fail(kOverHalo);
+      return;
+    }
+    for (int u = 0; u < kPer; u++) {
+      const int s = u * kTile + threadIdx.x;
+      const uint32_t key = hk[s];
+      if (key != 0xFFFFFFFFu) {
+        hv[s] = uint16_t(kTile + idx);
+        hal[idx] = key;
+        idx++;
+      }
+    }
+    __syncthreads();
+    for_entries(W, slot, cnt, off, [&](int k, uint32_t e) {
+      if (own(e)) {
+        if (pass == 0) item_at(k) = uint16_t((e & kJMask) - uint32_t(ts));
+        return;
+      }
+      if (halo_part(e, P) != uint32_t(pass)) return;
+      uint32_t s = halo_hash(e);
+      while (hk[s] != e) s = (s + 1u) & uint32_t(kHashN - 1);
+      item_at(k) = hv[s];
+    });
+    base += U;
+    pass++;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) W.hcnt[tile] = kTile + base;
+}
+
+// slot and mirror code of halo entry e of a tile
+__device__ __forceinline__ void halo_entry(const Work& W, int64_t tile, int64_t ts, uint32_t e, int64_t& q,
+                                           uint32_t& code) {
+  if (e < uint32_t(kTile)) {
+    q = ts + e;
+    code = 0u;
+    return;
+  }
+  const uint32_t v = W.halo[tile * W.ht + (e - kTile)];
+  q = v & kJMask;
+  code = v >> kJBits;
+}
+
+// A halo record as staged in LDS: the position of slot q's image as an unevaluated
+// pair of floats, (hi, h) and (lo, type word) -- the slot's own record for a plain
+// entry, the image 2w - x formed in FP64 and split again for a mirror image.  The type
+// word carries the type bits, the jumper bit and kHMir for an image.  Records are in
+// the box's frame, so two particles at the same point get the same relative positions
+// to every neighbour whatever tile they sit in.
+template <int D>
+__device__ __forceinline__ void halo_rec(const Work& W, const JobParams& J, int64_t q, uint32_t code, float4& rh,
+                                         float4& rl) {
+  rh = W.rec[4 * q + 0];
+  rl = W.rec[4 * q + 1];
+  if (code) {
+    const double x[3] = {double(rh.x) + double(rl.x), double(rh.y) + double(rl.y), double(rh.z) + double(rl.z)};
+    double g[3] = {x[0], x[1], x[2]};
+    image_of<D>(J, code, x, g);
+    const float gh[3] = {float(g[0]), float(g[1]), float(g[2])};
+    rh = make_float4(gh[0], gh[1], gh[2], rh.w);
+    rl = make_float4(float(g[0] - double(gh[0])), float(g[1] - double(gh[1])), float(g[2] - double(gh[2])),
+                     __int_as_float(__float_as_int(rl.w) | int(kHMir)));
+  }
+}
+
+// The reference's predicate in FP64 on the stored positions (for pairs whose FP32
+// distance falls in the error band around the cutoff)
+template <int D>
+__device__ __forceinline__ bool exact_pred(const Work& W, const JobParams& J, const PState& S, int64_t si, int64_t q,
+                                           uint32_t code) {
+  const float4 a0 = W.rec[4 * si + 0], a1 = W.rec[4 * si + 1];
+  const float4 b0 = W.rec[4 * q + 0], b1 = W.rec[4 * q + 1];
+  const double xi[3] = {double(a0.x) + double(a1.x), double(a0.y) + double(a1.y), double(a0.z) + double(a1.z)};
+  const double xq[3] = {double(b0.x) + double(b1.x), double(b0.y) + double(b1.y), double(b0.z) + double(b1.z)};
+  double g[3];
+  image_of<D>(J, code, xq, g);
+  double dist = 0.0;
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    const double dr = xi[a] - g[a];
+    dist += dr * dr;
+  }
+  dist = sqrt(dist);
+  return dist <= S.h[si] * J.cut_off + 1.e-10 || dist <= S.h[q] * J.cut_off + 1.e-10;
+}
+
+// a 16-byte group of eight 16-bit items as two 64-bit halves (consumed by shifting,
+// so that no item is picked by a dynamic index)
+struct Group8 {
+  uint64_t lo, hi;
+};
+__device__ __forceinline__ Group8 group_of(const uint4& g) {
+  return {uint64_t(g.x) | (uint64_t(g.y) << 32), uint64_t(g.z) | (uint64_t(g.w) << 32)};
+}
+
+// x_i - x_q from the hi / lo pairs (the image already applied to q's record)
+template <int D>
+__device__ __forceinline__ void rel_hl(const float* xhi, const float* xli, const float4& qh, const float4& ql,
+                                       float* dr) {
+  dr[0] = (xhi[0] - qh.x) + (xli[0] - ql.x);
+  if (D > 1) dr[1] = (xhi[1] - qh.y) + (xli[1] - ql.y);
+  if (D > 2) dr[2] = (xhi[2] - qh.z) + (xli[2] - ql.z);
+}
+
+// ---------------------------------------------------------------------------
+// P3: kernel summation + stage-1 force + stage-1 dt
+
+// the particle a pair pass is summing for
+struct Own {
+  float xh[3], xl[3];
+  float h, ih, Ck, kd, vol2i, vol, eta, rf;
+  float kdx, cth;   // the job's cut_off / cell_ratio and cos(angle_thres), in registers
+  float n[3], v[3];
+  bool surf, jump;
+};
+struct Acc1 {
+  float sum, g[3];
+  int nc, ntr;
+};
 
 template <int D>
+__device__ __forceinline__ void own_common(const JobParams& J, const float4& rh, const float4& rl, const float* n, Own& o) {
+  const int tw = __float_as_int(rl.w);
+  const int type = tw & ~(kJumpBit | int(kHMir));
+  o.xh[0] = rh.x; o.xh[1] = rh.y; o.xh[2] = rh.z;
+  o.xl[0] = rl.x; o.xl[1] = rl.y; o.xl[2] = rl.z;
+  o.h = rh.w;
+  o.surf = type == SMG_SURFACE;
+  o.jump = (tw & kJumpBit) != 0;
+#pragma unroll
+  for (int a = 0; a < 3; a++) o.n[a] = n[a];
+  const float dxi = o.h * J.kdx;
+  o.vol = ipow<D>(dxi);
+  o.vol2i = o.vol * o.vol;
+  o.ih = 1.f / o.h;
+  const float Cw = kC<D>() * ipow<D>(o.ih), Cs = kC<D - 1>() * ipow<D - 1>(o.ih);
+  o.Ck = o.surf ? Cs : Cw;
+  o.kd = o.surf ? kC<D - 1>() : kC<D>();
+  o.rf = o.h * float(J.cut_off);
+  o.kdx = J.kdx;
+  o.cth = J.cos_thres;
+}
+
+// One stage-1 pair (Get_kernel_summation and the stage-1 Accumulate_particle_force_real
+// / _surf, particle_mesh_generation.cpp:690-765, 883-1118) of a verified neighbour:
+// dr0 = x_i - x_q, d2 = |dr0|^2.  A surface particle uses the (D-1)-dimensional kernel
+// on the tangential part of dr (raw normal, Q3) and only surface neighbours with
+// aligned normals.  One straight-line body for both types: a REAL particle has n = 0,
+// so the projection leaves dr and d2 unchanged bit for bit.  Returns whether the pair
+// enters the stage-2 pass (a non-self neighbour the particle uses).
+template <int D>
+__device__ __forceinline__ bool pair1(const Work& W, const JobParams& J, const Own& o, bool tsurf, const float* dr0,
+                                      float d2, float hq, int tq, bool self, int64_t q, Acc1& acc, bool valid = true) {
+  // valid = false: the pair is not a neighbour (the sums are left as they are)
+  acc.ntr += valid ? 1 : 0;
+  bool use = valid && !(o.surf && tq == SMG_REAL);
+  if (use && o.surf && tq == SMG_SURFACE) {
+    const float4 qn = W.rec[4 * q + 3];
+    const float nq[3] = {qn.x, qn.y, qn.z};
+    float ca = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; a++) ca += o.n[a] * nq[a];
+    use = fabsf(ca) > o.cth;
+  }
+  float dr[3] = {dr0[0], dr0[1], dr0[2]};
+  float rt = d2;
+  if (tsurf) {
+    float dn = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; a++) dn += dr0[a] * o.n[a];
+    rt = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      dr[a] = dr0[a] - o.n[a] * dn;
+      rt += dr[a] * dr[a];
+    }
+  }
+  rt = fsqrt(rt);
+  const bool counts = use && (!o.surf || tq == SMG_SURFACE);   // the kernel summation
+  const float w = Wk_s<D>(rt * o.ih, o.Ck);
+  acc.sum = counts ? acc.sum + w : acc.sum;
+  acc.nc += counts ? 1 : 0;
+  const float dxq = hq * o.kdx;
+  const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
+  const float ihm = frcp(0.5f * (o.h + hq));
+  const float C = o.kd * (ipow<D>(ihm) * (o.surf ? 1.f : ihm));
+  const float s2 = rt * ihm;
+  const float t = fmaxf(2.f - s2, 0.f);
+  const float dw = (s2 < 1.f ? C * (-6.0f + 3.f * s2 * s2) : C * (-3.0f) * t * t) * frcp(rt + 1.e-20f);
+  const float aw = dw * (o.vol2i + vol2q);
+  const bool f = use && !self;
+#pragma unroll
+  for (int a = 0; a < D; a++) acc.g[a] = f ? acc.g[a] + dr[a] * aw : acc.g[a];
+  return f;
+}
+
+// One workgroup per tile; thread t sums for the tile's particle of rank t in the order
+// of list lengths (so each wavefront walks lists of about the same length), over its
+// items in list order, then the pairs with this step's jumpers.  Records come from the
+// tile's halo, staged in LDS (the first kHL1 entries; the rest read from the records).
+template <int D>
 __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
+  __shared__ float4 hh[kHL1];
+  __shared__ float4 hl[kHL1];
   __shared__ double lds_d[4];
   __shared__ int lds_i[4];
+  __shared__ int s_surf;
   __shared__ JumpTile jt;
-  const int j = W.tile_job[tile_id(W)];
+  const int64_t tile = tile_id(W);
+  const int j = W.tile_job[tile];
   if (j < 0) return;
   const JobParams& J = W.jp[j];
-  const int64_t slot = tile_id(W) * kTile + threadIdx.x;
+  const int64_t ts = tile * kTile;
+  // the particle of this thread
+  const int p = W.lane[ts + threadIdx.x];
+  const int64_t slot = ts + p;
   const bool live = slot - J.p0 < J.n;
   const float4 r0 = W.rec[4 * slot + 0], r1 = W.rec[4 * slot + 1];
   const float xhi[3] = {r0.x, r0.y, r0.z};
-  jumper_prepare<D>(W, J, j, live, xhi, jt);
+  if (threadIdx.x == 0) s_surf = 0;
+  jumper_prepare<D>(W, J, j, live, xhi, jt);   // ends with a barrier
+  const int H = W.hcnt[tile];
+  const int HS = H < kHL1 ? H : kHL1;
+  for (int e = threadIdx.x; e < HS; e += kTile) {
+    int64_t q;
+    uint32_t code;
+    halo_entry(W, tile, ts, uint32_t(e), q, code);
+    float4 rh, rl;
+    halo_rec<D>(W, J, q, code, rh, rl);
+    hh[e] = rh;
+    hl[e] = rl;
+  }
+  if (live && (__float_as_int(r1.w) & ~kJumpBit) == SMG_SURFACE) s_surf = 1;
+  __syncthreads();
+  const bool tsurf = s_surf != 0;
   double dt = 1.0e300;
   int ntr = 0;
   if (live) {
     float ni[3] = {0.f, 0.f, 0.f};
-    const double hd = S.h[slot];
-    const float h = r0.w;
-    const int type = __float_as_int(r1.w) & ~kJumpBit;
-    const bool surf = type == SMG_SURFACE;
-    if (surf) {
+    if ((__float_as_int(r1.w) & ~kJumpBit) == SMG_SURFACE) {
 #pragma unroll
-      for (int a = 0; a < D; a++) ni[a] = S.nrm[a][slot];
+      for (int c = 0; c < D; c++) ni[c] = S.nrm[c][slot];
     }
-    const float nub = W.nu_b[slot];   // loaded early: stored with a1 after the loop
-    const float dxi = h * J.kdx;
-    const float vol2i = ipow<D>(dxi) * ipow<D>(dxi);
-    const float ih = 1.f / h;
-    const float Cw = kC<D>() * ipow<D>(ih), Cs = kC<D - 1>() * ipow<D - 1>(ih);
-    // Get_kernel_summation (particle_mesh_generation.cpp:690-765) and the stage-1
-    // Accumulate_particle_force_real / _surf in one pass: the force is
-    // -sum_i * sum_j (vol_i^2 + vol_j^2) dW dr with sum_i known only at the end
-    float sum = 0.f;
-    int ncount = 0;
-    float G[3] = {0.f, 0.f, 0.f};
-    uint64_t mask[kMaskWords];
+    Own o;
+    own_common<D>(J, r0, r1, ni, o);
+    const float nub = W.nu_b[slot];   // stored with a1 below
+    Acc1 acc = {};
+    const float cutf = float(J.cut_off);
+    if (!o.jump) {   // a jumper's list is stale: it walks its own list below
+      const int cnt = H > 0 ? W.nbc[slot] : 0;   // hcnt = 0: the tile's build failed
+      const int64_t i0 = tile * kTile * W.ips + W.ioff[slot];
+      // items eight to a 16-byte group, groups 64 apart (the wavefront's lanes between)
+      const uint4* it = reinterpret_cast<const uint4*>(W.items + i0);
+      uint16_t* st2 = W.items2 + i0;   // the same group layout
+      int c2 = 0;
+      // One body, predicated rather than branched (a whole wavefront almost never skips
+      // an item together); a tile whose halo fits the LDS never reads a record from
+      // memory inside the loop, so the next group's load stays in flight across it.
+      auto run = [&](auto fits) __attribute__((always_inline)) {
+        constexpr bool kFits = decltype(fits)::value;
+        const int ng = (cnt + 7) >> 3;
+        uint4 gn = it[0];   // inside the tile's items even when cnt = 0
+        for (int g = 0; g < ng; g++) {
+          Group8 gv = group_of(gn);
+          // the group is taken into registers here, so the wait for its load stays out of
+          // the item loop (which then never waits on memory: its stores go unawaited)
+          asm volatile("" : "+v"(gv.lo), "+v"(gv.hi));
+          gn = it[(g + 1 < ng ? g + 1 : g) * 64];
+          const int kn = cnt - 8 * g < 8 ? cnt - 8 * g : 8;
+          for (int u = 0; u < kn; u++) {
+            const uint32_t e = uint32_t(gv.lo) & 0xFFFFu;   // the group's next item
+            gv.lo = (gv.lo >> 16) | (gv.hi << 48);
+            gv.hi >>= 16;
+            float4 qh, ql;
+            int64_t q = -1;
+            uint32_t code = 0u;
+            if (kFits || e < uint32_t(HS)) {
+              qh = hh[e];
+              ql = hl[e];
+            } else {   // past the staged records (a tile whose halo exceeds the LDS)
+              halo_entry(W, tile, ts, e, q, code);
+              halo_rec<D>(W, J, q, code, qh, ql);
+            }
+            const int tw = __float_as_int(ql.w);
+            float dr[3] = {0.f, 0.f, 0.f};
+            rel_hl<D>(o.xh, o.xl, qh, ql, dr);
+            float d2 = 0.f;
 #pragma unroll
-    for (int w = 0; w < kMaskWords; w++) mask[w] = 0ull;
-    // One straight-line body for both particle types.  A REAL particle has ni = 0, so
-    // its "projected" dr is dr itself and rt^2 = d2 bit for bit; a surface particle
-    // uses the (D-1)-dimensional kernel, whose constant kC<D-1> ih^D equals the D form's
-    // kC<D> ih^(D+1) with one factor ih dropped.
-    const float Ck = surf ? Cs : Cw;
-    const float kd = surf ? kC<D - 1>() : kC<D>();
-    const bool wave_surf = __ballot(surf) != 0ull;   // wave-uniform: skip the projection
-    for_each_neighbor<D>(W, S, J, slot, jt, [&](int k, int64_t q, uint32_t code, const float4& qh, const float4& ql,
-                                                 const float* dr0, float d2) {
-      ntr++;
-      const int tq = __float_as_int(ql.w) & ~kJumpBit;
-      const bool self = q == slot && code == 0;
-      // list entries the stage-2 pass visits: real neighbours other than itself, and
-      // for a surface particle not the REAL ones (jumper pairs it finds itself)
-      if (k >= 0 && k < 64 * kMaskWords && code == 0 && !self && !(surf && tq == SMG_REAL)) {
-#pragma unroll
-        for (int w = 0; w < kMaskWords; w++)
-          if ((k >> 6) == w) mask[w] |= 1ull << (k & 63);
-      }
-      // a surface particle sees surface neighbours only (Accumulate_particle_force_surf),
-      // and of those only the ones with aligned normals
-      bool use = !(surf && tq == SMG_REAL);
-      if (surf && tq == SMG_SURFACE) {
-        const float4 qn = W.rec[4 * q + 3];
-        const float nq[3] = {qn.x, qn.y, qn.z};
-        float ca = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; a++) ca += ni[a] * nq[a];
-        use = fabsf(ca) > J.cos_thres;
-      }
-      float dr[3] = {dr0[0], dr0[1], dr0[2]};
-      float rt = d2;
-      if (wave_surf) {
-        float dn = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; a++) dn += dr0[a] * ni[a];
-        rt = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; a++) {
-          dr[a] = dr0[a] - ni[a] * dn;
-          rt += dr[a] * dr[a];
+            for (int c = 0; c < D; c++) d2 += dr[c] * dr[c];
+            // The FP32 distance of the compensated relative position is within 3 2^-24 d
+            // (+ 1e-13) of the FP64 one and r within 2^-23 r; the FP64 predicate decides
+            // inside 1e-6 r + 2e-10 of the cutoff (its own +1e-10 included).
+            const float r = fmaxf(o.rf, qh.w * cutf);
+            const float bd = 1.e-6f * r + 2.e-10f;
+            const float rlo = r - bd, rhi = r + bd;
+            bool nb = d2 < rlo * rlo;
+            if (!nb && d2 <= rhi * rhi) {
+              if (q < 0) halo_entry(W, tile, ts, e, q, code);
+              nb = exact_pred<D>(W, J, S, slot, q, code);
+            }
+            nb = nb && !(tw & kJumpBit);   // a jumper: paired through the jumper scan
+            const int tq = tw & ~(kJumpBit | int(kHMir));
+            if (nb && o.surf && tq == SMG_SURFACE && q < 0) halo_entry(W, tile, ts, e, q, code);
+            const bool f = pair1<D>(W, J, o, tsurf, dr, d2, qh.w, tq, e == uint32_t(p), q, acc, nb);
+            if (f && !(tw & int(kHMir))) {
+              st2[(c2 >> 3) * 512 + (c2 & 7)] = uint16_t(e);
+              c2++;
+            }
+          }
         }
-      }
-      rt = fsqrt(rt);
-      const bool counts = use && (!surf || tq == SMG_SURFACE);   // the kernel summation
-      const float w = Wk_s<D>(rt * ih, Ck);
-      sum = counts ? sum + w : sum;
-      ncount += counts ? 1 : 0;
-      const float hq = qh.w;
-      const float dxq = hq * J.kdx;
-      const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
-      const float hm = 0.5f * (h + hq);
-      const float ihm = frcp(hm);
-      const float C = kd * (ipow<D>(ihm) * (surf ? 1.f : ihm));
-      const float s2 = rt * ihm;
-      const float t = fmaxf(2.f - s2, 0.f);
-      const float dw = (s2 < 1.f ? C * (-6.0f + 3.f * s2 * s2) : C * (-3.0f) * t * t) * frcp(rt + 1.e-20f);
-      const float aw = dw * (vol2i + vol2q);
-      const bool f = use && !self;
-#pragma unroll
-      for (int a = 0; a < D; a++) G[a] = f ? G[a] + dr[a] * aw : G[a];
-    });
-    if (ncount > 1) sum = 1.f / (sum + 1.e-15f);
+      };
+      if (H <= kHL1) run(std::true_type{});
+      else run(std::false_type{});
+      W.nbc2[slot] = c2;
+    } else {
+      W.nbc2[slot] = 0;
+    }
+    // pairs with this step's jumpers (and, for a jumper, its own list), after the items
+    for_each_neighbor<D>(W, S, J, slot, jt, [&](int, int64_t q, uint32_t code, const float4& qh, const float4& ql,
+                                                 const float* dr0, float d2) {
+      pair1<D>(W, J, o, tsurf, dr0, d2, qh.w, __float_as_int(ql.w) & ~kJumpBit, q == slot && code == 0u, q, acc);
+    }, true);
+    float sum = acc.sum;
+    if (acc.nc > 1) sum = 1.f / (sum + 1.e-15f);
     W.sum[slot] = sum;
-    W.ntrue[slot] = ntr;
-#pragma unroll
-    for (int w = 0; w < kMaskWords; w++) W.nbm[int64_t(w) * W.cap_total + slot] = mask[w];
+    W.ntrue[slot] = acc.ntr;
+    ntr = acc.ntr;
     float F[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int a = 0; a < D; a++) F[a] = -sum * G[a];
-    if (surf) {   // Map_surface_particle_force with the raw normal
+    for (int c = 0; c < D; c++) F[c] = -sum * acc.g[c];
+    if (o.surf) {   // Map_surface_particle_force with the raw normal
       float nf = 0.f;
 #pragma unroll
-      for (int a = 0; a < D; a++) nf += F[a] * ni[a];
+      for (int c = 0; c < D; c++) nf += F[c] * ni[c];
 #pragma unroll
-      for (int a = 0; a < D; a++) F[a] = F[a] - ni[a] * nf;
+      for (int c = 0; c < D; c++) F[c] = F[c] - ni[c] * nf;
     }
     double A = 0.;
 #pragma unroll
-    for (int a = 0; a < D; a++) {
-      W.a1[a][slot] = F[a];
-      A += double(F[a]) * double(F[a]);
+    for (int c = 0; c < D; c++) {
+      W.a1[c][slot] = F[c];
+      A += double(F[c]) * double(F[c]);
     }
     W.rec[4 * slot + 2] = make_float4(F[0], F[1], D > 2 ? F[2] : 0.f, nub);
     A = sqrt(A);
     // Set_timestep(1): v was zeroed by Reset_particle_force(1), so U = 0
+    const double hd = S.h[slot];
     const double t1 = 0.25 * sqrt(hd / (A + 1.e-15));
     const double t2 = hd / 40. / (0. + 1.e-15);
     dt = amin(t1, t2);
@@ -1636,218 +2015,192 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
 // ---------------------------------------------------------------------------
 // P4: stage-2 (viscous) force, stage-2 dt and nu
 
+// One stage-2 pair (Accumulate_particle_force_real / _surf, stage 2,
+// particle_mesh_generation.cpp:995-1118): eta_ij the harmonic mean of nu rho, a
+// non-REAL neighbour of a REAL particle moving at -v_i (Q8); dr0 = x_i - x_q, rr = |dr0|^2.
 template <int D>
-__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_force2(Work W, PState S, int par) {
-  __shared__ double lds_d[4];
-  __shared__ JumpTile jt;
-  const int j = W.tile_job[tile_id(W)];
-  if (j < 0) return;
-  const JobParams& J = W.jp[j];
-  const int64_t slot = tile_id(W) * kTile + threadIdx.x;
-  const bool live = slot - J.p0 < J.n;
-  {
-    const float4 rb = W.rec[4 * slot];
-    const float xb[3] = {rb.x, rb.y, rb.z};
-    jumper_prepare<D>(W, J, j, live, xb, jt);
-  }
-  double dt = 1.0e300, U = 0.;
-  if (live) {
-    const double dt1 = acc_dt(W, j, par, false);
-    const float hdt = float(0.5 * dt1);
-    const float4 r0 = W.rec[4 * int64_t(slot) + 0], r1 = W.rec[4 * int64_t(slot) + 1], r2 = W.rec[4 * int64_t(slot) + 2];
-    const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
-    const float a1i[3] = {r2.x, r2.y, r2.z};
-    float ni[3] = {0.f, 0.f, 0.f}, vi[3] = {0.f, 0.f, 0.f};
-    double vd[3] = {0., 0., 0.};
+__device__ __forceinline__ void pair2(const JobParams& J, const Own& o, bool tsurf, float hdt, const float* dr0, float rr,
+                                      float hq, int tq, const float4& qv, bool use, float* F) {
+  const float dxq = hq * o.kdx;
+  const float volq = ipow<D>(dxq);
+  const float rhoq = frcp(volq);
+  const float etaq = qv.w * rhoq;
+  // eta_ij = 0; and a pair at zero distance (particles seeded onto the same clamped
+  // point) adds nothing: its term is (v_i - v_j) dW(0) / 1e-20 with v_i = v_j, both
+  // sums of the same terms in the same order
+  use = use && etaq != 0.f && rr > 0.f;
+  float rt = rr;
+  if (tsurf) {
+    float dn = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; a++) dn += dr0[a] * o.n[a];
+    rt = 0.f;
 #pragma unroll
     for (int a = 0; a < D; a++) {
-      vi[a] = a1i[a] * hdt;
-      vd[a] = double(a1i[a]) * (0.5 * dt1);
+      const float t = dr0[a] - o.n[a] * dn;
+      rt += t * t;
     }
-    const double hd = S.h[slot];
-    const float h = r0.w;
-    const float rfi = float(hd * J.cut_off);
-    const int type = __float_as_int(r1.w) & ~kJumpBit;
-    const bool ijump = __float_as_int(r1.w) & kJumpBit;
-    const bool surf = type == SMG_SURFACE;
-    if (surf) {
+  }
+  rt = fsqrt(rt);
+  const float ihm = frcp(0.5f * (o.h + hq));
+  const float C = o.kd * (ipow<D>(ihm) * (o.surf ? 1.f : ihm));
+  const float s2 = rt * ihm;
+  const float t2 = fmaxf(2.f - s2, 0.f);
+  const float aw = (s2 < 1.f ? C * (-6.0f + 3.f * s2 * s2) : C * (-3.0f) * t2 * t2) * frcp(rt + 1.e-20f);
+  const bool mirror_v = !o.surf && tq != SMG_REAL;   // Q8: v_j := -v_i
+  const float etaij = 2.f * etaq * o.eta * frcp(o.eta + etaq + 1.e-15f);
+  const float temp0 = aw * etaij * (o.vol * o.vol + volq * volq);
 #pragma unroll
-      for (int a = 0; a < D; a++) ni[a] = S.nrm[a][slot];
+  for (int a = 0; a < D; a++) {
+    const float qa = a == 0 ? qv.x : (a == 1 ? qv.y : qv.z);
+    const float vq = mirror_v ? -o.v[a] : qa * hdt;
+    F[a] = use ? F[a] + (o.v[a] - vq) * temp0 : F[a];
+  }
+}
+
+// The stage-2 pairs the stage-1 pass kept (verified, unmirrored, not self), with
+// staged records carrying (a1, nu of the previous step) for the neighbour's velocity
+// and viscosity; the same thread-to-particle order as the stage-1 pass.
+template <int D>
+__global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
+  __shared__ float4 hh[kHL2];
+  __shared__ float4 hl[kHL2];
+  __shared__ float4 hv[kHL2];
+  __shared__ double lds_d[4];
+  __shared__ int s_surf;
+  __shared__ JumpTile jt;
+  const int64_t tile = tile_id(W);
+  const int j = W.tile_job[tile];
+  if (j < 0) return;
+  const JobParams& J = W.jp[j];
+  const int64_t ts = tile * kTile;
+  const int p = W.lane[ts + threadIdx.x];
+  const int64_t slot = ts + p;
+  const bool live = slot - J.p0 < J.n;
+  const float4 r0 = W.rec[4 * slot + 0], r1 = W.rec[4 * slot + 1], r2 = W.rec[4 * slot + 2];
+  {
+    const float xb[3] = {r0.x, r0.y, r0.z};
+    if (threadIdx.x == 0) s_surf = 0;
+    jumper_prepare<D>(W, J, j, live, xb, jt);   // ends with a barrier
+  }
+  const double dt1 = acc_dt(W, j, par, false);
+  const float hdt = float(0.5 * dt1);
+  const int H = W.hcnt[tile];
+  const int HS = H < kHL2 ? H : kHL2;
+  for (int e = threadIdx.x; e < HS; e += kTile) {
+    int64_t q;
+    uint32_t code;
+    halo_entry(W, tile, ts, uint32_t(e), q, code);
+    float4 rh, rl;
+    halo_rec<D>(W, J, q, code, rh, rl);
+    hh[e] = rh;
+    hl[e] = rl;
+    hv[e] = W.rec[4 * q + 2];
+  }
+  if (live && (__float_as_int(r1.w) & ~kJumpBit) == SMG_SURFACE) s_surf = 1;
+  __syncthreads();
+  const bool tsurf = s_surf != 0;
+  double dt = 1.0e300, U = 0.;
+  if (live) {
+    float ni[3] = {0.f, 0.f, 0.f};
+    if ((__float_as_int(r1.w) & ~kJumpBit) == SMG_SURFACE) {
+#pragma unroll
+      for (int c = 0; c < D; c++) ni[c] = S.nrm[c][slot];
     }
-    const int cnt = W.nbc[slot];
-    const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
-    const float dxi = h * J.kdx;
-    const float voli = ipow<D>(dxi);
-    const float rhoi = 1.f / voli;
-    const float etai = r2.w * rhoi;
+    Own o;
+    own_common<D>(J, r0, r1, ni, o);
+    o.eta = r2.w * (1.f / o.vol);
+    const float a1i[3] = {r2.x, r2.y, r2.z};
+    double vd[3] = {0., 0., 0.};
+#pragma unroll
+    for (int c = 0; c < 3; c++) o.v[c] = 0.f;
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      o.v[c] = a1i[c] * hdt;
+      vd[c] = double(a1i[c]) * (0.5 * dt1);
+    }
     float F[3] = {0.f, 0.f, 0.f};
-    // one stage-2 pair term; checked = a list entry that the stage-1 masks marked as a
-    // real (not mirrored), non-self neighbour that a surface particle keeps
-    // one straight-line body for both particle types, as in the stage-1 pass: with
-    // ni = 0 a REAL particle's projected rt^2 is its rr bit for bit
-    const float kd = surf ? kC<D - 1>() : kC<D>();
-    const bool wave_surf = __ballot(surf) != 0ull;   // wave-uniform: skip the projection
-    auto pair_rec = [&](int64_t q, uint32_t code, bool checked, const float4& qh, const float4& ql,
-                        const float4& qv) {
-      const int tq = __float_as_int(ql.w) & ~kJumpBit;
-      // ghosts carry nu = 0 (Q6); a surface particle sees no REAL neighbour
-      bool use = checked || (code == 0 && q != slot && !(surf && tq == SMG_REAL));
-      const float hq = qh.w;
-      const float dxq = hq * J.kdx;
-      const float volq = ipow<D>(dxq);
-      const float rhoq = frcp(volq);
-      const float etaq = qv.w * rhoq;
-      use = use && etaq != 0.f;   // eta_ij = 0
-      float dr[3] = {0.f, 0.f, 0.f};
-      rel_pos<D>(J, qh, ql, 0u, xhi, xli, dr);
-      float rr = 0.f;
-#pragma unroll
-      for (int a = 0; a < D; a++) rr += dr[a] * dr[a];
-      if (!checked && use) use = is_neighbor<D>(J, W, S, q, 0u, rr, rfi, hq, xhi, xli, hd);
-      if (surf && tq == SMG_SURFACE) {
-        const float4 qn = W.rec[4 * q + 3];
-        const float nq[3] = {qn.x, qn.y, qn.z};
-        float ca = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; a++) ca += ni[a] * nq[a];
-        use = use && fabsf(ca) > J.cos_thres;
-      }
-      float rt = rr;
-      if (wave_surf) {
-        float dn = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; a++) dn += dr[a] * ni[a];
-        rt = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; a++) {
-          const float t = dr[a] - ni[a] * dn;
-          rt += t * t;
-        }
-      }
-      rt = fsqrt(rt);
-      const float ihm = frcp(0.5f * (h + hq));
-      const float C = kd * (ipow<D>(ihm) * (surf ? 1.f : ihm));
-      const float s2 = rt * ihm;
-      const float t2 = fmaxf(2.f - s2, 0.f);
-      const float aw = (s2 < 1.f ? C * (-6.0f + 3.f * s2 * s2) : C * (-3.0f) * t2 * t2) * frcp(rt + 1.e-20f);
-      const bool mirror_v = !surf && tq != SMG_REAL;   // Q8: v_j := -v_i
-      const float etaij = 2.f * etaq * etai * frcp(etai + etaq + 1.e-15f);
-      const float temp0 = aw * etaij * (voli * voli + volq * volq);
-#pragma unroll
-      for (int a = 0; a < D; a++) {
-        const float qa = a == 0 ? qv.x : (a == 1 ? qv.y : qv.z);
-        const float vq = mirror_v ? -vi[a] : qa * hdt;
-        F[a] = use ? F[a] + (vi[a] - vq) * temp0 : F[a];
-      }
-    };
-    auto pair = [&](int64_t q, uint32_t code, bool checked) {
-      pair_rec(q, code, checked, W.rec[4 * q + 0], W.rec[4 * q + 1], W.rec[4 * q + 2]);
-    };
-    if (etai != 0.f) {   // eta_ij = 0 for every pair when this particle's nu is 0
-      auto pair_any = [&](int, int64_t q, uint32_t code, const float4&, const float4&, const float*, float) {
-        pair(q, code, false);
-      };
-      if (!ijump) {
-        // the masked entries by groups of four (k = 4g .. 4g + 3): the group of the
-        // lowest remaining mask bit, with its four mask bits taken out of m0 / m1; the
-        // records of a group's masked entries are gathered together, then the next
-        // group's load is issued while this group is used
-        static_assert(kMaskWords == 2, "two mask words");
-        uint64_t m0 = W.nbm[slot];
-        uint64_t m1 = 64 < cnt ? W.nbm[W.cap_total + slot] : 0ull;
-        auto next_group = [&](uint32_t& nib) -> int {
-          int g = -1;
-          if (m0) {
-            g = __builtin_ctzll(m0) >> 2;
-            nib = uint32_t(m0 >> (4 * g)) & 0xFu;
-            m0 &= ~(0xFull << (4 * g));
-          } else if (m1) {
-            const int g1 = __builtin_ctzll(m1) >> 2;
-            nib = uint32_t(m1 >> (4 * g1)) & 0xFu;
-            m1 &= ~(0xFull << (4 * g1));
-            g = 16 + g1;
-          }
-          return g;
-        };
-        auto load_group = [&](int g, uint32_t nib) -> uint4 {
-          if (4 * g < W.K) return nb_group(W, slot, g);
-          uint32_t ev[4] = {0u, 0u, 0u, 0u};   // a spilled group (K < 128): its masked entries
-#pragma unroll
-          for (int u = 0; u < 4; u++)
-            if ((nib >> u) & 1u) ev[u] = uint32_t(W.spill[int64_t(off) + (4 * g + u - W.K)]);
-          return make_uint4(ev[0], ev[1], ev[2], ev[3]);
-        };
-        uint32_t nibn = 0u;
-        int gn = next_group(nibn);
-        uint4 en = make_uint4(0u, 0u, 0u, 0u);
-        if (gn >= 0) en = load_group(gn, nibn);
-        while (gn >= 0) {
-          const uint32_t nib = nibn;
-          const uint32_t ev[4] = {en.x, en.y, en.z, en.w};
-          float4 qh[4], ql[4], qv[4];
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            qh[u] = ql[u] = qv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if ((nib >> u) & 1u) {
-              const int64_t q = ev[u] & kJMask;
-              qh[u] = W.rec[4 * q + 0];
-              ql[u] = W.rec[4 * q + 1];
-              qv[u] = W.rec[4 * q + 2];
+    if (o.eta != 0.f) {   // eta_ij = 0 for every pair when this particle's nu is 0
+      const int cnt = W.nbc2[slot];
+      const uint4* it = reinterpret_cast<const uint4*>(W.items2 + tile * kTile * W.ips + W.ioff[slot]);
+      auto run = [&](auto fits) __attribute__((always_inline)) {
+        constexpr bool kFits = decltype(fits)::value;
+        const int ng = (cnt + 7) >> 3;
+        uint4 gn = it[0];   // inside the tile's items even when cnt = 0
+        for (int g = 0; g < ng; g++) {
+          Group8 gv = group_of(gn);
+          // the group is taken into registers here, so the wait for its load stays out of
+          // the item loop (which then never waits on memory: its stores go unawaited)
+          asm volatile("" : "+v"(gv.lo), "+v"(gv.hi));
+          gn = it[(g + 1 < ng ? g + 1 : g) * 64];
+          const int kn = cnt - 8 * g < 8 ? cnt - 8 * g : 8;
+          for (int u = 0; u < kn; u++) {
+            const uint32_t e = uint32_t(gv.lo) & 0xFFFFu;   // the group's next item
+            gv.lo = (gv.lo >> 16) | (gv.hi << 48);
+            gv.hi >>= 16;
+            float4 qh, ql, qv;
+            if (kFits || e < uint32_t(HS)) {
+              qh = hh[e];
+              ql = hl[e];
+              qv = hv[e];
+            } else {
+              int64_t q;
+              uint32_t code;
+              halo_entry(W, tile, ts, e, q, code);
+              halo_rec<D>(W, J, q, code, qh, ql);
+              qv = W.rec[4 * q + 2];
             }
-          }
-          gn = next_group(nibn);
-          if (gn >= 0) en = load_group(gn, nibn);
+            float dr[3] = {0.f, 0.f, 0.f};
+            rel_hl<D>(o.xh, o.xl, qh, ql, dr);
+            float rr = 0.f;
 #pragma unroll
-          for (int u = 0; u < 4; u++)
-            if ((nib >> u) & 1u) pair_rec(ev[u] & kJMask, ev[u] >> kJBits, true, qh[u], ql[u], qv[u]);
-        }
-        // entries past the masks: the inline ones four at a time (one 16-byte load per
-        // group, the records gathered together), then the spilled tail
-        const int kin = cnt < W.K ? cnt : W.K;
-        for (int g = 16 * kMaskWords; 4 * g < kin; g++) {
-          const uint4 gv = nb_group(W, slot, g);
-          const uint32_t ev[4] = {gv.x, gv.y, gv.z, gv.w};
-          float4 qh[4], ql[4], qv[4];
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            // past the end a lane loads its own record and skips it
-            const int64_t q = 4 * g + u < kin ? int64_t(ev[u] & kJMask) : slot;
-            qh[u] = W.rec[4 * q + 0];
-            ql[u] = W.rec[4 * q + 1];
-            qv[u] = W.rec[4 * q + 2];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            if (4 * g + u >= kin || (__float_as_int(ql[u].w) & kJumpBit)) continue;
-            pair_rec(ev[u] & kJMask, ev[u] >> kJBits, false, qh[u], ql[u], qv[u]);
+            for (int c = 0; c < D; c++) rr += dr[c] * dr[c];
+            pair2<D>(J, o, tsurf, hdt, dr, rr, qh.w, __float_as_int(ql.w) & ~(kJumpBit | int(kHMir)), qv, true, F);
           }
         }
-        for (int k = kin > 64 * kMaskWords ? kin : 64 * kMaskWords; k < cnt; k++) {
-          const uint32_t e = nb_entry(W, slot, k, off);
-          if (__float_as_int(W.rec[4 * int64_t(e & kJMask) + 1].w) & kJumpBit) continue;
-          pair(e & kJMask, e >> kJBits, false);
+      };
+      if (H <= kHL2) run(std::true_type{});
+      else run(std::false_type{});
+      // pairs with this step's jumpers (and, for a jumper, its own list), after the items
+      const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
+      for_each_neighbor<D>(W, S, J, slot, jt, [&](int, int64_t q, uint32_t code, const float4& qh, const float4& ql,
+                                                   const float*, float) {
+        const int tq = __float_as_int(ql.w) & ~kJumpBit;
+        const bool use = code == 0u && q != slot && !(o.surf && tq == SMG_REAL);
+        bool aligned = true;
+        if (use && o.surf && tq == SMG_SURFACE) {
+          const float4 qn = W.rec[4 * q + 3];
+          const float nq[3] = {qn.x, qn.y, qn.z};
+          float ca = 0.f;
+#pragma unroll
+          for (int c = 0; c < D; c++) ca += ni[c] * nq[c];
+          aligned = fabsf(ca) > J.cos_thres;
         }
-        for_each_neighbor<D>(W, S, J, slot, jt, pair_any, true);   // the jumpers
-      } else {
-        for_each_neighbor<D>(W, S, J, slot, jt, pair_any);
-      }
+        float dr[3] = {0.f, 0.f, 0.f};
+        rel_pos<D>(J, qh, ql, 0u, xhi, xli, dr);
+        float rr = 0.f;
+#pragma unroll
+        for (int c = 0; c < D; c++) rr += dr[c] * dr[c];
+        pair2<D>(J, o, tsurf, hdt, dr, rr, qh.w, tq, W.rec[4 * q + 2], use && aligned, F);
+      }, true);
     }
-    if (surf) {
+    if (o.surf) {
       float nf = 0.f;
 #pragma unroll
-      for (int a = 0; a < D; a++) nf += F[a] * ni[a];
+      for (int c = 0; c < D; c++) nf += F[c] * ni[c];
 #pragma unroll
-      for (int a = 0; a < D; a++) F[a] = F[a] - ni[a] * nf;
+      for (int c = 0; c < D; c++) F[c] = F[c] - ni[c] * nf;
     }
     double A = 0.;
 #pragma unroll
-    for (int a = 0; a < D; a++) {
-      W.a2[a][slot] = F[a];
-      A += double(F[a]) * double(F[a]);
-      U += vd[a] * vd[a];
+    for (int c = 0; c < D; c++) {
+      W.a2[c][slot] = F[c];
+      A += double(F[c]) * double(F[c]);
+      U += vd[c] * vd[c];
     }
     A = sqrt(A);
     U = sqrt(U);
+    const double hd = S.h[slot];
     double t = 0.25 * sqrt(hd / (A + 1.e-15));
     t = amin(t, hd / 40. / (U + 1.e-15));
     const double nu = J.nu_coeff * hd * U;
@@ -2109,6 +2462,10 @@ void launch_nb_spill(hipStream_t st, int D, int64_t ntiles, const Work& W, const
   const int64_t g = ntiles < 1024 ? ntiles : 1024;   // grid-stride over the (usually empty) queue
   if (D == 2) k_nb_spill<2><<<g, kTile, 0, st>>>(W, S);
   else k_nb_spill<3><<<g, kTile, 0, st>>>(W, S);
+}
+void launch_halo(hipStream_t st, int D, int64_t ntiles, const Work& W) {
+  if (D == 2) k_halo<2><<<ntiles, kTile, 0, st>>>(W);
+  else k_halo<3><<<ntiles, kTile, 0, st>>>(W);
 }
 void launch_force1(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int par) {
   if (D == 2) k_force1<2><<<ntiles, kTile, 0, st>>>(W, S, par);
