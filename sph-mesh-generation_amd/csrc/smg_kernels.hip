@@ -1572,16 +1572,15 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W) {
     rank += (cq < cnt || (cq == cnt && q < int(threadIdx.x))) ? 1 : 0;
   }
   __syncthreads();
-  // rank r goes to wavefront r % 4, lane r / 4: every wavefront gets a like share of
-  // short and long lists (so the four finish together), and lanes next to each other
-  // lists of about the same length
-  s_len[rank] = cnt;   // sorted: wave w's longest list has rank 252 + w
-  const int rw = rank & 3, rl = rank >> 2;
-  W.lane[ts + rw * 64 + rl] = uint8_t(threadIdx.x);
+  // rank r goes to wavefront r / 64, lane r % 64: the lists of a wavefront are a
+  // quarter of the sorted lengths, so its padding to its longest list stays small
+  s_len[rank] = cnt;   // sorted: wave w's longest list has rank 64 w + 63
+  const int rw = rank >> 6, rl = rank & 63;
+  W.lane[ts + rank] = uint8_t(threadIdx.x);
   __syncthreads();
   int M = 0, woff = 0;
   for (int w = 0; w < kTile / 64; w++) {
-    const int g = (s_len[kTile - 4 + w] + 7) >> 3;
+    const int g = (s_len[64 * w + 63] + 7) >> 3;
     if (w < rw) woff += g * 512;
     M += g * 512;
   }
@@ -1702,18 +1701,58 @@ __device__ __forceinline__ void halo_entry(const Work& W, int64_t tile, int64_t 
 // the box's frame, so two particles at the same point get the same relative positions
 // to every neighbour whatever tile they sit in.
 template <int D>
+__device__ __forceinline__ void image_rec(const JobParams& J, uint32_t code, float4& rh, float4& rl) {
+  const double x[3] = {double(rh.x) + double(rl.x), double(rh.y) + double(rl.y), double(rh.z) + double(rl.z)};
+  double g[3] = {x[0], x[1], x[2]};
+  image_of<D>(J, code, x, g);
+  const float gh[3] = {float(g[0]), float(g[1]), float(g[2])};
+  rh = make_float4(gh[0], gh[1], gh[2], rh.w);
+  rl = make_float4(float(g[0] - double(gh[0])), float(g[1] - double(gh[1])), float(g[2] - double(gh[2])),
+                   __int_as_float(__float_as_int(rl.w) | int(kHMir)));
+}
+
+template <int D>
 __device__ __forceinline__ void halo_rec(const Work& W, const JobParams& J, int64_t q, uint32_t code, float4& rh,
                                          float4& rl) {
   rh = W.rec[4 * q + 0];
   rl = W.rec[4 * q + 1];
-  if (code) {
-    const double x[3] = {double(rh.x) + double(rl.x), double(rh.y) + double(rl.y), double(rh.z) + double(rl.z)};
-    double g[3] = {x[0], x[1], x[2]};
-    image_of<D>(J, code, x, g);
-    const float gh[3] = {float(g[0]), float(g[1]), float(g[2])};
-    rh = make_float4(gh[0], gh[1], gh[2], rh.w);
-    rl = make_float4(float(g[0] - double(gh[0])), float(g[1] - double(gh[1])), float(g[2] - double(gh[2])),
-                     __int_as_float(__float_as_int(rl.w) | int(kHMir)));
+  if (code) image_rec<D>(J, code, rh, rl);
+}
+
+// Stages the records of a tile's halo entries [0, HS) in LDS (hh, hl; hv: the third
+// record word, when V), HS <= NU kTile.  Each thread takes entries threadIdx.x + u kTile:
+// all its halo words are loaded first, then all its records, so the loads of one thread
+// are in flight together.
+template <int D, int NU, bool V>
+__device__ __forceinline__ void stage_halo(const Work& W, const JobParams& J, int64_t tile, int64_t ts, int HS,
+                                           float4* hh, float4* hl, float4* hv) {
+  int64_t q[NU];
+  uint32_t code[NU];
+#pragma unroll
+  for (int u = 0; u < NU; u++) {
+    const int e = threadIdx.x + u * kTile;
+    q[u] = ts;   // a valid slot for the entries past HS (their records are never stored)
+    code[u] = 0u;
+    if (e < HS) halo_entry(W, tile, ts, uint32_t(e), q[u], code[u]);
+  }
+  float4 rh[NU], rl[NU];
+#pragma unroll
+  for (int u = 0; u < NU; u++) {
+    const int e = threadIdx.x + u * kTile;
+    if (e < HS) {
+      rh[u] = W.rec[4 * q[u] + 0];
+      rl[u] = W.rec[4 * q[u] + 1];
+      if (V) hv[e] = W.rec[4 * q[u] + 2];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NU; u++) {
+    const int e = threadIdx.x + u * kTile;
+    if (e < HS) {
+      if (code[u]) image_rec<D>(J, code[u], rh[u], rl[u]);
+      hh[e] = rh[u];
+      hl[e] = rl[u];
+    }
   }
 }
 
@@ -1875,15 +1914,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
   jumper_prepare<D>(W, J, j, live, xhi, jt);   // ends with a barrier
   const int H = W.hcnt[tile];
   const int HS = H < kHL1 ? H : kHL1;
-  for (int e = threadIdx.x; e < HS; e += kTile) {
-    int64_t q;
-    uint32_t code;
-    halo_entry(W, tile, ts, uint32_t(e), q, code);
-    float4 rh, rl;
-    halo_rec<D>(W, J, q, code, rh, rl);
-    hh[e] = rh;
-    hl[e] = rl;
-  }
+  stage_halo<D, kHL1 / kTile, false>(W, J, tile, ts, HS, hh, hl, nullptr);
   if (live && (__float_as_int(r1.w) & ~kJumpBit) == SMG_SURFACE) s_surf = 1;
   __syncthreads();
   const bool tsurf = s_surf != 0;
@@ -1910,61 +1941,86 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
       // One body, predicated rather than branched (a whole wavefront almost never skips
       // an item together); a tile whose halo fits the LDS never reads a record from
       // memory inside the loop, so the next group's load stays in flight across it.
-      auto run = [&](auto fits) __attribute__((always_inline)) {
-        constexpr bool kFits = decltype(fits)::value;
-        const int ng = (cnt + 7) >> 3;
-        uint4 gn = it[0];   // inside the tile's items even when cnt = 0
-        for (int g = 0; g < ng; g++) {
-          Group8 gv = group_of(gn);
-          // the group is taken into registers here, so the wait for its load stays out of
-          // the item loop (which then never waits on memory: its stores go unawaited)
-          asm volatile("" : "+v"(gv.lo), "+v"(gv.hi));
-          gn = it[(g + 1 < ng ? g + 1 : g) * 64];
-          const int kn = cnt - 8 * g < 8 ? cnt - 8 * g : 8;
-          for (int u = 0; u < kn; u++) {
-            const uint32_t e = uint32_t(gv.lo) & 0xFFFFu;   // the group's next item
-            gv.lo = (gv.lo >> 16) | (gv.hi << 48);
-            gv.hi >>= 16;
-            float4 qh, ql;
-            int64_t q = -1;
-            uint32_t code = 0u;
-            if (kFits || e < uint32_t(HS)) {
-              qh = hh[e];
-              ql = hl[e];
-            } else {   // past the staged records (a tile whose halo exceeds the LDS)
-              halo_entry(W, tile, ts, e, q, code);
-              halo_rec<D>(W, J, q, code, qh, ql);
-            }
-            const int tw = __float_as_int(ql.w);
-            float dr[3] = {0.f, 0.f, 0.f};
-            rel_hl<D>(o.xh, o.xl, qh, ql, dr);
-            float d2 = 0.f;
+      // one item: the record (qh, ql) of halo entry e; q, code: its slot and mirror code
+      // if already known (q < 0: look them up when a rare path needs them)
+      auto body = [&](uint32_t e, const float4& qh, const float4& ql, int64_t q, uint32_t code)
+                      __attribute__((always_inline)) {
+        const int tw = __float_as_int(ql.w);
+        float dr[3] = {0.f, 0.f, 0.f};
+        rel_hl<D>(o.xh, o.xl, qh, ql, dr);
+        float d2 = 0.f;
 #pragma unroll
-            for (int c = 0; c < D; c++) d2 += dr[c] * dr[c];
-            // The FP32 distance of the compensated relative position is within 3 2^-24 d
-            // (+ 1e-13) of the FP64 one and r within 2^-23 r; the FP64 predicate decides
-            // inside 1e-6 r + 2e-10 of the cutoff (its own +1e-10 included).
-            const float r = fmaxf(o.rf, qh.w * cutf);
-            const float bd = 1.e-6f * r + 2.e-10f;
-            const float rlo = r - bd, rhi = r + bd;
-            bool nb = d2 < rlo * rlo;
-            if (!nb && d2 <= rhi * rhi) {
-              if (q < 0) halo_entry(W, tile, ts, e, q, code);
-              nb = exact_pred<D>(W, J, S, slot, q, code);
-            }
-            nb = nb && !(tw & kJumpBit);   // a jumper: paired through the jumper scan
-            const int tq = tw & ~(kJumpBit | int(kHMir));
-            if (nb && o.surf && tq == SMG_SURFACE && q < 0) halo_entry(W, tile, ts, e, q, code);
-            const bool f = pair1<D>(W, J, o, tsurf, dr, d2, qh.w, tq, e == uint32_t(p), q, acc, nb);
-            if (f && !(tw & int(kHMir))) {
-              st2[(c2 >> 3) * 512 + (c2 & 7)] = uint16_t(e);
-              c2++;
-            }
-          }
+        for (int c = 0; c < D; c++) d2 += dr[c] * dr[c];
+        // The FP32 distance of the compensated relative position is within 3 2^-24 d
+        // (+ 1e-13) of the FP64 one and r within 2^-23 r; the FP64 predicate decides
+        // inside 1e-6 r + 2e-10 of the cutoff (its own +1e-10 included).
+        const float r = fmaxf(o.rf, qh.w * cutf);
+        const float bd = 1.e-6f * r + 2.e-10f;
+        const float rlo = r - bd, rhi = r + bd;
+        bool nb = d2 < rlo * rlo;
+        if (!nb && d2 <= rhi * rhi) {
+          if (q < 0) halo_entry(W, tile, ts, e, q, code);
+          nb = exact_pred<D>(W, J, S, slot, q, code);
+        }
+        nb = nb && !(tw & kJumpBit);   // a jumper: paired through the jumper scan
+        const int tq = tw & ~(kJumpBit | int(kHMir));
+        if (nb && o.surf && tq == SMG_SURFACE && q < 0) halo_entry(W, tile, ts, e, q, code);
+        const bool f = pair1<D>(W, J, o, tsurf, dr, d2, qh.w, tq, e == uint32_t(p), q, acc, nb);
+        if (f && !(tw & int(kHMir))) {
+          st2[(c2 >> 3) * 512 + (c2 & 7)] = uint16_t(e);
+          c2++;
         }
       };
-      if (H <= kHL1) run(std::true_type{});
-      else run(std::false_type{});
+      const int ng = (cnt + 7) >> 3;
+      uint4 gn = it[0];   // inside the tile's items even when cnt = 0
+      Group8 gv = group_of(gn);
+      // the group is taken into registers here, so the wait for its load stays out of the
+      // item loop (which then never waits on memory: its stores go unawaited)
+      asm volatile("" : "+v"(gv.lo), "+v"(gv.hi));
+      gn = it[(1 < ng ? 1 : 0) * 64];
+      auto pop = [&](int k) __attribute__((always_inline)) {   // item k (k >= 1: the next one)
+        if (k > 0 && (k & 7) == 0) {   // the same k in every lane: a uniform branch
+          gv = group_of(gn);
+          asm volatile("" : "+v"(gv.lo), "+v"(gv.hi));
+          const int g = k >> 3;   // g <= ng (k = cnt when cnt is a multiple of 8)
+          gn = it[(g + 1 < ng ? g + 1 : ng - 1) * 64];
+        }
+        const uint32_t e = uint32_t(gv.lo) & 0xFFFFu;
+        gv.lo = (gv.lo >> 16) | (gv.hi << 48);
+        gv.hi >>= 16;
+        return e;
+      };
+      if (H <= kHL1) {
+        // the whole halo is in LDS: item k + 1's record is read while item k is summed
+        uint32_t en = pop(0);
+        en = 0 < cnt ? en : uint32_t(p);
+        float4 nh = hh[en], nl = hl[en];
+        for (int k = 0; k < cnt; k++) {
+          const uint32_t e = en;
+          const float4 qh = nh, ql = nl;
+          uint32_t e2 = pop(k + 1);
+          e2 = k + 1 < cnt ? e2 : uint32_t(p);   // past the end: the particle's own record
+          en = e2;
+          nh = hh[e2];
+          nl = hl[e2];
+          body(e, qh, ql, -1, 0u);
+        }
+      } else {
+        for (int k = 0; k < cnt; k++) {
+          const uint32_t e = pop(k);
+          float4 qh, ql;
+          int64_t q = -1;
+          uint32_t code = 0u;
+          if (e < uint32_t(HS)) {
+            qh = hh[e];
+            ql = hl[e];
+          } else {   // past the staged records (a tile whose halo exceeds the LDS)
+            halo_entry(W, tile, ts, e, q, code);
+            halo_rec<D>(W, J, q, code, qh, ql);
+          }
+          body(e, qh, ql, q, code);
+        }
+      }
       W.nbc2[slot] = c2;
     } else {
       W.nbc2[slot] = 0;
@@ -2087,16 +2143,7 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
   const float hdt = float(0.5 * dt1);
   const int H = W.hcnt[tile];
   const int HS = H < kHL2 ? H : kHL2;
-  for (int e = threadIdx.x; e < HS; e += kTile) {
-    int64_t q;
-    uint32_t code;
-    halo_entry(W, tile, ts, uint32_t(e), q, code);
-    float4 rh, rl;
-    halo_rec<D>(W, J, q, code, rh, rl);
-    hh[e] = rh;
-    hl[e] = rl;
-    hv[e] = W.rec[4 * q + 2];
-  }
+  stage_halo<D, kHL2 / kTile, true>(W, J, tile, ts, HS, hh, hl, hv);
   if (live && (__float_as_int(r1.w) & ~kJumpBit) == SMG_SURFACE) s_surf = 1;
   __syncthreads();
   const bool tsurf = s_surf != 0;
@@ -2123,44 +2170,63 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
     if (o.eta != 0.f) {   // eta_ij = 0 for every pair when this particle's nu is 0
       const int cnt = W.nbc2[slot];
       const uint4* it = reinterpret_cast<const uint4*>(W.items2 + tile * kTile * W.ips + W.ioff[slot]);
-      auto run = [&](auto fits) __attribute__((always_inline)) {
-        constexpr bool kFits = decltype(fits)::value;
-        const int ng = (cnt + 7) >> 3;
-        uint4 gn = it[0];   // inside the tile's items even when cnt = 0
-        for (int g = 0; g < ng; g++) {
-          Group8 gv = group_of(gn);
-          // the group is taken into registers here, so the wait for its load stays out of
-          // the item loop (which then never waits on memory: its stores go unawaited)
+      const int ng = (cnt + 7) >> 3;
+      uint4 gn = it[0];   // inside the tile's items even when cnt = 0
+      Group8 gv = group_of(gn);
+      asm volatile("" : "+v"(gv.lo), "+v"(gv.hi));
+      gn = it[(1 < ng ? 1 : 0) * 64];
+      auto pop = [&](int k) __attribute__((always_inline)) {   // item k (k >= 1: the next one)
+        if (k > 0 && (k & 7) == 0) {   // the same k in every lane: a uniform branch
+          gv = group_of(gn);
           asm volatile("" : "+v"(gv.lo), "+v"(gv.hi));
-          gn = it[(g + 1 < ng ? g + 1 : g) * 64];
-          const int kn = cnt - 8 * g < 8 ? cnt - 8 * g : 8;
-          for (int u = 0; u < kn; u++) {
-            const uint32_t e = uint32_t(gv.lo) & 0xFFFFu;   // the group's next item
-            gv.lo = (gv.lo >> 16) | (gv.hi << 48);
-            gv.hi >>= 16;
-            float4 qh, ql, qv;
-            if (kFits || e < uint32_t(HS)) {
-              qh = hh[e];
-              ql = hl[e];
-              qv = hv[e];
-            } else {
-              int64_t q;
-              uint32_t code;
-              halo_entry(W, tile, ts, e, q, code);
-              halo_rec<D>(W, J, q, code, qh, ql);
-              qv = W.rec[4 * q + 2];
-            }
-            float dr[3] = {0.f, 0.f, 0.f};
-            rel_hl<D>(o.xh, o.xl, qh, ql, dr);
-            float rr = 0.f;
-#pragma unroll
-            for (int c = 0; c < D; c++) rr += dr[c] * dr[c];
-            pair2<D>(J, o, tsurf, hdt, dr, rr, qh.w, __float_as_int(ql.w) & ~(kJumpBit | int(kHMir)), qv, true, F);
-          }
+          const int g = k >> 3;   // g <= ng (k = cnt when cnt is a multiple of 8)
+          gn = it[(g + 1 < ng ? g + 1 : ng - 1) * 64];
         }
+        const uint32_t e = uint32_t(gv.lo) & 0xFFFFu;
+        gv.lo = (gv.lo >> 16) | (gv.hi << 48);
+        gv.hi >>= 16;
+        return e;
       };
-      if (H <= kHL2) run(std::true_type{});
-      else run(std::false_type{});
+      auto body = [&](const float4& qh, const float4& ql, const float4& qv) __attribute__((always_inline)) {
+        float dr[3] = {0.f, 0.f, 0.f};
+        rel_hl<D>(o.xh, o.xl, qh, ql, dr);
+        float rr = 0.f;
+#pragma unroll
+        for (int c = 0; c < D; c++) rr += dr[c] * dr[c];
+        pair2<D>(J, o, tsurf, hdt, dr, rr, qh.w, __float_as_int(ql.w) & ~(kJumpBit | int(kHMir)), qv, true, F);
+      };
+      if (H <= kHL2) {
+        // the whole halo is in LDS: item k + 1's record is read while item k is summed
+        uint32_t en = pop(0);
+        en = 0 < cnt ? en : uint32_t(p);
+        float4 nh = hh[en], nl = hl[en], nv = hv[en];
+        for (int k = 0; k < cnt; k++) {
+          const float4 qh = nh, ql = nl, qv = nv;
+          uint32_t e2 = pop(k + 1);
+          e2 = k + 1 < cnt ? e2 : uint32_t(p);   // past the end: the particle's own record
+          nh = hh[e2];
+          nl = hl[e2];
+          nv = hv[e2];
+          body(qh, ql, qv);
+        }
+      } else {
+        for (int k = 0; k < cnt; k++) {
+          const uint32_t e = pop(k);
+          float4 qh, ql, qv;
+          if (e < uint32_t(HS)) {
+            qh = hh[e];
+            ql = hl[e];
+            qv = hv[e];
+          } else {
+            int64_t q;
+            uint32_t code;
+            halo_entry(W, tile, ts, e, q, code);
+            halo_rec<D>(W, J, q, code, qh, ql);
+            qv = W.rec[4 * q + 2];
+          }
+          body(qh, ql, qv);
+        }
+      }
       // pairs with this step's jumpers (and, for a jumper, its own list), after the items
       const float xhi[3] = {r0.x, r0.y, r0.z}, xli[3] = {r1.x, r1.y, r1.z};
       for_each_neighbor<D>(W, S, J, slot, jt, [&](int, int64_t q, uint32_t code, const float4& qh, const float4& ql,
