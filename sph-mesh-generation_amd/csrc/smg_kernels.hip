@@ -427,6 +427,7 @@ __device__ __forceinline__ float dWk(float r, float h) {   // (dW/dr) / (r + 1e-
 // pair's mean h with one reciprocal.
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 // Both branches are evaluated and selected (no divergent control flow); t = 0 for
 // s >= 2 makes the outer branch the zero of the kernel's support.
 template <int DK>
@@ -1719,13 +1720,37 @@ __device__ __forceinline__ void halo_rec(const Work& W, const JobParams& J, int6
   if (code) image_rec<D>(J, code, rh, rl);
 }
 
-// Stages the records of a tile's halo entries [0, HS) in LDS (hh, hl; hv: the third
-// record word, when V), HS <= NU kTile.  Each thread takes entries threadIdx.x + u kTile:
-// all its halo words are loaded first, then all its records, so the loads of one thread
-// are in flight together.
+// The pair passes' compact form of a record's second half: the lo parts rounded to
+// bfloat16 (8 significant bits, round to nearest even) and a 16-bit type word (type,
+// kQJump, kQMir), 8 bytes.  A lo part is at most half an ulp of its hi part, so the
+// rounding moves a position by at most 2^-9 of that: 2^-33 |x| (the pair passes widen
+// their cutoff band by it).  A particle's own lo parts are rounded the same way, so
+// two particles at one point still sit at one point.
+constexpr uint32_t kQJump = 1u << 14, kQMir = 1u << 15, kQType = kQJump - 1u;
+__device__ __forceinline__ uint32_t bf16_rn(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ uint2 pack_lo(const float4& rl) {
+  const int tw = __float_as_int(rl.w);
+  const uint32_t t16 = (uint32_t(tw) & kQType) | ((tw & kJumpBit) ? kQJump : 0u) | ((tw & int(kHMir)) ? kQMir : 0u);
+  return make_uint2(bf16_rn(rl.x) | (bf16_rn(rl.y) << 16), bf16_rn(rl.z) | (t16 << 16));
+}
+__device__ __forceinline__ void unpack_lo(const uint2& v, float* lo, int& tw) {
+  lo[0] = __uint_as_float(v.x << 16);
+  lo[1] = __uint_as_float(v.x & 0xFFFF0000u);
+  lo[2] = __uint_as_float(v.y << 16);
+  tw = int(v.y >> 16);
+}
+
+// Stages the records of a tile's halo entries [0, HS) in LDS (hh: hi parts and h; hq:
+// the compact lo parts and type word; hv: the third record word, when V),
+// HS <= NU kTile.  Each thread takes entries threadIdx.x + u kTile: all its halo words
+// are loaded first, then all its records, so the loads of one thread are in flight
+// together.
 template <int D, int NU, bool V>
 __device__ __forceinline__ void stage_halo(const Work& W, const JobParams& J, int64_t tile, int64_t ts, int HS,
-                                           float4* hh, float4* hl, float4* hv) {
+                                           float4* hh, uint2* hq, float4* hv) {
   int64_t q[NU];
   uint32_t code[NU];
 #pragma unroll
@@ -1751,7 +1776,7 @@ __device__ __forceinline__ void stage_halo(const Work& W, const JobParams& J, in
     if (e < HS) {
       if (code[u]) image_rec<D>(J, code[u], rh[u], rl[u]);
       hh[e] = rh[u];
-      hl[e] = rl[u];
+      hq[e] = pack_lo(rl[u]);
     }
   }
 }
@@ -1777,22 +1802,13 @@ __device__ __forceinline__ bool exact_pred(const Work& W, const JobParams& J, co
   return dist <= S.h[si] * J.cut_off + 1.e-10 || dist <= S.h[q] * J.cut_off + 1.e-10;
 }
 
-// a 16-byte group of eight 16-bit items as two 64-bit halves (consumed by shifting,
-// so that no item is picked by a dynamic index)
-struct Group8 {
-  uint64_t lo, hi;
-};
-__device__ __forceinline__ Group8 group_of(const uint4& g) {
-  return {uint64_t(g.x) | (uint64_t(g.y) << 32), uint64_t(g.z) | (uint64_t(g.w) << 32)};
-}
-
 // x_i - x_q from the hi / lo pairs (the image already applied to q's record)
 template <int D>
-__device__ __forceinline__ void rel_hl(const float* xhi, const float* xli, const float4& qh, const float4& ql,
+__device__ __forceinline__ void rel_hl(const float* xhi, const float* xli, const float4& qh, const float* lo,
                                        float* dr) {
-  dr[0] = (xhi[0] - qh.x) + (xli[0] - ql.x);
-  if (D > 1) dr[1] = (xhi[1] - qh.y) + (xli[1] - ql.y);
-  if (D > 2) dr[2] = (xhi[2] - qh.z) + (xli[2] - ql.z);
+  dr[0] = (xhi[0] - qh.x) + (xli[0] - lo[0]);
+  if (D > 1) dr[1] = (xhi[1] - qh.y) + (xli[1] - lo[1]);
+  if (D > 2) dr[2] = (xhi[2] - qh.z) + (xli[2] - lo[2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1801,8 +1817,9 @@ __device__ __forceinline__ void rel_hl(const float* xhi, const float* xli, const
 // the particle a pair pass is summing for
 struct Own {
   float xh[3], xl[3];
-  float h, ih, Ck, kd, vol2i, vol, eta, rf;
+  float h, ih, Ck, kd3, vol2i, vol, eta, rf;
   float kdx, cth;   // the job's cut_off / cell_ratio and cos(angle_thres), in registers
+  float bq;         // the cutoff band's absolute part (FP32 rounding and the bf16 lo parts)
   float n[3], v[3];
   bool surf, jump;
 };
@@ -1816,7 +1833,10 @@ __device__ __forceinline__ void own_common(const JobParams& J, const float4& rh,
   const int tw = __float_as_int(rl.w);
   const int type = tw & ~(kJumpBit | int(kHMir));
   o.xh[0] = rh.x; o.xh[1] = rh.y; o.xh[2] = rh.z;
-  o.xl[0] = rl.x; o.xl[1] = rl.y; o.xl[2] = rl.z;
+  // the lo parts as the compact records hold them (bf16)
+  o.xl[0] = __uint_as_float(bf16_rn(rl.x) << 16);
+  o.xl[1] = __uint_as_float(bf16_rn(rl.y) << 16);
+  o.xl[2] = __uint_as_float(bf16_rn(rl.z) << 16);
   o.h = rh.w;
   o.surf = type == SMG_SURFACE;
   o.jump = (tw & kJumpBit) != 0;
@@ -1828,26 +1848,41 @@ __device__ __forceinline__ void own_common(const JobParams& J, const float4& rh,
   o.ih = 1.f / o.h;
   const float Cw = kC<D>() * ipow<D>(o.ih), Cs = kC<D - 1>() * ipow<D - 1>(o.ih);
   o.Ck = o.surf ? Cs : Cw;
-  o.kd = o.surf ? kC<D - 1>() : kC<D>();
+  o.kd3 = 3.f * (o.surf ? kC<D - 1>() : kC<D>());
   o.rf = o.h * float(J.cut_off);
   o.kdx = J.kdx;
   o.cth = J.cos_thres;
+  // every coordinate of a record (mirror images included) lies within c0 of the box,
+  // and rounding a lo part moves it by at most 2^-33 |x|: 2^-30 max |x| bounds what
+  // that adds to a distance, with room to spare
+  double xm = 0.;
+#pragma unroll
+  for (int a = 0; a < D; a++) xm = fmax(xm, fmax(fabs(J.box_l[a] - J.c0), fabs(J.box_r[a] + J.c0)));
+  o.bq = 2.e-10f + 9.32e-10f * float(xm);
 }
 
 // One stage-1 pair (Get_kernel_summation and the stage-1 Accumulate_particle_force_real
 // / _surf, particle_mesh_generation.cpp:690-765, 883-1118) of a verified neighbour:
 // dr0 = x_i - x_q, d2 = |dr0|^2.  A surface particle uses the (D-1)-dimensional kernel
 // on the tangential part of dr (raw normal, Q3) and only surface neighbours with
-// aligned normals.  One straight-line body for both types: a REAL particle has n = 0,
-// so the projection leaves dr and d2 unchanged bit for bit.  Returns whether the pair
-// enters the stage-2 pass (a non-self neighbour the particle uses).
-template <int D>
-__device__ __forceinline__ bool pair1(const Work& W, const JobParams& J, const Own& o, bool tsurf, const float* dr0,
-                                      float d2, float hq, int tq, bool self, int64_t q, Acc1& acc, bool valid = true) {
+// aligned normals.  TS: the particle's tile may hold surface particles (the general
+// form); without TS no particle of the tile is a surface one.  In the general form a
+// REAL particle has n = 0, so the projection leaves dr and d2 unchanged bit for bit and
+// both forms give the same sums.  With s = r / h_i and s2 = r / h_m (h_m the pair's
+// mean h), W = Ck w(s) with w = 6 - 6s + s^3 (s < 1), (2 - s)^3 (s < 2), and
+// (dW/dr) / r = 3 kC h_m^-(DK+1) dq(s2) / r with dq = s2^2 - 2 (s2 < 1), -(2 - s2)^2:
+// the cubic spline and its derivative, factored.  1 / r is the hardware reciprocal
+// square root of r^2 (of 1e-30 at r = 0, where dr = 0 makes the term vanish) and
+// r = r^2 / r.  Returns whether the pair enters the stage-2 pass (a non-self
+// neighbour the particle uses).
+template <int D, bool TS>
+__device__ __forceinline__ bool pair1(const Work& W, const JobParams& J, const Own& o, const float* dr0, float d2,
+                                      float hq, int tq, bool self, int64_t q, Acc1& acc, bool valid = true) {
   // valid = false: the pair is not a neighbour (the sums are left as they are)
+  const bool osurf = TS && o.surf;
   acc.ntr += valid ? 1 : 0;
-  bool use = valid && !(o.surf && tq == SMG_REAL);
-  if (use && o.surf && tq == SMG_SURFACE) {
+  bool use = valid && !(osurf && tq == SMG_REAL);
+  if (TS && use && osurf && tq == SMG_SURFACE) {
     const float4 qn = W.rec[4 * q + 3];
     const float nq[3] = {qn.x, qn.y, qn.z};
     float ca = 0.f;
@@ -1856,31 +1891,34 @@ __device__ __forceinline__ bool pair1(const Work& W, const JobParams& J, const O
     use = fabsf(ca) > o.cth;
   }
   float dr[3] = {dr0[0], dr0[1], dr0[2]};
-  float rt = d2;
-  if (tsurf) {
+  float rt2 = d2;
+  if (TS) {
     float dn = 0.f;
 #pragma unroll
     for (int a = 0; a < D; a++) dn += dr0[a] * o.n[a];
-    rt = 0.f;
+    rt2 = 0.f;
 #pragma unroll
     for (int a = 0; a < D; a++) {
       dr[a] = dr0[a] - o.n[a] * dn;
-      rt += dr[a] * dr[a];
+      rt2 += dr[a] * dr[a];
     }
   }
-  rt = fsqrt(rt);
-  const bool counts = use && (!o.surf || tq == SMG_SURFACE);   // the kernel summation
-  const float w = Wk_s<D>(rt * o.ih, o.Ck);
-  acc.sum = counts ? acc.sum + w : acc.sum;
+  const float irt = frsq(fmaxf(rt2, 1.e-30f));
+  const float rt = rt2 * irt;
+  const bool counts = use && (!osurf || tq == SMG_SURFACE);   // the kernel summation
+  const float s = rt * o.ih;
+  const float t = fmaxf(2.f - s, 0.f);
+  const float w = s < 1.f ? (6.0f - 6.0f * s) + s * s * s : t * t * t;
+  acc.sum = counts ? acc.sum + o.Ck * w : acc.sum;
   acc.nc += counts ? 1 : 0;
   const float dxq = hq * o.kdx;
   const float vol2q = ipow<D>(dxq) * ipow<D>(dxq);
   const float ihm = frcp(0.5f * (o.h + hq));
-  const float C = o.kd * (ipow<D>(ihm) * (o.surf ? 1.f : ihm));
+  const float C3 = osurf ? o.kd3 * ipow<D>(ihm) : o.kd3 * ipow<D + 1>(ihm);
   const float s2 = rt * ihm;
-  const float t = fmaxf(2.f - s2, 0.f);
-  const float dw = (s2 < 1.f ? C * (-6.0f + 3.f * s2 * s2) : C * (-3.0f) * t * t) * frcp(rt + 1.e-20f);
-  const float aw = dw * (o.vol2i + vol2q);
+  const float t2 = fmaxf(2.f - s2, 0.f);
+  const float dq = s2 < 1.f ? s2 * s2 - 2.f : -(t2 * t2);
+  const float aw = C3 * dq * irt * (o.vol2i + vol2q);
   const bool f = use && !self;
 #pragma unroll
   for (int a = 0; a < D; a++) acc.g[a] = f ? acc.g[a] + dr[a] * aw : acc.g[a];
@@ -1892,9 +1930,9 @@ __device__ __forceinline__ bool pair1(const Work& W, const JobParams& J, const O
 // items in list order, then the pairs with this step's jumpers.  Records come from the
 // tile's halo, staged in LDS (the first kHL1 entries; the rest read from the records).
 template <int D>
-__global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void k_force1(Work W, PState S, int par) {
   __shared__ float4 hh[kHL1];
-  __shared__ float4 hl[kHL1];
+  __shared__ uint2 hq[kHL1];
   __shared__ double lds_d[4];
   __shared__ int lds_i[4];
   __shared__ int s_surf;
@@ -1914,8 +1952,12 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
   jumper_prepare<D>(W, J, j, live, xhi, jt);   // ends with a barrier
   const int H = W.hcnt[tile];
   const int HS = H < kHL1 ? H : kHL1;
-  stage_halo<D, kHL1 / kTile, false>(W, J, tile, ts, HS, hh, hl, nullptr);
+  stage_halo<D, kHL1 / kTile, false>(W, J, tile, ts, HS, hh, hq, nullptr);
   if (live && (__float_as_int(r1.w) & ~kJumpBit) == SMG_SURFACE) s_surf = 1;
+  // this lane's item count (hcnt = 0: the tile's build failed; a jumper's list is
+  // stale) and the wavefront's largest
+  const int cnt = (live && !(__float_as_int(r1.w) & kJumpBit) && H > 0) ? W.nbc[slot] : 0;
+  const int kmax = __builtin_amdgcn_readfirstlane(wave_reduce(cnt, OpMaxI()));
   __syncthreads();
   const bool tsurf = s_surf != 0;
   double dt = 1.0e300;
@@ -1932,93 +1974,95 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
     Acc1 acc = {};
     const float cutf = float(J.cut_off);
     if (!o.jump) {   // a jumper's list is stale: it walks its own list below
-      const int cnt = H > 0 ? W.nbc[slot] : 0;   // hcnt = 0: the tile's build failed
-      const int64_t i0 = tile * kTile * W.ips + W.ioff[slot];
-      // items eight to a 16-byte group, groups 64 apart (the wavefront's lanes between)
-      const uint4* it = reinterpret_cast<const uint4*>(W.items + i0);
-      uint16_t* st2 = W.items2 + i0;   // the same group layout
       int c2 = 0;
-      // One body, predicated rather than branched (a whole wavefront almost never skips
-      // an item together); a tile whose halo fits the LDS never reads a record from
-      // memory inside the loop, so the next group's load stays in flight across it.
-      // one item: the record (qh, ql) of halo entry e; q, code: its slot and mirror code
-      // if already known (q < 0: look them up when a rare path needs them)
-      auto body = [&](uint32_t e, const float4& qh, const float4& ql, int64_t q, uint32_t code)
-                      __attribute__((always_inline)) {
-        const int tw = __float_as_int(ql.w);
-        float dr[3] = {0.f, 0.f, 0.f};
-        rel_hl<D>(o.xh, o.xl, qh, ql, dr);
-        float d2 = 0.f;
+      if (kmax > 0) {
+        const int64_t i0 = tile * kTile * W.ips + W.ioff[slot];
+        // items eight to a 16-byte group, groups 64 apart (the wavefront's lanes between)
+        const uint4* it = reinterpret_cast<const uint4*>(W.items + i0);
+        uint16_t* st2 = W.items2 + i0;   // the same group layout
+        // One item of the walk: the record (qh, ql) of halo entry e; q, code: its slot
+        // and mirror code if already known (q < 0: looked up when a rare path needs
+        // them).  An item past the lane's list (valid = false) is its own record and
+        // adds nothing.  Every item writes e at the lane's stage-2 position c2, which
+        // only a kept pair advances (so a later item overwrites a dropped one; c2 stays
+        // below the wavefront's item count, inside its part of the tile's items).
+        auto body = [&](auto ts_, uint32_t e, const float4& qh, const uint2& qc, int64_t q, uint32_t code,
+                        bool valid) __attribute__((always_inline)) {
+          constexpr bool TS = decltype(ts_)::value;
+          float lo[3];
+          int tw;
+          unpack_lo(qc, lo, tw);
+          float dr[3] = {0.f, 0.f, 0.f};
+          rel_hl<D>(o.xh, o.xl, qh, lo, dr);
+          float d2 = 0.f;
 #pragma unroll
-        for (int c = 0; c < D; c++) d2 += dr[c] * dr[c];
-        // The FP32 distance of the compensated relative position is within 3 2^-24 d
-        // (+ 1e-13) of the FP64 one and r within 2^-23 r; the FP64 predicate decides
-        // inside 1e-6 r + 2e-10 of the cutoff (its own +1e-10 included).
-        const float r = fmaxf(o.rf, qh.w * cutf);
-        const float bd = 1.e-6f * r + 2.e-10f;
-        const float rlo = r - bd, rhi = r + bd;
-        bool nb = d2 < rlo * rlo;
-        if (!nb && d2 <= rhi * rhi) {
-          if (q < 0) halo_entry(W, tile, ts, e, q, code);
-          nb = exact_pred<D>(W, J, S, slot, q, code);
-        }
-        nb = nb && !(tw & kJumpBit);   // a jumper: paired through the jumper scan
-        const int tq = tw & ~(kJumpBit | int(kHMir));
-        if (nb && o.surf && tq == SMG_SURFACE && q < 0) halo_entry(W, tile, ts, e, q, code);
-        const bool f = pair1<D>(W, J, o, tsurf, dr, d2, qh.w, tq, e == uint32_t(p), q, acc, nb);
-        if (f && !(tw & int(kHMir))) {
-          st2[(c2 >> 3) * 512 + (c2 & 7)] = uint16_t(e);
-          c2++;
-        }
-      };
-      const int ng = (cnt + 7) >> 3;
-      uint4 gn = it[0];   // inside the tile's items even when cnt = 0
-      Group8 gv = group_of(gn);
-      // the group is taken into registers here, so the wait for its load stays out of the
-      // item loop (which then never waits on memory: its stores go unawaited)
-      asm volatile("" : "+v"(gv.lo), "+v"(gv.hi));
-      gn = it[(1 < ng ? 1 : 0) * 64];
-      auto pop = [&](int k) __attribute__((always_inline)) {   // item k (k >= 1: the next one)
-        if (k > 0 && (k & 7) == 0) {   // the same k in every lane: a uniform branch
-          gv = group_of(gn);
-          asm volatile("" : "+v"(gv.lo), "+v"(gv.hi));
-          const int g = k >> 3;   // g <= ng (k = cnt when cnt is a multiple of 8)
-          gn = it[(g + 1 < ng ? g + 1 : ng - 1) * 64];
-        }
-        const uint32_t e = uint32_t(gv.lo) & 0xFFFFu;
-        gv.lo = (gv.lo >> 16) | (gv.hi << 48);
-        gv.hi >>= 16;
-        return e;
-      };
-      if (H <= kHL1) {
-        // the whole halo is in LDS: item k + 1's record is read while item k is summed
-        uint32_t en = pop(0);
-        en = 0 < cnt ? en : uint32_t(p);
-        float4 nh = hh[en], nl = hl[en];
-        for (int k = 0; k < cnt; k++) {
-          const uint32_t e = en;
-          const float4 qh = nh, ql = nl;
-          uint32_t e2 = pop(k + 1);
-          e2 = k + 1 < cnt ? e2 : uint32_t(p);   // past the end: the particle's own record
-          en = e2;
-          nh = hh[e2];
-          nl = hl[e2];
-          body(e, qh, ql, -1, 0u);
-        }
-      } else {
-        for (int k = 0; k < cnt; k++) {
-          const uint32_t e = pop(k);
-          float4 qh, ql;
-          int64_t q = -1;
-          uint32_t code = 0u;
-          if (e < uint32_t(HS)) {
-            qh = hh[e];
-            ql = hl[e];
-          } else {   // past the staged records (a tile whose halo exceeds the LDS)
-            halo_entry(W, tile, ts, e, q, code);
-            halo_rec<D>(W, J, q, code, qh, ql);
+          for (int c = 0; c < D; c++) d2 += dr[c] * dr[c];
+          // The FP32 distance of the compensated relative position is within 3 2^-24 d
+          // (+ 1e-13, + the bf16 lo parts' share) of the FP64 one and r within 2^-23 r;
+          // the FP64 predicate decides inside about 1e-6 r + bq of the cutoff (its own
+          // +1e-10 included).
+          const float r = fmaxf(o.rf, qh.w * cutf);
+          const float rlo = r * (1.f - 1.e-6f) - o.bq;
+          bool nb = d2 < rlo * rlo;
+          if (valid && !nb) {
+            const float rhi = r * (1.f + 1.e-6f) + o.bq;
+            if (d2 <= rhi * rhi) {
+              if (q < 0) halo_entry(W, tile, ts, e, q, code);
+              nb = exact_pred<D>(W, J, S, slot, q, code);
+            }
           }
-          body(e, qh, ql, q, code);
+          nb = valid && nb && !(tw & int(kQJump));   // a jumper: paired through the jumper scan
+          const int tq = tw & int(kQType);
+          if (TS && nb && o.surf && tq == SMG_SURFACE && q < 0) halo_entry(W, tile, ts, e, q, code);
+          const bool f = pair1<D, TS>(W, J, o, dr, d2, qh.w, tq, e == uint32_t(p), q, acc, nb);
+          st2[(c2 >> 3) * 512 + (c2 & 7)] = uint16_t(e);
+          c2 += (f && !(tw & int(kQMir))) ? 1 : 0;
+        };
+        // The walk runs to the wavefront's longest list, so its loop and group loads are
+        // the same in every lane; each group of eight items is unrolled.
+        auto walk = [&](auto ts_, auto fits_) __attribute__((always_inline)) {
+          constexpr bool kFits = decltype(fits_)::value;
+          const int ngw = (kmax + 7) >> 3;
+          uint4 gn = it[0];
+          for (int g = 0; g < ngw; g++) {
+            uint4 gc = gn;
+            // the group is taken into registers here, so the wait for its load stays
+            // out of the items
+            asm volatile("" : "+v"(gc.x), "+v"(gc.y), "+v"(gc.z), "+v"(gc.w));
+            gn = it[(g + 1 < ngw ? g + 1 : g) * 64];
+            uint32_t w0 = gc.x, w1 = gc.y, w2 = gc.z, w3 = gc.w;
+#pragma unroll 1
+            for (int v = 0; v < 4; v++) {   // two items from one word
+#pragma unroll
+              for (int u = 0; u < 2; u++) {
+                const bool valid = 8 * g + 2 * v + u < cnt;
+                const uint32_t e = valid ? (u ? w0 >> 16 : w0 & 0xFFFFu) : uint32_t(p);
+                float4 qh;
+                uint2 qc;
+                int64_t q = -1;
+                uint32_t code = 0u;
+                if (kFits || e < uint32_t(HS)) {
+                  qh = hh[e];
+                  qc = hq[e];
+                } else {   // past the staged records (a tile whose halo exceeds the LDS)
+                  halo_entry(W, tile, ts, e, q, code);
+                  float4 rl;
+                  halo_rec<D>(W, J, q, code, qh, rl);
+                  qc = pack_lo(rl);
+                }
+                body(ts_, e, qh, qc, q, code, valid);
+              }
+              w0 = w1;
+              w1 = w2;
+              w2 = w3;
+            }
+          }
+        };
+        if (H <= kHL1) {
+          if (tsurf) walk(std::true_type{}, std::true_type{});
+          else walk(std::false_type{}, std::true_type{});
+        } else {
+          walk(std::true_type{}, std::false_type{});
         }
       }
       W.nbc2[slot] = c2;
@@ -2028,7 +2072,7 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
     // pairs with this step's jumpers (and, for a jumper, its own list), after the items
     for_each_neighbor<D>(W, S, J, slot, jt, [&](int, int64_t q, uint32_t code, const float4& qh, const float4& ql,
                                                  const float* dr0, float d2) {
-      pair1<D>(W, J, o, tsurf, dr0, d2, qh.w, __float_as_int(ql.w) & ~kJumpBit, q == slot && code == 0u, q, acc);
+      pair1<D, true>(W, J, o, dr0, d2, qh.w, __float_as_int(ql.w) & ~kJumpBit, q == slot && code == 0u, q, acc);
     }, true);
     float sum = acc.sum;
     if (acc.nc > 1) sum = 1.f / (sum + 1.e-15f);
@@ -2074,9 +2118,10 @@ __global__ __launch_bounds__(kTile) void k_force1(Work W, PState S, int par) {
 // One stage-2 pair (Accumulate_particle_force_real / _surf, stage 2,
 // particle_mesh_generation.cpp:995-1118): eta_ij the harmonic mean of nu rho, a
 // non-REAL neighbour of a REAL particle moving at -v_i (Q8); dr0 = x_i - x_q, rr = |dr0|^2.
-template <int D>
-__device__ __forceinline__ void pair2(const JobParams& J, const Own& o, bool tsurf, float hdt, const float* dr0, float rr,
-                                      float hq, int tq, const float4& qv, bool use, float* F) {
+template <int D, bool TS>
+__device__ __forceinline__ void pair2(const JobParams& J, const Own& o, float hdt, const float* dr0, float rr, float hq,
+                                      int tq, const float4& qv, bool use, float* F) {
+  const bool osurf = TS && o.surf;
   const float dxq = hq * o.kdx;
   const float volq = ipow<D>(dxq);
   const float rhoq = frcp(volq);
@@ -2085,25 +2130,28 @@ __device__ __forceinline__ void pair2(const JobParams& J, const Own& o, bool tsu
   // point) adds nothing: its term is (v_i - v_j) dW(0) / 1e-20 with v_i = v_j, both
   // sums of the same terms in the same order
   use = use && etaq != 0.f && rr > 0.f;
-  float rt = rr;
-  if (tsurf) {
+  float rt2 = rr;
+  if (TS) {
     float dn = 0.f;
 #pragma unroll
     for (int a = 0; a < D; a++) dn += dr0[a] * o.n[a];
-    rt = 0.f;
+    rt2 = 0.f;
 #pragma unroll
     for (int a = 0; a < D; a++) {
       const float t = dr0[a] - o.n[a] * dn;
-      rt += t * t;
+      rt2 += t * t;
     }
   }
-  rt = fsqrt(rt);
+  // (dW/dr) / r as in the stage-1 pair
+  const float irt = frsq(fmaxf(rt2, 1.e-30f));
+  const float rt = rt2 * irt;
   const float ihm = frcp(0.5f * (o.h + hq));
-  const float C = o.kd * (ipow<D>(ihm) * (o.surf ? 1.f : ihm));
+  const float C3 = osurf ? o.kd3 * ipow<D>(ihm) : o.kd3 * ipow<D + 1>(ihm);
   const float s2 = rt * ihm;
   const float t2 = fmaxf(2.f - s2, 0.f);
-  const float aw = (s2 < 1.f ? C * (-6.0f + 3.f * s2 * s2) : C * (-3.0f) * t2 * t2) * frcp(rt + 1.e-20f);
-  const bool mirror_v = !o.surf && tq != SMG_REAL;   // Q8: v_j := -v_i
+  const float dq = s2 < 1.f ? s2 * s2 - 2.f : -(t2 * t2);
+  const float aw = C3 * dq * irt;
+  const bool mirror_v = !osurf && tq != SMG_REAL;   // Q8: v_j := -v_i
   const float etaij = 2.f * etaq * o.eta * frcp(o.eta + etaq + 1.e-15f);
   const float temp0 = aw * etaij * (o.vol * o.vol + volq * volq);
 #pragma unroll
@@ -2118,9 +2166,9 @@ __device__ __forceinline__ void pair2(const JobParams& J, const Own& o, bool tsu
 // staged records carrying (a1, nu of the previous step) for the neighbour's velocity
 // and viscosity; the same thread-to-particle order as the stage-1 pass.
 template <int D>
-__global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void k_force2(Work W, PState S, int par) {
   __shared__ float4 hh[kHL2];
-  __shared__ float4 hl[kHL2];
+  __shared__ uint2 hq[kHL2];
   __shared__ float4 hv[kHL2];
   __shared__ double lds_d[4];
   __shared__ int s_surf;
@@ -2143,8 +2191,11 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
   const float hdt = float(0.5 * dt1);
   const int H = W.hcnt[tile];
   const int HS = H < kHL2 ? H : kHL2;
-  stage_halo<D, kHL2 / kTile, true>(W, J, tile, ts, HS, hh, hl, hv);
+  stage_halo<D, kHL2 / kTile, true>(W, J, tile, ts, HS, hh, hq, hv);
   if (live && (__float_as_int(r1.w) & ~kJumpBit) == SMG_SURFACE) s_surf = 1;
+  // this lane's stage-2 item count (0 when its nu is 0) and the wavefront's largest
+  const int cnt = (live && r2.w != 0.f) ? W.nbc2[slot] : 0;
+  const int kmax = __builtin_amdgcn_readfirstlane(wave_reduce(cnt, OpMaxI()));
   __syncthreads();
   const bool tsurf = s_surf != 0;
   double dt = 1.0e300, U = 0.;
@@ -2168,63 +2219,60 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
     }
     float F[3] = {0.f, 0.f, 0.f};
     if (o.eta != 0.f) {   // eta_ij = 0 for every pair when this particle's nu is 0
-      const int cnt = W.nbc2[slot];
-      const uint4* it = reinterpret_cast<const uint4*>(W.items2 + tile * kTile * W.ips + W.ioff[slot]);
-      const int ng = (cnt + 7) >> 3;
-      uint4 gn = it[0];   // inside the tile's items even when cnt = 0
-      Group8 gv = group_of(gn);
-      asm volatile("" : "+v"(gv.lo), "+v"(gv.hi));
-      gn = it[(1 < ng ? 1 : 0) * 64];
-      auto pop = [&](int k) __attribute__((always_inline)) {   // item k (k >= 1: the next one)
-        if (k > 0 && (k & 7) == 0) {   // the same k in every lane: a uniform branch
-          gv = group_of(gn);
-          asm volatile("" : "+v"(gv.lo), "+v"(gv.hi));
-          const int g = k >> 3;   // g <= ng (k = cnt when cnt is a multiple of 8)
-          gn = it[(g + 1 < ng ? g + 1 : ng - 1) * 64];
-        }
-        const uint32_t e = uint32_t(gv.lo) & 0xFFFFu;
-        gv.lo = (gv.lo >> 16) | (gv.hi << 48);
-        gv.hi >>= 16;
-        return e;
-      };
-      auto body = [&](const float4& qh, const float4& ql, const float4& qv) __attribute__((always_inline)) {
-        float dr[3] = {0.f, 0.f, 0.f};
-        rel_hl<D>(o.xh, o.xl, qh, ql, dr);
-        float rr = 0.f;
+      if (kmax > 0) {
+        const uint4* it = reinterpret_cast<const uint4*>(W.items2 + tile * kTile * W.ips + W.ioff[slot]);
+        // the items of the stage-1 walk, to the wavefront's longest list; an item past
+        // the lane's list is its own record (rr = 0: it adds nothing)
+        auto walk = [&](auto ts_, auto fits_) __attribute__((always_inline)) {
+          constexpr bool TS = decltype(ts_)::value, kFits = decltype(fits_)::value;
+          const int ngw = (kmax + 7) >> 3;
+          uint4 gn = it[0];
+          for (int g = 0; g < ngw; g++) {
+            uint4 gc = gn;
+            asm volatile("" : "+v"(gc.x), "+v"(gc.y), "+v"(gc.z), "+v"(gc.w));
+            gn = it[(g + 1 < ngw ? g + 1 : g) * 64];
+            uint32_t w0 = gc.x, w1 = gc.y, w2 = gc.z, w3 = gc.w;
+#pragma unroll 1
+            for (int v = 0; v < 8; v++) {   // an item: the low half of w0, then the high half
+              const bool valid = 8 * g + v < cnt;
+              const uint32_t e = valid ? w0 & 0xFFFFu : uint32_t(p);
+              w0 = (v & 1) ? w1 : w0 >> 16;
+              if (v & 1) {
+                w1 = w2;
+                w2 = w3;
+              }
+              float4 qh, qv;
+              uint2 qc;
+              if (kFits || e < uint32_t(HS)) {
+                qh = hh[e];
+                qc = hq[e];
+                qv = hv[e];
+              } else {
+                int64_t q;
+                uint32_t code;
+                halo_entry(W, tile, ts, e, q, code);
+                float4 rl;
+                halo_rec<D>(W, J, q, code, qh, rl);
+                qc = pack_lo(rl);
+                qv = W.rec[4 * q + 2];
+              }
+              float lo[3];
+              int tw;
+              unpack_lo(qc, lo, tw);
+              float dr[3] = {0.f, 0.f, 0.f};
+              rel_hl<D>(o.xh, o.xl, qh, lo, dr);
+              float rr = 0.f;
 #pragma unroll
-        for (int c = 0; c < D; c++) rr += dr[c] * dr[c];
-        pair2<D>(J, o, tsurf, hdt, dr, rr, qh.w, __float_as_int(ql.w) & ~(kJumpBit | int(kHMir)), qv, true, F);
-      };
-      if (H <= kHL2) {
-        // the whole halo is in LDS: item k + 1's record is read while item k is summed
-        uint32_t en = pop(0);
-        en = 0 < cnt ? en : uint32_t(p);
-        float4 nh = hh[en], nl = hl[en], nv = hv[en];
-        for (int k = 0; k < cnt; k++) {
-          const float4 qh = nh, ql = nl, qv = nv;
-          uint32_t e2 = pop(k + 1);
-          e2 = k + 1 < cnt ? e2 : uint32_t(p);   // past the end: the particle's own record
-          nh = hh[e2];
-          nl = hl[e2];
-          nv = hv[e2];
-          body(qh, ql, qv);
-        }
-      } else {
-        for (int k = 0; k < cnt; k++) {
-          const uint32_t e = pop(k);
-          float4 qh, ql, qv;
-          if (e < uint32_t(HS)) {
-            qh = hh[e];
-            ql = hl[e];
-            qv = hv[e];
-          } else {
-            int64_t q;
-            uint32_t code;
-            halo_entry(W, tile, ts, e, q, code);
-            halo_rec<D>(W, J, q, code, qh, ql);
-            qv = W.rec[4 * q + 2];
+              for (int c = 0; c < D; c++) rr += dr[c] * dr[c];
+              pair2<D, TS>(J, o, hdt, dr, rr, qh.w, tw & int(kQType), qv, valid, F);
+            }
           }
-          body(qh, ql, qv);
+        };
+        if (H <= kHL2) {
+          if (tsurf) walk(std::true_type{}, std::true_type{});
+          else walk(std::false_type{}, std::true_type{});
+        } else {
+          walk(std::true_type{}, std::false_type{});
         }
       }
       // pairs with this step's jumpers (and, for a jumper, its own list), after the items
@@ -2247,7 +2295,7 @@ __global__ __launch_bounds__(kTile) void k_force2(Work W, PState S, int par) {
         float rr = 0.f;
 #pragma unroll
         for (int c = 0; c < D; c++) rr += dr[c] * dr[c];
-        pair2<D>(J, o, tsurf, hdt, dr, rr, qh.w, tq, W.rec[4 * q + 2], use && aligned, F);
+        pair2<D, true>(J, o, hdt, dr, rr, qh.w, tq, W.rec[4 * q + 2], use && aligned, F);
       }, true);
     }
     if (o.surf) {
