@@ -1983,9 +1983,8 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void
         // One item of the walk: the record (qh, ql) of halo entry e; q, code: its slot
         // and mirror code if already known (q < 0: looked up when a rare path needs
         // them).  An item past the lane's list (valid = false) is its own record and
-        // adds nothing.  Every item writes e at the lane's stage-2 position c2, which
-        // only a kept pair advances (so a later item overwrites a dropped one; c2 stays
-        // below the wavefront's item count, inside its part of the tile's items).
+        // adds nothing.  A kept pair (not self, not a mirror image) goes to the lane's
+        // stage-2 items.
         auto body = [&](auto ts_, uint32_t e, const float4& qh, const uint2& qc, int64_t q, uint32_t code,
                         bool valid) __attribute__((always_inline)) {
           constexpr bool TS = decltype(ts_)::value;
@@ -2015,8 +2014,10 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void
           const int tq = tw & int(kQType);
           if (TS && nb && o.surf && tq == SMG_SURFACE && q < 0) halo_entry(W, tile, ts, e, q, code);
           const bool f = pair1<D, TS>(W, J, o, dr, d2, qh.w, tq, e == uint32_t(p), q, acc, nb);
-          st2[(c2 >> 3) * 512 + (c2 & 7)] = uint16_t(e);
-          c2 += (f && !(tw & int(kQMir))) ? 1 : 0;
+          if (f && !(tw & int(kQMir))) {
+            st2[(c2 >> 3) * 512 + (c2 & 7)] = uint16_t(e);
+            c2++;
+          }
         };
         // The walk runs to the wavefront's longest list, so its loop and group loads are
         // the same in every lane; each group of eight items is unrolled.
