@@ -8,10 +8,11 @@
 // GeometryCalculator::Get_corner / Get_cutcell_volume / get_face / get_subvolume
 // (geometry_calculator.cpp:12-68, :166-241, :447-566) and
 // SPH_mesh_generation::Get_num_particles (sph_mesh_generation.cpp:900-941).
-// The cut-cell surface area is restated with marching squares (2D) and a
-// six-tetrahedron split (3D) instead of the marching-cube tables of polygonise.h;
-// both converge to the same isosurface area, and the particle count built on it is
-// pinned by closed forms (circle perimeter, sphere area), not bit for bit.
+// The cut-cell surface area restates Get_surface_area / get_area_MQ / get_area_tri
+// (geometry_calculator.cpp:73-157, :284-345) and Polygonise_MQ::Polygonise /
+// VertexInterp (polygonise.cpp:10-95): marching cubes over the 2x2x2 sub-cubes of the
+// cell's corner samples (a 2D cell as three equal layers of unit depth), triangle areas
+// as 0.5 |v3| |v1| sin(acos(c)) with the host's libm, summed in the reference's order.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -127,52 +128,93 @@ double get_subvolume(int lev, double multi, const double cor[2][2][2]) {
   return vof;
 }
 
-// isoline length in one square (marching squares, linear edge interpolation)
-double square_len(const double v[4], const double px[4], const double py[4]) {
-  // corners in cyclic order 0-1-2-3; edges (0,1),(1,2),(2,3),(3,0)
-  double ex[4], ey[4]; int ne = 0;
-  for (int e = 0; e < 4; e++) {
-    int a = e, b = (e + 1) & 3;
-    bool ia = v[a] < 0.0, ib = v[b] < 0.0;
-    if (ia != ib) {
-      double t = (0.0 - v[a]) / (v[b] - v[a]);
-      ex[ne] = px[a] + t * (px[b] - px[a]); ey[ne] = py[a] + t * (py[b] - py[a]); ne++;
-    }
-  }
-  // lengths as sqrt of the sum of squares (geometry_calculator.cpp:340-342)
-  auto len = [](double dx, double dy) { return std::sqrt(dx * dx + dy * dy); };
-  double L = 0.;
-  if (ne == 2) L = len(ex[1] - ex[0], ey[1] - ey[0]);
-  else if (ne == 4) L = len(ex[1] - ex[0], ey[1] - ey[0]) + len(ex[3] - ex[2], ey[3] - ey[2]);
-  return L;
+
+// ---- marching-cube area (get_area_MQ) ----
+// The published marching-cubes triangle table (the one polygonise.h carries), each
+// case's triangles as the hex digits of their edge indices, three per triangle.
+const char* const kMcTri[256] = {
+    "", "083", "019", "183981", "12a", "08312a", "92a029", "2832a8a98",
+    "3b2", "0b28b0", "19023b", "1b219b98b", "3a1ba3", "0a108a8ba", "3903b9ba9", "98aa8b",
+    "478", "430734", "019847", "419471731", "12a847", "34730412a", "92a902847", "2a9297273794",
+    "8473b2", "b47b24204", "90184723b", "47b94b9b2921", "3a13ba784", "1ba14b1047b4", "47890b9bab03", "47b4b99ba",
+    "954", "954083", "054150", "854835315", "12a954", "30812a495", "52a542402", "2a5325354348",
+    "95423b", "0b208b495", "05401523b", "21525828b485", "a3ba13954", "4950818a18ba", "54050b5bab03", "54858aa8b",
+    "978579", "930953573", "078017157", "153357", "978957a12", "a12950530573", "802825857a52", "2a5253357",
+    "7957893b2", "95797292027b", "23b018178157", "b21b17715", "958857a13a3b", "5705097b010aba0", "ba0b03a50807570", "ba57b5",
+    "a65", "0835a6", "9015a6", "1831985a6", "165261", "165126308", "965906026", "598582526328",
+    "23ba65", "b08b20a65", "01923b5a6", "5a61929b298b", "63b653513", "08b0b50515b6", "3b6036065059", "65969bb98",
+    "5a6478", "43047365a", "1905a6847", "a65197173794", "612651478", "125526304347", "847905065026", "739794329596269",
+    "3b2784a65", "5a647242027b", "01947823b5a6", "9219b294b7b45a6", "8473b53515b6", "51b5b610b7b404b", "059065036b63847", "65969b4797b9",
+    "a4964a", "4a649a083", "a01a60640", "83181686461a", "149124264", "308129249264", "024426", "832824426",
+    "a49a64b23", "08228b49a4a6", "3b201606461a", "64161a48121b8b1", "964936913b63", "8b1810b61914641", "3b6360064", "648b68",
+    "7a678a89a", "0730a709a67a", "a671a7178180", "a67a71173", "126168189867", "269291679093739", "780706602", "732672",
+    "23ba68a89867", "20727b09767a9a7", "1801781a767a23b", "b21b17a61671", "896867916b63136", "091b67", "7807063b0b60", "7b6",
+    "76b", "308b76", "019b76", "819831b76", "a126b7", "12a3086b7", "2902a96b7", "6b72a3a83a98",
+    "723627", "708760620", "276237019", "162186198876", "a76a17137", "a7617a187108", "03707a0a96a7", "76a7a88a9",
+    "684b86", "36b306046", "86b846901", "946963931b36", "6846b82a1", "12a30b06b046", "4b846b0292a9", "a93a32943b36463",
+    "823842462", "042462", "190234246438", "194142246", "8138618466a1", "a10a06604", "4634386a3039a93", "a946a4",
+    "49576b", "083495b76", "50154076b", "b76834354315", "954a1276b", "6b712a083495", "76b54a42a402", "348354325a52b76",
+    "723762549", "954086062687", "362376150540", "628687218485158", "954a16176137", "16a176107870954", "40a4a503a6a737a", "76a7a854a48a",
+    "6956b9b89", "36b063056095", "0b805b01556b", "6b3635531", "12a95b9b8b56", "0b306b09656912a", "b85b56805a52025", "6b36352a3a53",
+    "589528562382", "956960062", "158180568382628", "156216", "13616a386569896", "a10a06950560", "03856a", "a56",
+    "b5a75b", "b5ab75830", "5b75ab190", "a75ab7981831", "b12b71751", "08312717572b", "9759279022b7", "75272b592328982",
+    "25a235375", "820852875a25", "9015a35373a2", "982921872a25752", "135375", "087071175", "903935537", "987597",
+    "5845a8ab8", "5045b05abb30", "01984a8aba45", "ab4a45b34941314", "2512852b8458", "04b0b345b2b151b", "0250592b5458b85", "9452b3",
+    "25a352345384", "5a2524420", "3a235a385458019", "5a2524192942", "845853351", "045105", "845853905035", "945",
+    "4b749b9ab", "0834979b79ab", "1ab1b414074b", "3143481a474bab4", "4b79b492b912", "9749b791b2b1083", "b74b42240", "b74b42834324",
+    "29a279237749", "9a7974a27870207", "37a3a274a1a040a", "1a2874", "491417713", "491417081871", "403743", "487",
+    "9a8ab8", "30939bb9a", "01a0a88ab", "31ab3a", "12b1b99b8", "30939b1292b9", "02b80b", "32b",
+    "23828aa89", "9a2092", "23828a0181a8", "1a2", "138918", "091", "038", "",
+};
+// sub-cube vertex v at corner offset kMcV[v] (the reference's numbering: 1 is +y, 3 is
+// +x), and the two vertices of edge e
+const int kMcV[8][3] = {{0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 0, 0}, {0, 0, 1}, {0, 1, 1}, {1, 1, 1}, {1, 0, 1}};
+const int kMcE[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6}, {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+void mq_interp(const double* p1, const double* p2, double v1, double v2, double* out) {
+  const double iso = 0.0;
+  if (std::fabs(iso - v1) < 1.0e-15) { for (int c = 0; c < 3; c++) out[c] = p1[c]; return; }
+  if (std::fabs(iso - v2) < 1.0e-15) { for (int c = 0; c < 3; c++) out[c] = p2[c]; return; }
+  if (std::fabs(v1 - v2) < 1.0e-15) { for (int c = 0; c < 3; c++) out[c] = p1[c]; return; }
+  const double mu = (iso - v1) / (v2 - v1);
+  for (int c = 0; c < 3; c++) out[c] = p1[c] + mu * (p2[c] - p1[c]);
 }
 
-// isosurface area in one tetrahedron
-double tet_area(const double v[4], const double p[4][3]) {
-  int in[4], out[4], ni = 0, no = 0;
-  for (int a = 0; a < 4; a++) { if (v[a] < 0.0) in[ni++] = a; else out[no++] = a; }
-  if (ni == 0 || no == 0) return 0.;
-  auto cut = [&](int a, int b, double* q) {
-    double t = (0.0 - v[a]) / (v[b] - v[a]);
-    for (int k = 0; k < 3; k++) q[k] = p[a][k] + t * (p[b][k] - p[a][k]);
-  };
-  auto tri = [](const double* A, const double* B, const double* C) {
-    double u[3], w[3];
-    for (int k = 0; k < 3; k++) { u[k] = B[k] - A[k]; w[k] = C[k] - A[k]; }
-    double cx = u[1] * w[2] - u[2] * w[1], cy = u[2] * w[0] - u[0] * w[2], cz = u[0] * w[1] - u[1] * w[0];
-    return 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
-  };
-  if (ni == 1 || no == 1) {
-    int s = ni == 1 ? in[0] : out[0];
-    const int* o = ni == 1 ? out : in;
-    double q[3][3];
-    for (int k = 0; k < 3; k++) cut(s, o[k], q[k]);
-    return tri(q[0], q[1], q[2]);
-  }
-  double q[4][3];
-  cut(in[0], out[0], q[0]); cut(in[0], out[1], q[1]); cut(in[1], out[1], q[2]); cut(in[1], out[0], q[3]);
-  return tri(q[0], q[1], q[2]) + tri(q[0], q[2], q[3]);
+double mq_tri(const double* a, const double* b, const double* c) {
+  const double v1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const double v3[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  const double d1 = std::sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+  const double d3 = std::sqrt(v3[0] * v3[0] + v3[1] * v3[1] + v3[2] * v3[2]);
+  double area = 0.5 * (d3 * d1) * std::sin(std::acos((v1[0] * v3[0] + v1[1] * v3[1] + v1[2] * v3[2]) / (d3 * d1 + 1.e-20)));
+  if (std::isnan(area)) area = 0.;
+  return area;
 }
+
+double area_mq(const double C[3][3][3], double dx, double dy, double dz) {
+  double area = 0.0;
+  for (int i = 0; i < 2; i++) for (int j = 0; j < 2; j++) for (int k = 0; k < 2; k++) {
+    double p[8][3], val[8];
+    int idx = 0;
+    for (int v = 0; v < 8; v++) {
+      const int vx = i + kMcV[v][0], vy = j + kMcV[v][1], vz = k + kMcV[v][2];
+      p[v][0] = vx * 0.5 * dx; p[v][1] = vy * 0.5 * dy; p[v][2] = vz * 0.5 * dz;
+      val[v] = C[vx][vy][vz];
+      if (val[v] < 0.0) idx |= 1 << v;
+    }
+    const char* t = kMcTri[idx];
+    for (int q = 0; t[q]; q += 3) {
+      double tp[3][3];
+      for (int s = 0; s < 3; s++) {
+        const char ch = t[q + s];
+        const int e = ch <= '9' ? ch - '0' : ch - 'a' + 10;
+        mq_interp(p[kMcE[e][0]], p[kMcE[e][1]], val[kMcE[e][0]], val[kMcE[e][1]], tp[s]);
+      }
+      area += mq_tri(tp[0], tp[1], tp[2]);
+    }
+  }
+  return area;
+}
+
 
 }  // namespace
 
@@ -309,12 +351,9 @@ int or_lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin,
                       H += get_face(0, 0.25, sub);
                     }
                     H = std::min(H, 1.0);
-                    for (int a = 0; a < 2; a++) for (int b = 0; b < 2; b++) {
-                      double v4[4] = {face[a][b], face[a + 1][b], face[a + 1][b + 1], face[a][b + 1]};
-                      double px[4] = {a * 0.5 * dx, (a + 1) * 0.5 * dx, (a + 1) * 0.5 * dx, a * 0.5 * dx};
-                      double py[4] = {b * 0.5 * dy, b * 0.5 * dy, (b + 1) * 0.5 * dy, (b + 1) * 0.5 * dy};
-                      area += square_len(v4, px, py);
-                    }
+                    double ext[3][3][3];
+                    for (int r = 0; r < 3; r++) for (int s = 0; s < 3; s++) for (int t = 0; t < 3; t++) ext[r][s][t] = face[r][s];
+                    area = area_mq(ext, dx, dy, 1.0);
                   }
                   vol = H * dx * dy;
                 } else {
@@ -332,20 +371,7 @@ int or_lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin,
                       H += get_subvolume(0, 0.125, sub);
                     }
                     H = std::min(H, 1.0);
-                    static const int tets[6][4] = {{0, 1, 3, 7}, {0, 3, 2, 7}, {0, 2, 6, 7}, {0, 6, 4, 7}, {0, 4, 5, 7}, {0, 5, 1, 7}};
-                    for (int a = 0; a < 2; a++) for (int b = 0; b < 2; b++) for (int e = 0; e < 2; e++) {
-                      double cv[8], cp[8][3];
-                      for (int q = 0; q < 8; q++) {
-                        int qx = q & 1, qy = (q >> 1) & 1, qz = (q >> 2) & 1;
-                        cv[q] = C[a + qx][b + qy][e + qz];
-                        cp[q][0] = (a + qx) * 0.5 * dx; cp[q][1] = (b + qy) * 0.5 * dy; cp[q][2] = (e + qz) * 0.5 * dz;
-                      }
-                      for (int t = 0; t < 6; t++) {
-                        double tv4[4], tp[4][3];
-                        for (int q = 0; q < 4; q++) { tv4[q] = cv[tets[t][q]]; for (int m = 0; m < 3; m++) tp[q][m] = cp[tets[t][q]][m]; }
-                        area += tet_area(tv4, tp);
-                      }
-                    }
+                    area = area_mq(C, dx, dy, dz);
                   }
                   vol = H * dx * dy * dz;
                 }

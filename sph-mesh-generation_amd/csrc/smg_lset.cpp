@@ -17,9 +17,10 @@
 // or zeroed (zero_margin = 1): after Load_lset_restart_file only interior packages
 // are read back, so margin phi and normals are zero in the relaxation run (quirk Q9);
 // the tags always come from the first pass, as in the restart file.
-// Cut-cell surface area: marching squares (2D) / six tetrahedra per sub-cube (3D)
-// instead of the marching-cube triangle tables; see the particle-count note in the
-// design document.
+// Cut-cell surface area: the reference's marching cubes over the cell's 2x2x2
+// sub-cubes (Get_surface_area / get_area_MQ, geometry_calculator.cpp:73-157, :284-345),
+// a 2D cell as three equal layers of unit depth; smg_mc.hpp holds the form shared with
+// the GPU builder.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -30,6 +31,7 @@
 
 #include "smg.h"
 #include "smg_internal.hpp"
+#include "smg_mc.hpp"
 
 namespace smg {
 namespace {
@@ -166,56 +168,7 @@ double cube_fraction(int lev, double w, const double c[2][2][2]) {
   return acc;
 }
 
-inline void lerp_zero(const double* pa, const double* pb, double va, double vb, int n, double* out) {
-  const double t = -va / (vb - va);
-  for (int q = 0; q < n; q++) out[q] = pa[q] + t * (pb[q] - pa[q]);
-}
-
-double isoline_length(const double v[4], const double p[4][2]) {
-  double e[4][2];
-  int m = 0;
-  for (int s = 0; s < 4; s++) {
-    const int a = s, b = (s + 1) & 3;
-    if ((v[a] < 0.0) != (v[b] < 0.0)) lerp_zero(p[a], p[b], v[a], v[b], 2, e[m++]);
-  }
-  double L = 0.0;
-  // segment length as the reference measures lengths (sqrt of the sum of squares,
-  // geometry_calculator.cpp:340-342), also what the GPU builder evaluates
-  for (int q = 0; q + 1 < m; q += 2) {
-    const double dx = e[q + 1][0] - e[q][0], dy = e[q + 1][1] - e[q][1];
-    L += std::sqrt(dx * dx + dy * dy);
-  }
-  return L;
-}
-
-double tri_area(const double* A, const double* B, const double* C) {
-  const double u[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
-  const double w[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
-  const double x = u[1] * w[2] - u[2] * w[1], y = u[2] * w[0] - u[0] * w[2], z = u[0] * w[1] - u[1] * w[0];
-  return 0.5 * std::sqrt(x * x + y * y + z * z);
-}
-
-double isosurface_area_tet(const double v[4], const double p[4][3]) {
-  int neg[4], pos[4], nn = 0, np = 0;
-  for (int a = 0; a < 4; a++) {
-    if (v[a] < 0.0) neg[nn++] = a;
-    else pos[np++] = a;
-  }
-  if (nn == 0 || np == 0) return 0.0;
-  if (nn == 1 || np == 1) {
-    const int apex = (nn == 1) ? neg[0] : pos[0];
-    const int* rest = (nn == 1) ? pos : neg;
-    double q[3][3];
-    for (int s = 0; s < 3; s++) lerp_zero(p[apex], p[rest[s]], v[apex], v[rest[s]], 3, q[s]);
-    return tri_area(q[0], q[1], q[2]);
-  }
-  double q[4][3];
-  lerp_zero(p[neg[0]], p[pos[0]], v[neg[0]], v[pos[0]], 3, q[0]);
-  lerp_zero(p[neg[0]], p[pos[1]], v[neg[0]], v[pos[1]], 3, q[1]);
-  lerp_zero(p[neg[1]], p[pos[1]], v[neg[1]], v[pos[1]], 3, q[2]);
-  lerp_zero(p[neg[1]], p[pos[0]], v[neg[1]], v[pos[0]], 3, q[3]);
-  return tri_area(q[0], q[1], q[2]) + tri_area(q[0], q[2], q[3]);
-}
+constexpr uint64_t kMcTable[256] = SMG_MC_TRI_TABLE;
 
 // corner / edge / face / centre samples of a cut cell from its 3^D neighbourhood,
 // GeometryCalculator::Get_corner
@@ -267,12 +220,13 @@ void cut_cell_measure(const Grid& g, int i, int j, int k, double& vol, double& a
         for (int b = 0; b < 2; b++) {
           const double s[2][2] = {{C[a][b][0], C[a][b + 1][0]}, {C[a + 1][b][0], C[a + 1][b + 1][0]}};
           H += face_fraction(0, 0.25, s);
-          const double v[4] = {C[a][b][0], C[a + 1][b][0], C[a + 1][b + 1][0], C[a][b + 1][0]};
-          const double p[4][2] = {{a * 0.5 * dx, b * 0.5 * dy}, {(a + 1) * 0.5 * dx, b * 0.5 * dy},
-                                  {(a + 1) * 0.5 * dx, (b + 1) * 0.5 * dy}, {a * 0.5 * dx, (b + 1) * 0.5 * dy}};
-          area += isoline_length(v, p);
         }
       H = std::min(H, 1.0);
+      double E[3][3][3];
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+          for (int e = 0; e < 3; e++) E[a][b][e] = C[a][b][0];
+      area = mc_cell_area(E, dx, dy, 1.0, kMcTable);
     }
     vol = H * dx * dy;
     return;
@@ -284,29 +238,18 @@ void cut_cell_measure(const Grid& g, int i, int j, int k, double& vol, double& a
   double H = f == 27 ? 1.0 : 0.0;
   if (f != 27 && f != -27) {
     H = 0.0;
-    static const int tet[6][4] = {{0, 1, 3, 7}, {0, 3, 2, 7}, {0, 2, 6, 7}, {0, 6, 4, 7}, {0, 4, 5, 7}, {0, 5, 1, 7}};
     for (int a = 0; a < 2; a++)
       for (int b = 0; b < 2; b++)
         for (int e = 0; e < 2; e++) {
-          double s[2][2][2], cv[8], cp[8][3];
+          double s[2][2][2];
           for (int q = 0; q < 8; q++) {
             const int qx = q & 1, qy = (q >> 1) & 1, qz = (q >> 2) & 1;
-            s[qx][qy][qz] = cv[q] = C[a + qx][b + qy][e + qz];
-            cp[q][0] = (a + qx) * 0.5 * dx;
-            cp[q][1] = (b + qy) * 0.5 * dy;
-            cp[q][2] = (e + qz) * 0.5 * dz;
+            s[qx][qy][qz] = C[a + qx][b + qy][e + qz];
           }
           H += cube_fraction(0, 0.125, s);
-          for (int t = 0; t < 6; t++) {
-            double tv[4], tp[4][3];
-            for (int q = 0; q < 4; q++) {
-              tv[q] = cv[tet[t][q]];
-              for (int m = 0; m < 3; m++) tp[q][m] = cp[tet[t][q]][m];
-            }
-            area += isosurface_area_tet(tv, tp);
-          }
         }
     H = std::min(H, 1.0);
+    area = mc_cell_area(C, dx, dy, dz, kMcTable);
   }
   vol = H * dx * dy * dz;
 }

@@ -35,6 +35,7 @@
 
 #include "smg.h"
 #include "smg_internal.hpp"
+#include "smg_mc.hpp"
 
 namespace smg {
 namespace {
@@ -386,44 +387,31 @@ __device__ __forceinline__ void node2_of(const double C[3][3][3], int path, doub
 struct Q2 {
   int32_t cell, path;
 };
-__device__ __forceinline__ void lerp_zero(const double* pa, const double* pb, double va, double vb, int n, double* out) {
-  const double t = -va / (vb - va);
-  for (int q = 0; q < n; q++) out[q] = pa[q] + t * (pb[q] - pa[q]);
-}
 
-__device__ double tri_area(const double* A, const double* B, const double* C) {
-  const double u[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
-  const double w[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
-  const double x = u[1] * w[2] - u[2] * w[1], y = u[2] * w[0] - u[0] * w[2], z = u[0] * w[1] - u[1] * w[0];
-  return 0.5 * sqrt(x * x + y * y + z * z);
-}
+// the marching-cubes triangle table (smg_mc.hpp), for the cut-cell area
+__constant__ uint64_t kMcTableD[256] = SMG_MC_TRI_TABLE;
 
-__device__ double isosurface_area_tet(const double v[4], const double p[4][3]) {
-  int neg[4], pos[4], nn = 0, np = 0;
-  for (int a = 0; a < 4; a++) {
-    if (v[a] < 0.0) neg[nn++] = a;
-    else pos[np++] = a;
+// get_area_MQ of a cell with corner samples C over a group of lanes: lane t < 8 of the
+// group evaluates sub-cube t (sub-cube indices i, j, k with k fastest); every lane of the
+// group then adds all triangles in the reference's order, as mc_cell_area does
+__device__ double mc_area_lanes(const double C[3][3][3], double dx, double dy, double dz, int t, int width) {
+  double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int n = 0;
+  if (t < 8) n = mc_subcube_areas(C, t >> 2, (t >> 1) & 1, t & 1, dx, dy, dz, kMcTableD, a);
+  double area = 0.0;
+  for (int sb = 0; sb < 8; sb++) {
+    const int ns = __shfl(n, sb, width);
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      const double v = __shfl(a[q], sb, width);
+      if (q < ns) area += v;
+    }
   }
-  if (nn == 0 || np == 0) return 0.0;
-  if (nn == 1 || np == 1) {
-    const int apex = (nn == 1) ? neg[0] : pos[0];
-    const int* rest = (nn == 1) ? pos : neg;
-    double q[3][3];
-    for (int s = 0; s < 3; s++) lerp_zero(p[apex], p[rest[s]], v[apex], v[rest[s]], 3, q[s]);
-    return tri_area(q[0], q[1], q[2]);
-  }
-  double q[4][3];
-  lerp_zero(p[neg[0]], p[pos[0]], v[neg[0]], v[pos[0]], 3, q[0]);
-  lerp_zero(p[neg[0]], p[pos[1]], v[neg[0]], v[pos[1]], 3, q[1]);
-  lerp_zero(p[neg[1]], p[pos[1]], v[neg[1]], v[pos[1]], 3, q[2]);
-  lerp_zero(p[neg[1]], p[pos[0]], v[neg[1]], v[pos[0]], 3, q[3]);
-  return tri_area(q[0], q[1], q[2]) + tri_area(q[0], q[2], q[3]);
+  return area;
 }
 
 constexpr int kCutWaves = 4;
 
-// six tetrahedra around the main diagonal of a cube, corner q = qx + 2 qy + 4 qz
-__constant__ int8_t kSubTet[6][4] = {{0, 1, 3, 7}, {0, 3, 2, 7}, {0, 2, 6, 7}, {0, 6, 4, 7}, {0, 4, 5, 7}, {0, 5, 1, 7}};
 
 // 3D, first pass: one wavefront per listed cut cell, a lane per lev-1 node (8 sub-cubes
 // x 8 children).  Levels 0-2 are evaluated here and split lev-2 nodes go to the queue;
@@ -476,26 +464,7 @@ __global__ __launch_bounds__(64 * kCutWaves) void k_lset_cut3a(LGrid g, const in
     }
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
     H = part;
-    // area: lane = 6 * sub-cube + tetrahedron (sub-cubes a, b, e with e fastest),
-    // added in that order
-    double at_l = 0.0;
-    if (lane < 48) {
-      const int sb = lane / 6, t = lane % 6;
-      const int aa = (sb >> 2) & 1, bb = (sb >> 1) & 1, ee = sb & 1;
-      double cs[8];
-      select_child(C, sb, cs);
-      double tv[4], tp[4][3];
-      for (int q = 0; q < 4; q++) {
-        const int c = kSubTet[t][q];
-        const int qx = c & 1, qy = (c >> 1) & 1, qz = (c >> 2) & 1;
-        tv[q] = cs[4 * qx + 2 * qy + qz];
-        tp[q][0] = (aa + qx) * 0.5 * dx;
-        tp[q][1] = (bb + qy) * 0.5 * dy;
-        tp[q][2] = (ee + qz) * 0.5 * dz;
-      }
-      at_l = isosurface_area_tet(tv, tp);
-    }
-    for (int u = 0; u < 48; u++) area += __shfl(at_l, u);
+    area = mc_area_lanes(C, dx, dy, dz, lane, 64);
   }
   // queue range of the block: inclusive scan of nopen over the wavefront, then the
   // wavefronts' totals
@@ -769,20 +738,6 @@ __device__ double subtree2(const double* c0, int lev0, double w0) {
   }
 }
 
-__device__ double isoline_length(const double v[4], const double p[4][2]) {
-  double e[4][2];
-  int m = 0;
-  for (int s = 0; s < 4; s++) {
-    const int a = s, b = (s + 1) & 3;
-    if ((v[a] < 0.0) != (v[b] < 0.0)) lerp_zero(p[a], p[b], v[a], v[b], 2, e[m++]);
-  }
-  double L = 0.0;
-  for (int q = 0; q + 1 < m; q += 2) {
-    const double dx = e[q + 1][0] - e[q][0], dy = e[q + 1][1] - e[q][1];
-    L += sqrt(dx * dx + dy * dy);
-  }
-  return L;
-}
 
 // 2D: 16 lanes per listed cut cell, 4 cells per wavefront
 __global__ __launch_bounds__(64 * kCutWaves) void k_lset_cut2(LGrid g, const int32_t* __restrict__ list, int64_t nlist,
@@ -802,7 +757,7 @@ __global__ __launch_bounds__(64 * kCutWaves) void k_lset_cut2(LGrid g, const int
   // every lane of the group takes part in the shuffles below
   const int sq = lane >> 2, ch = lane & 3;
   const int a = sq >> 1, b = sq & 1;
-  double part = 0.0, len = 0.0, v;
+  double part = 0.0, v;
   if (split) {
     double s0[4] = {C[a][b][0], C[a][b + 1][0], C[a + 1][b][0], C[a + 1][b + 1][0]};
     if (node2(s0, 0, 0.25, v)) {
@@ -812,16 +767,15 @@ __global__ __launch_bounds__(64 * kCutWaves) void k_lset_cut2(LGrid g, const int
       child2(s0, ch, c1);
       part = subtree2(c1, 1, 0.25 * 0.25);
     }
-    if (ch == 0) {
-      const double vv[4] = {C[a][b][0], C[a + 1][b][0], C[a + 1][b + 1][0], C[a][b + 1][0]};
-      const double pp[4][2] = {{a * 0.5 * dx, b * 0.5 * dy}, {(a + 1) * 0.5 * dx, b * 0.5 * dy},
-                               {(a + 1) * 0.5 * dx, (b + 1) * 0.5 * dy}, {a * 0.5 * dx, (b + 1) * 0.5 * dy}};
-      len = isoline_length(vv, pp);
-    }
   }
   for (int off = 8; off > 0; off >>= 1) part += __shfl_xor(part, off, 16);
-  double area = 0.0;
-  for (int u = 0; u < 4; u++) area += __shfl(len, 4 * u, 16);
+  // the area as the reference measures a 2D cell: three equal layers of unit depth
+  double E[3][3][3];
+  for (int r = 0; r < 3; r++)
+    for (int q = 0; q < 3; q++)
+      for (int t = 0; t < 3; t++) E[r][q][t] = C[r][q][0];
+  const double am = mc_area_lanes(E, dx, dy, 1.0, lane, 16);   // every lane of the group
+  const double area = split ? am : 0.0;
   double H = f == 9 ? 1.0 : 0.0;
   if (split) H = part < 1.0 ? part : 1.0;
   if (live && lane == 0) {

@@ -115,3 +115,75 @@ def test_product_builder_equals_oracle(smg_mod, orc, dim):
             assert np.array_equal(P[k], O[k]), k
         for f, _ in smg_mod.SmgLsetStats._fields_:
             assert getattr(P["stats"], f) == getattr(O["stats"], f), f
+
+
+def _clip(poly, a, b, c):
+    """Sutherland-Hodgman: the part of polygon poly (k x 2) with a x + b y + c >= 0."""
+    out = []
+    k = len(poly)
+    for q in range(k):
+        P, Q = poly[q], poly[(q + 1) % k]
+        fp, fq = a * P[0] + b * P[1] + c, a * Q[0] + b * Q[1] + c
+        if fp >= 0:
+            out.append(P)
+        if (fp >= 0) != (fq >= 0):
+            t = fp / (fp - fq)
+            out.append(P + t * (Q - P))
+    return np.array(out)
+
+
+def _shoelace(poly):
+    x, y = poly[:, 0], poly[:, 1]
+    return 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
+
+
+def _plane_cfg(smg_mod, dim, npk):
+    cfg = smg_mod.default_config(dim)
+    for a in range(dim):
+        cfg.num_pkg[a] = npk
+    lo = np.array([cfg.box_l[a] for a in range(dim)])
+    hi = lo + np.array([cfg.domain[a] for a in range(dim)])
+    return cfg, lo, hi
+
+
+def test_mq_area_oblique_plane_3d(smg_mod, orc):
+    """Marching cubes is exact on a linear field: the cut cells' triangles tile the plane's
+    section of the box, whose area is the area of its projection on the xy plane over
+    |n_z| (the unit square clipped by 0 <= z(x, y) <= 1)."""
+    cfg, lo, hi = _plane_cfg(smg_mod, 3, 6)
+    n = np.array([0.3, -0.7, 0.45])
+    n /= np.linalg.norm(n)
+    centre = lo + np.array([0.47, 0.52, 0.61]) * (hi - lo)
+    d = -n @ centre
+    X, Y, Z = smg_mod.cell_centres_ext(cfg)
+    L = orc.lset_build(cfg, n[0] * X + n[1] * Y + n[2] * Z + d, zero_margin=False)
+    sq = np.array([[lo[0], lo[1]], [hi[0], lo[1]], [hi[0], hi[1]], [lo[0], hi[1]]])
+    # z(x, y) = -(n_x x + n_y y + d) / n_z within [lo_z, hi_z]
+    poly = _clip(sq, -n[0] / n[2], -n[1] / n[2], -d / n[2] - lo[2])
+    poly = _clip(poly, n[0] / n[2], n[1] / n[2], d / n[2] + hi[2])
+    exact = _shoelace(poly) / abs(n[2])
+    assert L["stats"].total_area == pytest.approx(exact, rel=1e-12)
+    P = smg_mod.lset_build(cfg, n[0] * X + n[1] * Y + n[2] * Z + d, zero_margin=False)
+    assert P["stats"].total_area == pytest.approx(exact, rel=1e-12)
+
+
+def test_mq_area_oblique_line_2d(smg_mod, orc):
+    """2D: the cell is measured as three equal layers of unit depth, so the area is the
+    length of the line's section of the box."""
+    cfg, lo, hi = _plane_cfg(smg_mod, 2, 8)
+    n = np.array([0.8, -0.35])
+    n /= np.linalg.norm(n)
+    centre = lo + np.array([0.43, 0.58]) * (hi - lo)
+    d = -n @ centre
+    X, Y, Z = smg_mod.cell_centres_ext(cfg)
+    L = orc.lset_build(cfg, n[0] * X + n[1] * Y + d, zero_margin=False)
+    # the line's ends on the box: y(x) = -(n_x x + d) / n_y at x = lo, hi clipped to [lo_y, hi_y]
+    ts = []
+    for x in (lo[0], hi[0]):
+        ts.append((x, -(n[0] * x + d) / n[1]))
+    for y in (lo[1], hi[1]):
+        ts.append((-(n[1] * y + d) / n[0], y))
+    pts = [p for p in ts if lo[0] - 1e-12 <= p[0] <= hi[0] + 1e-12 and lo[1] - 1e-12 <= p[1] <= hi[1] + 1e-12]
+    pts = np.array(sorted(pts))
+    exact = np.linalg.norm(pts[-1] - pts[0])
+    assert L["stats"].total_area == pytest.approx(exact, rel=1e-12)
