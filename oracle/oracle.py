@@ -60,6 +60,7 @@ def lib():
         L.or_num_threads.restype = C.c_int
         L.or_set_threads.argtypes = [C.c_int]
         L.or_set_position_rounding.argtypes = [C.c_int]
+        L.or_set_levels.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -88,6 +89,29 @@ def set_threads(n):
 
 def num_threads():
     return lib().or_num_threads()
+
+
+_levels_keep = None
+
+
+def set_levels(levels):
+    """Register refined level-set levels (make_levelset_levels' dicts, finest last) for
+    every later sample, tag and step (the multi-resolution extension); None clears them."""
+    global _levels_keep
+    L = lib()
+    if not levels:
+        L.or_set_levels(0, None, None, None, None, None)
+        _levels_keep = None
+        return
+    n = len(levels)
+    cfgs = (type(levels[0]["cfg"]) * n)(*[lv["cfg"] for lv in levels])
+    keep = [(np.ascontiguousarray(lv["phi"], np.float64), np.ascontiguousarray(lv["normal"], np.float32),
+             np.ascontiguousarray(lv["tag"], np.int8), np.ascontiguousarray(lv["active"], np.uint8)) for lv in levels]
+    arr = [(C.c_void_p * n)(*[_p(k[q]) for k in keep]) for q in range(4)]
+    rc = L.or_set_levels(n, C.byref(cfgs), *arr)
+    if rc != 0:
+        raise RuntimeError(f"or_set_levels failed ({rc})")
+    _levels_keep = (cfgs, keep, arr)
 
 
 def set_position_rounding(on):

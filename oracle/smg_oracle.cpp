@@ -98,7 +98,24 @@ struct Ctx {
   const double* phi;  // one-cell-halo grid
   const float* nrm;
   const int8_t* tag;
+  // refined level-set levels (the multi-resolution extension; none for the reference's
+  // single-level set): level l's grid and which of its packages are stored
+  int nlev = 0;
+  const Ctx* lv[3] = {nullptr, nullptr, nullptr};
+  const uint8_t* act[3] = {nullptr, nullptr, nullptr};
 };
+
+// the refined levels or_set_levels registered (applied to every context built after)
+struct LevelReg {
+  smg_config cfg;
+  const double* phi;
+  const float* nrm;
+  const int8_t* tag;
+  const uint8_t* active;
+};
+LevelReg g_lreg[3];
+int g_nlreg = 0;
+Ctx g_lctx[3];
 
 inline int64_t gidx(const Ctx& c, int gi, int gj, int gk) {  // global cell (may be -1 .. nc)
   int64_t i = gi + 1, j = (c.D >= 2) ? gj + 1 : 0, k = (c.D >= 3) ? gk + 1 : 0;
@@ -119,6 +136,26 @@ void ctx_init(Ctx& c, const smg_config* cfg, const double* phi, const float* nrm
     c.box_r[a] = cfg->box_l[a] + cfg->domain[a];
   }
   c.phi = phi; c.nrm = nrm; c.tag = tag;
+  c.nlev = 0;
+  if (phi && cfg != nullptr) {
+    for (int l = 0; l < g_nlreg; l++) {
+      Ctx& L = g_lctx[l];
+      L.cfg = &g_lreg[l].cfg; L.D = c.D; L.dl = 1.e20; L.nlev = 0;
+      for (int a = 0; a < 3; a++) {
+        bool on = a < c.D;
+        L.nc[a] = on ? 4 * g_lreg[l].cfg.num_pkg[a] : 1;
+        L.ng[a] = on ? L.nc[a] + 2 : 1;
+        L.dpkg[a] = on ? g_lreg[l].cfg.domain[a] / double(g_lreg[l].cfg.num_pkg[a]) : 1.0;
+        L.dcell[a] = on ? L.dpkg[a] / 4.0 : 1.0;
+        if (on) { L.dl = std::min(L.dl, L.dcell[a]); c.dl = std::min(c.dl, L.dcell[a]); }
+        L.box_r[a] = c.box_r[a];
+      }
+      L.phi = g_lreg[l].phi; L.nrm = g_lreg[l].nrm; L.tag = g_lreg[l].tag;
+      c.lv[l] = &L;
+      c.act[l] = g_lreg[l].active;
+      c.nlev = l + 1;
+    }
+  }
 }
 
 // clamped shift into the level-set box, particle_mesh_generation.cpp:182-185
@@ -143,7 +180,7 @@ inline int pkg_of(const Ctx& c, int a, double cs) {
 }
 
 // Calculate_particle_infor(SOLVER*), particle_mesh_generation.cpp:175-265
-void sample(const Ctx& c, const double* x, double& phi, double* n) {
+void sample_grid(const Ctx& c, const double* x, double& phi, double* n) {
   double cs[3]; coord_shift(c, x, cs);
   int pk[3]; double rr[3] = {0., 0., 0.};
   for (int a = 0; a < 3; a++) pk[a] = pkg_of(c, a, cs[a]);
@@ -174,9 +211,65 @@ void sample(const Ctx& c, const double* x, double& phi, double* n) {
   // particle_mesh_generation.cpp:262-263: the renormalised copy is discarded (Q3)
 }
 
+// A refined level stores package p (clamped into the level: a halo package is stored
+// where the interior package next to it is)
+bool level_stores(const Ctx& c, int l, const int* p) {
+  const Ctx& L = *c.lv[l];
+  int q[3] = {0, 0, 0};
+  for (int a = 0; a < c.D; a++) {
+    const int np = L.cfg->num_pkg[a];
+    q[a] = p[a] < 0 ? 0 : (p[a] > np - 1 ? np - 1 : p[a]);
+  }
+  const int np0 = L.cfg->num_pkg[0], np1 = c.D >= 2 ? L.cfg->num_pkg[1] : 1;
+  return c.act[l][(int64_t(q[2]) * np1 + q[1]) * np0 + q[0]] != 0;
+}
+
+// The sample on the finest refined level that stores the packages of all 2^D corner
+// cells of the interpolation, else on the base grid (the multi-resolution extension)
+void sample(const Ctx& c, const double* x, double& phi, double* n) {
+  for (int l = c.nlev - 1; l >= 0; l--) {
+    const Ctx& L = *c.lv[l];
+    double cs[3]; coord_shift(L, x, cs);
+    int pk[3] = {0, 0, 0}, m[3] = {0, 0, 0};
+    for (int a = 0; a < c.D; a++) {
+      pk[a] = pkg_of(L, a, cs[a]);
+      double rr = (cs[a] - L.dpkg[a] * pk[a]) / L.dcell[a] - 0.5;
+      rr = (rr <= 3.5) ? rr : 3.5;
+      rr = (-0.5 >= rr) ? -0.5 : rr;
+      m[a] = int(std::floor(rr));
+    }
+    bool ok = true;
+    for (int corner = 0; corner < (1 << c.D) && ok; corner++) {
+      int p[3] = {0, 0, 0};
+      for (int a = 0; a < c.D; a++) {
+        const int g = pk[a] * 4 + m[a] + ((corner >> a) & 1);
+        p[a] = (g + 4) / 4 - 1;   // floor(g / 4), g >= -4
+      }
+      ok = level_stores(c, l, p);
+    }
+    if (ok) { sample_grid(L, x, phi, n); return; }
+  }
+  sample_grid(c, x, phi, n);
+}
+
+int tag_grid(const Ctx& c, const double* x);
+
+// the tag of the cell holding x on the finest refined level that stores that cell's
+// package, else on the base grid
+int tag_at(const Ctx& c, const double* x) {
+  for (int l = c.nlev - 1; l >= 0; l--) {
+    const Ctx& L = *c.lv[l];
+    double cs[3]; coord_shift(L, x, cs);
+    int p[3] = {0, 0, 0};
+    for (int a = 0; a < c.D; a++) p[a] = pkg_of(L, a, cs[a]);
+    if (level_stores(c, l, p)) return tag_grid(L, x);
+  }
+  return tag_grid(c, x);
+}
+
 // Get_level_set_char_cell_tag_and_idx -> Levelset_levelinfo::Get_id_pkg_cell
 // (particle_mesh_generation.cpp:552-573, lset_level_infor.cpp:350-366)
-int tag_at(const Ctx& c, const double* x) {
+int tag_grid(const Ctx& c, const double* x) {
   double cs[3]; coord_shift(c, x, cs);
   int g[3] = {0, 0, 0};
   for (int a = 0; a < c.D; a++) {
@@ -557,6 +650,16 @@ void do_map_all(const Ctx& c, std::vector<P>& ps, uint64_t stream, bool scale_af
 // ===========================================================================
 // C API for the tests (ctypes).  Particle arrays are by id, x/v as n*3 doubles.
 extern "C" {
+
+// Registers n refined level-set levels (finest last) for every later call, or clears
+// them (n = 0).  The arrays must stay alive while they are registered.
+int or_set_levels(int n, const smg_config* cfgs, const double* const* phi, const float* const* nrm,
+                  const int8_t* const* tag, const uint8_t* const* active) {
+  if (n < 0 || n > 3) return SMG_ERR_ARG;
+  for (int l = 0; l < n; l++) g_lreg[l] = LevelReg{cfgs[l], phi[l], nrm[l], tag[l], active[l]};
+  g_nlreg = n;
+  return SMG_OK;
+}
 
 struct or_state {
   int64_t n;

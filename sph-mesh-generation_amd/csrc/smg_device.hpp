@@ -125,6 +125,7 @@ struct JobRed {
   int64_t step;                    // steps taken by this job (re-seeding RNG stream)
   unsigned long long drift;        // max_i (|x_i - x_b| + cut_off |h_i - h_b|) / h_b this step
   unsigned long long spill_used;   // entries of the job's spill part handed out
+  unsigned long long items_used;   // items of the job's part of the item pool handed out
   int64_t nrebuild;                // list builds so far
   int32_t nremap[2];
   int32_t fail, overflow;
@@ -185,12 +186,14 @@ struct Work {
   int32_t* ntrue;   // neighbours within the reference's cutoff (the force passes' count)
   // neighbour tiles (see kHashN): per tile t, halo entries 256.. at halo[t * ht] as
   // (slot | code << 26), hcnt[t] = 256 + their number (0: not built); items of slot s at
-  // items[t * 256 * ips + ioff[s] ..] (nbc[s] of them); lane[t * 256 + r] = the tile's
-  // particle of list-length rank r (the pair passes' thread order)
+  // items[tbase[t] + ioff[s] ..] (nbc[s] of them), tile t's items taken from its job's
+  // part of the pool (ips per slot on average); lane[t * 256 + r] = the tile's particle
+  // of list-length rank r (the pair passes' thread order)
   uint16_t* items;
   uint16_t* items2;  // per slot, from the same offset: its stage-2 pairs (nbc2 of them)
   uint32_t* halo;
   int32_t* hcnt;
+  int64_t* tbase;
   int32_t* ioff;
   int32_t* nbc2;
   uint8_t* lane;      // lane order of the pair passes

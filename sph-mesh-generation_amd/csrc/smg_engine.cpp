@@ -273,8 +273,9 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   if (sps_ < 1) { set_error("setup: too many slots for the spill pool"); return SMG_ERR_CAPACITY; }
   W_.spill_cap = int64_t(sps_) * cap_total_;
   W_.spill = alloc_<int32_t>(W_.spill_cap);
-  // neighbour tiles: a tile holds 256 (K + spill_per_slot) items (the lists' average
-  // capacity) and (K + spill_per_slot) 32 halo entries (at least 4096, at most 65536)
+  // neighbour tiles: a job's tiles take their items from a pool of (K + spill_per_slot)
+  // items per slot (the lists' average capacity); a tile holds (K + spill_per_slot) 32
+  // halo entries (at least 4096, at most 65536)
   W_.ips = int64_t(K_) + sps_;
   W_.ht = std::min<int64_t>(kMaxHalo, int64_t(kTile) * std::max<int64_t>(16, W_.ips / 8));
   const int64_t ntl = cap_total_ / kTile;
@@ -282,6 +283,7 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   W_.items2 = alloc_<uint16_t>(W_.ips * cap_total_);
   W_.halo = alloc_<uint32_t>(std::max<int64_t>(ntl, 1) * W_.ht);
   W_.hcnt = alloc_<int32_t>(std::max<int64_t>(ntl, 1));
+  W_.tbase = alloc_<int64_t>(std::max<int64_t>(ntl, 1));
   W_.ioff = alloc_<int32_t>(cap_total_);
   W_.nbc2 = alloc_<int32_t>(cap_total_);
   W_.lane = alloc_<uint8_t>(cap_total_);
@@ -946,8 +948,9 @@ int Engine::check_health() {
       return SMG_ERR_CAPACITY;
     }
     if (ov & kOverItems) {
-      set_error("job " + std::to_string(j) + ": a tile's lists exceed its " + std::to_string(W_.ips * kTile) +
-                " pair items (max_neighbors + spill_per_slot per particle); raise max_neighbors or spill_per_slot");
+      set_error("job " + std::to_string(j) + ": its lists, padded to each wavefront's longest, exceed its " +
+                std::to_string(W_.ips) +
+                " pair items per particle (max_neighbors + spill_per_slot); raise max_neighbors or spill_per_slot");
       return SMG_ERR_CAPACITY;
     }
     if (ov) {

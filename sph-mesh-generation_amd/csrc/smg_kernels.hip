@@ -799,6 +799,7 @@ __global__ __launch_bounds__(64) void k_decide(Work W, int njobs, int force) {
     if (l == 0) {
       R.rebuild = 1;
       R.spill_used = 0ull;
+      R.items_used = 0ull;
       R.njump = 0;
       R.jover = 0;
       if (!force) R.nrebuild += 1;
@@ -1552,6 +1553,7 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W) {
   __shared__ int s_scan[4];
   __shared__ int s_len[kTile];
   __shared__ int s_cnt, s_over;
+  __shared__ unsigned long long s_ibase;
   const int64_t tile = tile_id(W);
   const int j = W.tile_job[tile];
   if (j < 0 || !W.red[j].rebuild) return;
@@ -1595,11 +1597,18 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W) {
       W.hcnt[tile] = 0;
     }
   };
-  if (int64_t(M) > int64_t(kTile) * W.ips) {   // M is the same in every thread
+  // the tile's M items from its job's part of the pool (where a tile lands depends on
+  // the order of the atomics; what it holds does not)
+  if (threadIdx.x == 0) s_ibase = atomicAdd(&W.red[j].items_used, (unsigned long long)M);
+  __syncthreads();
+  const unsigned long long ib = s_ibase;   // the same in every thread, like M
+  if (ib + (unsigned long long)M > (unsigned long long)(J.cap * W.ips)) {
     fail(kOverItems);
     return;
   }
-  uint16_t* it0 = W.items + tile * kTile * W.ips + excl;
+  const int64_t tb = J.p0 * W.ips + int64_t(ib);
+  if (threadIdx.x == 0) W.tbase[tile] = tb;
+  uint16_t* it0 = W.items + tb + excl;
   auto item_at = [&](int k) -> uint16_t& { return it0[(k >> 3) * 512 + (k & 7)]; };
   uint32_t* hal = W.halo + tile * W.ht;
   const int cap = int(W.ht < int64_t(kMaxHalo) ? W.ht : int64_t(kMaxHalo)) - kTile;
@@ -1976,7 +1985,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void
     if (!o.jump) {   // a jumper's list is stale: it walks its own list below
       int c2 = 0;
       if (kmax > 0) {
-        const int64_t i0 = tile * kTile * W.ips + W.ioff[slot];
+        const int64_t i0 = W.tbase[tile] + W.ioff[slot];
         // items eight to a 16-byte group, groups 64 apart (the wavefront's lanes between)
         const uint4* it = reinterpret_cast<const uint4*>(W.items + i0);
         uint16_t* st2 = W.items2 + i0;   // the same group layout
@@ -2221,7 +2230,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void
     float F[3] = {0.f, 0.f, 0.f};
     if (o.eta != 0.f) {   // eta_ij = 0 for every pair when this particle's nu is 0
       if (kmax > 0) {
-        const uint4* it = reinterpret_cast<const uint4*>(W.items2 + tile * kTile * W.ips + W.ioff[slot]);
+        const uint4* it = reinterpret_cast<const uint4*>(W.items2 + W.tbase[tile] + W.ioff[slot]);
         // the items of the stage-1 walk, to the wavefront's longest list; an item past
         // the lane's list is its own record (rr = 0: it adds nothing)
         auto walk = [&](auto ts_, auto fits_) __attribute__((always_inline)) {

@@ -45,7 +45,7 @@ Sizes sizes_for(int K, int sps) {
   const int64_t ips = int64_t(K) + sps;
   const int64_t ht = std::min<int64_t>(smg::kMaxHalo, int64_t(smg::kTile) * std::max<int64_t>(16, ips / 8));
   z.per_slot = 2 * state + 2 * 4 + 6 * 4 + 4 + 4 * 16 + 5 * 4 + int64_t(K) * 4 + 4 * 4 + 16 + int64_t(sps) * 4 +
-               ips * 4 + ht / smg::kTile * 4 + 4 + 4 + 1 + 1;
+               ips * 4 + ht / smg::kTile * 4 + 4 + 4 + 1 + 1 + 1;
   z.per_table = 2 * 4 + 1;
   z.per_cell = smg::kLsCellBytes;
   z.per_job = int64_t(sizeof(smg::JobParams) + sizeof(smg::JobRed) + smg::kAccCopies * sizeof(smg::JobAcc));

@@ -1,0 +1,231 @@
+"""Single-step parity and long-run statistics on the configurations beyond the two
+base cases of test_gpu_parity.py: C1 with h = 1.3 dx, C2 at full size (256^3 level
+set, 989,563 particles), the literal C2 spacing profile over 1000 steps, the tanh
+spacing profile, a batch of jobs from 1e3 to 2e5 particles (2D and 3D), a server mix
+of 5e3-5e5-particle jobs with a retired job, and steps where particles are projected
+onto the surface and re-seeded.
+
+Bars, as in test_gpu_parity.py: neighbour sets, positions, types and h bit-exact; FP32
+pair terms within 2e-6 of the largest magnitude; the displacement a step sets up
+(v dt2) within 1e-5 of the largest, with the per-particle relative error reported;
+spacing statistics within 1% (min 5%)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from gpu_util import coincident, df_round, disp_check, spacing_stats, step_parity
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def test_c1_h13_singlestep_parity(smg_mod, orc_stored):
+    cfg, sdf = smg_mod.config_c1(h_over_dx=1.3)
+    lset = smg_mod.make_levelset(cfg, sdf)
+    job = smg_mod.Job(cfg, lset)
+    job.seed(int(lset["stats"].num_particle))
+    job.step(40)
+    res = step_parity(orc_stored, cfg, lset, job, 3)
+    assert min(r["n_surface"] for r in res) > 0
+
+
+def test_c1_h13_spacing_500_steps(smg_mod, orc):
+    cfg, sdf = smg_mod.config_c1(h_over_dx=1.3)
+    lset = smg_mod.make_levelset(cfg, sdf)
+    n = int(lset["stats"].num_particle)
+    job = smg_mod.Job(cfg, lset)
+    job.seed(n)
+    job.step(500)
+    g = job.state()
+    o = orc.seed(cfg, lset, n)
+    ex = coincident(o.x)
+    assert ex.sum() < 1e-2 * n
+    orc.prerun(cfg, lset, o)
+    for k in range(1, 501):
+        orc.step(cfg, lset, o, k)
+    sg = spacing_stats(g["x"], g["h"], g["type"], cfg, 2, ex)
+    so = spacing_stats(o.x, o.h, o.type, cfg, 2, ex)
+    _say("gpu", sg, "oracle", so)
+    for key in ("mean", "std", "p1", "p99"):
+        assert sg[key] == pytest.approx(so[key], rel=1e-2), key
+    assert sg["min"] == pytest.approx(so["min"], rel=5e-2)
+
+
+def test_c2_full_size_singlestep_parity(smg_mod, orc_stored):
+    """The bench workload itself: 989,563 particles on the 256^3 level set."""
+    cfg, sdf = smg_mod.config_c2()
+    lset = smg_mod.make_levelset(cfg, sdf)
+    n = int(lset["stats"].num_particle)
+    assert n == 989563
+    job = smg_mod.Job(cfg, lset)
+    job.seed(n)
+    job.step(5)
+    res = step_parity(orc_stored, cfg, lset, job, 1, terms=True)
+    assert res[0]["n_surface"] > 0
+    job.close()
+
+
+def test_c2_literal_profile_1000_steps(smg_mod, orc):
+    """The configured C2 spacing profile (scale 1: dx 0.008 -> 0.024, ~1.8e5 particles)
+    relaxed 1000 steps, against the oracle's 1000-step statistics
+    (tests/golden/c2_literal_1000.json, written by tests/golden/make_relaxed_stats.py)."""
+    gold = json.load(open(os.path.join(HERE, "golden", "c2_literal_1000.json")))
+    cfg, sdf = smg_mod.config_c2(scale=1.0)
+    lset = smg_mod.make_levelset(cfg, sdf)
+    n = int(lset["stats"].num_particle)
+    assert n == gold["n"]
+    ex = coincident(orc.seed(cfg, lset, n).x)
+    assert int(ex.sum()) == gold["excluded"]
+    job = smg_mod.Job(cfg, lset)
+    job.seed(n)
+    job.step(gold["steps"])
+    g = job.state()
+    sg = spacing_stats(g["x"], g["h"], g["type"], cfg, 3, ex)
+    so = gold["stats"]
+    _say("gpu", sg, "oracle", so, "surface", int((g["type"] == 1).sum()), gold["n_surface"])
+    for key in ("mean", "std", "p1", "p99"):
+        assert sg[key] == pytest.approx(so[key], rel=1e-2), key
+    assert sg["min"] == pytest.approx(so["min"], rel=5e-2)
+    job.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_tanh_profile_singlestep_parity(smg_mod, orc_stored, dim):
+    """The reference's default spacing profile, dx = min_dl + tanh(2.5 phi / phi_max) /
+    tanh(2.5) (max_dl - min_dl), with min_dl != max_dl."""
+    cfg = smg_mod.default_config(dim)
+    cfg.profile = smg_mod.PROFILE_TANH
+    for a in range(dim):
+        cfg.num_pkg[a] = 32 if dim == 2 else 12
+    cfg.min_dl, cfg.max_dl = (0.006, 0.02) if dim == 2 else (0.012, 0.04)
+    sdf = smg_mod.sdf_circle_hole((0.5,) * dim, 0.23)
+    lset = smg_mod.make_levelset(cfg, sdf)
+    # Get_scale's phi_max is the level set's maximum_phi (the engine takes it from the
+    # level set when the config leaves it 0; the oracle reads it from the config)
+    cfg.max_phi = lset["stats"].maximum_phi
+    job = smg_mod.Job(cfg, lset)
+    job.seed(int(lset["stats"].num_particle))
+    job.step(20)
+    h = job.state()["h"]
+    assert h.max() > 1.5 * h.min()   # the spacing does vary
+    res = step_parity(orc_stored, cfg, lset, job, 2, h_ulps=4)
+    assert min(r["n_surface"] for r in res) > 0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_remapped_particles_parity(smg_mod, orc_stored, dim):
+    """Particles placed inside the hole are re-seeded (Remap_this_particle_to_positive_
+    phase) and particles in cut cells projected onto the surface: the pre-run that does
+    it and the steps after it, against the oracle."""
+    orc = orc_stored
+    if dim == 2:
+        cfg, sdf = smg_mod.config_c1()
+    else:
+        cfg, sdf = smg_mod.config_c2(scale=2.0, num_pkg=16)
+    lset = smg_mod.make_levelset(cfg, sdf)
+    n = int(lset["stats"].num_particle)
+    rng = np.random.default_rng(5)
+    x = df_round(rng.uniform(0.02, 0.98, size=(n, dim)))   # over the whole box, hole included
+    job = smg_mod.Job(cfg, lset)
+    job.init_particles(x)
+    job.prerun()
+    st = job.stats()
+    g = job.state()
+    o = orc.State.from_arrays(x, np.zeros((n, dim)), np.zeros(n), np.zeros(n), np.zeros(n, np.int32), 0.0)
+    orc.prerun(cfg, lset, o)
+    _say("prerun: remapped", st["n_remapped"], "surface", st["n_surface"])
+    assert st["n_remapped"] > 0 and st["n_surface"] > 0
+    assert np.array_equal(g["x"][:, :dim], o.x[:, :dim])
+    assert np.array_equal(g["type"], o.type)
+    assert np.array_equal(g["h"], o.h)
+    res = step_parity(orc, cfg, lset, job, 3)
+    assert min(r["n_surface"] for r in res) > 0
+    job.close()
+
+
+def test_batch_1e3_to_2e5_per_job_parity(smg_mod, orc_stored):
+    """A batch of 2D and 3D jobs from 1e3 to 2e5 particles stepped together: every job's
+    step from the batch's state against the oracle's."""
+    orc = orc_stored
+    rng = np.random.default_rng(41)
+    targets = [(2, 1000), (3, 3000), (2, 20000), (3, 60000), (2, 200000), (3, 200000)]
+    jobs = [smg_mod.union_job(dim, nt, rng, 41 + 7919 * q) for q, (dim, nt) in enumerate(targets)]
+    ns = [n for _, _, n in jobs]
+    _say("jobs", ns)
+    assert min(ns) < 2000 and max(ns) > 150000
+    b = smg_mod.Batch(jobs)
+    b.seed()
+    b.step(10)
+    for k in range(2):
+        before = []
+        for j, (cfg, ls, n) in enumerate(jobs):
+            s = b.state(j)
+            st = b.stats(j)
+            before.append(orc.State.from_arrays(s["x"], s["v"], s["h"], s["nu"].astype(np.float64), s["type"], st["dt2"]))
+        step_idx = b.stats(0)["step"] + 1
+        b.step(1)
+        for j, (cfg, ls, n) in enumerate(jobs):
+            ref = before[j]
+            d = orc.step(cfg, ls, ref, step_idx)
+            g1 = b.state(j)
+            st = b.stats(j)
+            assert np.array_equal(g1["type"], ref.type), j
+            assert np.array_equal(g1["h"], ref.h), j
+            assert np.array_equal(g1["x"], ref.x), j
+            assert st["dt2"] == pytest.approx(d["dt2"], rel=1e-5), j
+            r = disp_check(g1, st, ref)
+            _say(f"job {j} n={n} dim={cfg.dim}", r)
+    b.close()
+
+
+def test_server_mix_with_retired_job(smg_mod, orc_stored):
+    """A server pool with 3D jobs from 5e3 to 5e5 particles: the smallest runs to its
+    step limit and retires while the others continue; a job admitted after the
+    retirement, in the range the retired job freed, gives the bits of the same job
+    alone on the single-job path, whose next step matches the oracle's."""
+    orc = orc_stored
+    rng = np.random.default_rng(53)
+    jobs = [smg_mod.union_job(3, nt, rng, 53 + 7919 * q) for q, nt in enumerate((5000, 30000, 150000, 500000))]
+    ns = [n for _, _, n in jobs]
+    _say("jobs", ns)
+    assert ns[0] < 10000 and ns[-1] > 300000
+    chunk = 10
+    scfg = smg_mod.server_config(3, 8, 1 << 21, 1 << 24, 1 << 23, chunk_steps=chunk, min_steps=2 * chunk,
+                                 max_steps=3 * chunk, tol=0.0)
+    srv = smg_mod.Server(scfg)
+    first = srv.admit(*jobs[0])
+    srv.advance()
+    others = [srv.admit(*jobs[q]) for q in (2, 3)]   # one chunk behind the first
+    assert first is not None and None not in others
+    done = {}
+    late = None
+    for k in range(40):
+        srv.advance()
+        for f in srv.finished():
+            done[f["ticket"]] = f
+        if first in done and late is None:
+            assert not any(t in done for t in others)   # the first retired alone
+            late = srv.admit(*jobs[1])   # into the freed range, the others still running
+            assert late is not None
+        if late is not None and late in done:
+            break
+    assert late is not None and late in done
+    steps = done[late]["steps"]
+    r = srv.result(late)
+    srv.close()
+    cfg, ls, n = jobs[1]
+    job = smg_mod.Job(cfg, ls)
+    job.seed(n)
+    job.step(steps)
+    a = job.state()
+    for key in ("x", "type", "h"):
+        assert np.array_equal(a[key], r[key]), key
+    step_parity(orc, cfg, ls, job, 1, terms=False)
+    job.close()
