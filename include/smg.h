@@ -131,7 +131,11 @@ int smg_lset_build(const smg_config* cfg, const double* phi_ext, int zero_margin
  * the uniform and linear spacing profiles, masses and particle counts are bit-identical
  * to the host builder's; with the tanh profile the per-cell spacing uses the device tanh
  * (within 2 ulp).  Replaces the same reference routines as smg_lset_build.  kernel_ms
- * (may be NULL): device time of the build, transfers excluded.  No GPU: SMG_ERR_HIP. */
+ * (may be NULL): device time of the build, transfers excluded.  No GPU: SMG_ERR_HIP.
+ * The 3D cut-cell pass queues split subdivision nodes in a buffer sized at 96 entries
+ * per cut cell and regrown when short; the environment variable SMG_LSET_QUEUE_PER_CELL
+ * (an integer in [1, 2^20]) overrides the 96 for tests of the regrow, and any other
+ * value gives SMG_ERR_ARG. */
 int smg_lset_build_gpu(const smg_config* cfg, const double* phi_ext, int zero_margin,
                        double* phi, float* normal, int8_t* tag, smg_lset_stats* stats, double* kernel_ms);
 

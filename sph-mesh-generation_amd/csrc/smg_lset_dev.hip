@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -962,11 +963,24 @@ int lset_build_device(const smg_config* cfg, const double* phi_ext, int zero_mar
   if (nlist > n_int) { set_error("smg_lset_build_gpu: cut-cell list overflow"); rc = SMG_ERR_HIP; goto done; }
   if (nlist > 0) {
     if (g.D == 3) {
-      // about 50 split lev-2 nodes per cut cell on smooth surfaces; a cell that finds
-      // the queue full walks its nodes itself
-      // SMG_LSET_QUEUE_PER_CELL overrides the 96 (a test reaches the regrow this way)
+      // about 50 split lev-2 nodes per cut cell on smooth surfaces; k_lset_cut3a drops
+      // the entries past qcap but still counts them, and the host then regrows the
+      // queue to the count and runs the pass again.
+      // The environment variable SMG_LSET_QUEUE_PER_CELL (a positive integer) overrides
+      // the 96 entries per cut cell, so that a test can force the regrow; any other
+      // value is an error.
       int64_t per_cell = 96;
-      if (const char* env = std::getenv("SMG_LSET_QUEUE_PER_CELL")) per_cell = std::max(1L, std::atol(env));
+      if (const char* env = std::getenv("SMG_LSET_QUEUE_PER_CELL")) {
+        char* end = nullptr;
+        errno = 0;
+        const long v = std::strtol(env, &end, 10);
+        if (end == env || *end != '\0' || errno != 0 || v < 1 || v > (1L << 20)) {
+          set_error("smg_lset_build_gpu: SMG_LSET_QUEUE_PER_CELL must be an integer in [1, 2^20]");
+          rc = SMG_ERR_ARG;
+          goto done;
+        }
+        per_cell = v;
+      }
       qcap = per_cell == 96 ? std::max<int64_t>(96 * nlist, int64_t(1) << 16) : per_cell * nlist;
       for (int pass = 0; pass < 2; pass++) {   // the second pass only if the queue was short
         HIP_TRY_LS(hipMalloc(&d_queue, sizeof(Q2) * qcap));

@@ -76,11 +76,18 @@ class SPH_mesh_generation {
     phi_.assign(m, 0.);
     normal_.assign(3 * m, 0.f);
     tag_.assign(m, 0);
-    // the level-set finishing on the GPU (bit-identical to smg_lset_build for the
-    // uniform and linear spacing profiles)
-    check(smg_lset_build_gpu(&cfg_, c.phi_ext.data(), c.zero_margin, phi_.data(), normal_.data(), tag_.data(), &lset_,
-                             nullptr),
-          "smg_lset_build_gpu");
+    // the level-set finishing on the GPU, bit-identical to smg_lset_build for the
+    // uniform (tanh with minimum_dl == maximum_dl) and linear spacing profiles; a tanh
+    // profile with minimum_dl != maximum_dl has masses that depend on the device tanh
+    // (within 2 ulp, the particle count within 1), so it keeps the host builder and
+    // its count
+    if (cfg_.profile == SMG_PROFILE_TANH && cfg_.min_dl != cfg_.max_dl)
+      check(smg_lset_build(&cfg_, c.phi_ext.data(), c.zero_margin, phi_.data(), normal_.data(), tag_.data(), &lset_),
+            "smg_lset_build");
+    else
+      check(smg_lset_build_gpu(&cfg_, c.phi_ext.data(), c.zero_margin, phi_.data(), normal_.data(), tag_.data(),
+                               &lset_, nullptr),
+            "smg_lset_build_gpu");
     total_num_particle = lset_.num_particle;
     check(smg_create(&cfg_, &job_), "smg_create");
     check(smg_set_levelset(job_, phi_.data(), normal_.data(), tag_.data()), "smg_set_levelset");

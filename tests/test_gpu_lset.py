@@ -141,3 +141,7 @@ def test_short_queue_is_regrown(smg_mod, monkeypatch):
         assert getattr(ref["stats"], f) == getattr(G["stats"], f), f
     P = smg_mod.lset_build(cfg, ext)
     assert P["stats"].total_volume == G["stats"].total_volume
+    for bad in ("1x", "", "0", "-3"):
+        monkeypatch.setenv("SMG_LSET_QUEUE_PER_CELL", bad)
+        with pytest.raises(smg_mod.SmgError):
+            smg_mod.lset_build_gpu(cfg, ext)
