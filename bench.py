@@ -442,13 +442,17 @@ def serve_once(smg, specs, lam, duration, max_jobs, budget_gb, rng_seed, label, 
     dim = specs[0][0].dim
     tbl = max(smg.job_size(c, n)[1] for c, _, n in specs)
     cells = max(smg.job_size(c, n)[2] for c, _, n in specs)
-    slots = (1 << 26) - 256   # 26-bit slot indices of the neighbour entries
+    slots = (1 << 31) - 512   # 32-bit slot indices (neighbour entries are job-relative)
     jobs_cap = max_jobs
     sc = smg.server_config(dim, jobs_cap, slots, min(jobs_cap * tbl, (1 << 31) - 16), jobs_cap * cells,
                            chunk_steps=50, min_steps=100, max_steps=2000, tol=1e-3,
                            max_active_particles=max_active)
-    while smg.server_footprint(sc) > budget_gb * 1e9 and sc.max_slots > (1 << 20):
-        sc.max_slots //= 2
+    # the footprint is affine in the slot count: take as many slots as the budget holds
+    sc.max_slots = 1 << 20
+    f0 = smg.server_footprint(sc)
+    sc.max_slots = 2 << 20
+    per_slot = (smg.server_footprint(sc) - f0) / float(1 << 20)
+    sc.max_slots = int(min(slots, (1 << 20) + (budget_gb * 1e9 - f0) / per_slot)) // 256 * 256
     srv = smg.Server(sc)
     rng = np.random.default_rng(rng_seed)
     arrivals = np.cumsum(rng.exponential(1.0 / lam, size=int(lam * duration * 3) + 10))
@@ -508,6 +512,57 @@ def serve_once(smg, specs, lam, duration, max_jobs, budget_gb, rng_seed, label, 
     return r
 
 
+def plain_once(smg, specs, lam, duration, rng_seed, chunk=50, min_steps=100, max_steps=2000, tol=1e-3):
+    """The plain GPU path under the same arrivals: one job at a time in arrival order,
+    each in an engine of its own (smg_create, smg_set_levelset, smg_seed_particles,
+    smg_step in chunks, smg_destroy), retired by the server's rule (mean residual over
+    the last chunk, from smg_stats.residual_sum, below tol after min_steps, or
+    max_steps).  Returns the completions inside the window."""
+    rng = np.random.default_rng(rng_seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=int(lam * duration * 3) + 10))
+    arrivals = arrivals[arrivals < duration]
+    done, k = [], 0
+    t0 = time.perf_counter()
+    while k < len(arrivals):
+        now = time.perf_counter() - t0
+        if now >= duration:
+            break
+        if arrivals[k] > now:
+            time.sleep(min(arrivals[k], duration) - now)
+            continue
+        c, ls, n = specs[k % len(specs)]
+        cc = smg.SmgConfig.from_buffer_copy(c)
+        cc.seed = c.seed + 1000003 * k
+        job = smg.Job(cc, ls)
+        job.seed(n)
+        steps, prev, status = 0, 0.0, None
+        while time.perf_counter() - t0 < duration:
+            job.step(chunk)
+            steps += chunk
+            rs = job.stats()["residual_sum"]
+            win = (rs - prev) / chunk
+            prev = rs
+            if steps >= min_steps and win < tol:
+                status = 0
+            elif steps >= max_steps:
+                status = 1
+            if status is not None:
+                break
+        job.close()
+        if status is None:
+            break   # the window ended inside this job
+        done.append({"latency": time.perf_counter() - t0 - arrivals[k], "n": n, "steps": steps, "status": status})
+        k += 1
+    lat = [d["latency"] for d in done]
+    r = {"jobs_per_s": len(done) / duration, "completed": len(done), "arrived": int(len(arrivals)),
+         "queued_at_end": int(len(arrivals)) - len(done), "p50_s": pct(lat, 50), "p99_s": pct(lat, 99),
+         "mean_steps": float(np.mean([d["steps"] for d in done])) if done else None,
+         "converged": sum(d["status"] == 0 for d in done),
+         "particle_steps_per_s": sum(d["n"] * d["steps"] for d in done) / duration}
+    log(f"plain lam={lam}: {json.dumps(r)}")
+    return r
+
+
 def serve(args, smg):
     t0 = time.perf_counter()
     specs = smg.batched_jobs(args.specs, 5000, 500000, seed=777, dims=(3,), max_pkg=16)
@@ -517,7 +572,7 @@ def serve(args, smg):
         res[str(lam)] = {
             "served": serve_once(smg, specs, lam, args.duration, args.max_jobs, args.budget_gb, 5 + int(lam), "served",
                                  args.max_active),
-            "plain": serve_once(smg, specs, lam, args.duration, 1, args.budget_gb, 5 + int(lam), "plain"),
+            "plain": plain_once(smg, specs, lam, args.duration, 5 + int(lam)),
         }
     top = res[str(args.lam[-1])]
     out = {

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define SMG_ABI_VERSION 4
+#define SMG_ABI_VERSION 5
 
 enum {
   SMG_OK = 0,
@@ -105,6 +105,10 @@ typedef struct smg_stats {
   int64_t n_rebuild;     /* neighbour-list (and cell-sort) builds so far        */
   double drift;          /* max_i (|x_i - x_b| + cut_off |h_i - h_b|) / h_b of   */
                          /* the last step (x_b, h_b: at the last list build)     */
+  double residual_sum;   /* residual summed over the steps so far: the mapping  */
+                         /* pass of step s adds step s-1's, so after s steps it  */
+                         /* covers steps 1..s-1; differences of it give windowed */
+                         /* means (the server's convergence test)                */
 } smg_stats;
 
 typedef struct smg_job smg_job;
@@ -202,7 +206,8 @@ int smg_set_levelset(smg_job* job, const double* phi, const float* normal, const
  * to the base grid; the cut-cell tag the finest level that stores the cell.  With
  * every package active the result equals a single-level run on the finest grid.  The
  * spacing profile's level-set cell and maximum_phi come from the finest level.  At
- * most 3 levels; call before seeding; not available in a server pool. */
+ * most 3 levels; call before seeding.  A server pool takes such jobs through
+ * smg_server_admit_levels. */
 typedef struct smg_lset_level {
   const double* phi;
   const float* normal;
@@ -429,6 +434,9 @@ typedef struct smg_server_config {
   int64_t max_active_particles; /* admit only while the pool's particles stay within
                                    this (0 = no limit; an empty pool admits any job):
                                    bounds the work per chunk, hence latency          */
+  int64_t max_lset_maps;     /* package-map entries of refined level-set levels (a job
+                                with refined levels takes (num_pkg << l + 2)^dim per
+                                level l); 0 = jobs with refined levels are refused */
 } smg_server_config;
 
 /* finished-job status */
@@ -454,8 +462,16 @@ int smg_server_destroy(smg_server* server);
  * Returns SMG_ERR_CAPACITY, with the pool unchanged, when the job does not fit now. */
 int smg_server_admit(smg_server* server, const smg_config* cfg, const double* phi, const float* normal,
                      const int8_t* tag, int64_t n_particles, int32_t* ticket);
+/* smg_server_admit for a job with refined level-set levels (the arrays and rules of
+ * smg_set_levelset_levels): the levels' stored cells take pool level-set cells and
+ * their package maps take max_lset_maps entries.  nlev = 0 is smg_server_admit. */
+int smg_server_admit_levels(smg_server* server, const smg_config* cfg, const double* phi, const float* normal,
+                            const int8_t* tag, int32_t nlev, const smg_lset_level* levels, int64_t n_particles,
+                            int32_t* ticket);
 /* One chunk of steps for every job in the pool, then convergence checks and
- * retirement.  n_active: jobs still in the pool afterwards (may be NULL). */
+ * retirement.  n_active: jobs still in the pool afterwards (may be NULL).  With an even
+ * chunk_steps the chunk replays a captured hipGraph of the step loop (captured again
+ * only when the pool's extent changes); the results are those of eager steps. */
 int smg_server_advance(smg_server* server, int32_t* n_active);
 /* Jobs finished since the last call (at most cap entries). */
 int smg_server_finished(smg_server* server, smg_server_job* out, int32_t cap, int32_t* n);

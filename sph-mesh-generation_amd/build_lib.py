@@ -1,4 +1,5 @@
 """Build libsmg.so (gfx950 kernels + host engine + C ABI) in-tree with hipcc."""
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -26,17 +27,22 @@ def build(verbose=False):
     """Compile every source and link the shared library; raises on failure."""
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    objs = []
     obj_dir = os.path.join(HERE, "lib", "obj")
     os.makedirs(obj_dir, exist_ok=True)
-    for src in SOURCES:
-        obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
+    objs = [os.path.join(obj_dir, os.path.splitext(src)[0] + ".o") for src in SOURCES]
+
+    def compile_one(k):
         cmd = [hipcc] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-I", CSRC,
-                                       "-c", os.path.join(CSRC, src), "-o", obj]
+                                       "-c", os.path.join(CSRC, SOURCES[k]), "-o", objs[k]]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    # the translation units are independent: compile them side by side
+    workers = max(1, min(len(SOURCES), os.cpu_count() or 1, 8))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for f in [pool.submit(compile_one, k) for k in range(len(SOURCES))]:
+            f.result()
     tmp = LIB + ".tmp"
     cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)

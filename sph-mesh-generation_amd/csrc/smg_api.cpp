@@ -40,9 +40,9 @@ struct smg_job {
   bool profile = false;
 };
 
-namespace {
+namespace smg {
 
-int64_t halo_cells(const smg_config& c, int shift = 0) {
+int64_t halo_cells(const smg_config& c, int shift) {
   int64_t m = 1;
   for (int a = 0; a < c.dim; a++) m *= 4 * (int64_t(c.num_pkg[a]) << shift) + 2;
   return m;
@@ -63,6 +63,13 @@ double interior_max_phi(const smg_config& c, const double* phi, int shift) {
       }
   return mp;
 }
+
+}  // namespace smg
+
+using smg::halo_cells;
+using smg::interior_max_phi;
+
+namespace {
 
 int check_config(const smg_config* c) {
   if (!c) { set_error("config is NULL"); return SMG_ERR_ARG; }

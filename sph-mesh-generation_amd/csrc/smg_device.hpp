@@ -12,8 +12,15 @@ namespace smg {
 
 constexpr int kTile = 256;       // slots per workgroup
 constexpr int kMaxLevels = 8;    // cell-list levels (cell size c0 / 2^L)
-constexpr int kJBits = 26;       // neighbour entry = slot | (mirror code << 26)
+constexpr int kJBits = 26;       // neighbour entry = job-relative slot | (mirror code << 26)
 constexpr uint32_t kJMask = (1u << kJBits) - 1u;
+// A neighbour entry (lists, tile halos, downloads) names a slot of the same job
+// relative to the job's first slot p0, with the mirror code above kJBits: one job holds
+// at most 2^26 slots, a pool of jobs any number.
+constexpr uint32_t nb_pack(int64_t slot, int64_t p0, uint32_t code) {
+  return uint32_t(slot - p0) | (code << kJBits);
+}
+constexpr int64_t nb_slot(uint32_t e, int64_t p0) { return p0 + int64_t(e & kJMask); }
 constexpr int kScanItems = 4096; // table entries per scan workgroup (256 threads x 16)
 // Neighbour tiles of the pair passes.  At every list build, each 256-slot tile gets a
 // halo: the distinct (slot, mirror code) pairs its candidate lists name, the tile's own

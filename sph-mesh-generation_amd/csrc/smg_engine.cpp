@@ -184,6 +184,7 @@ int make_job_params(const JobInput& in, int D, JobParams& J) {
   J.tbl_size = off + 1;   // + pad cell for dead slots
   J.n = in.n;
   J.cap = (J.n + kTile - 1) / kTile * kTile;
+  if (J.cap > (int64_t(1) << kJBits)) { set_error("setup: a job holds at most 2^26 particles"); return SMG_ERR_CAPACITY; }
   return SMG_OK;
 }
 
@@ -216,7 +217,7 @@ int Engine::set_table_owner_(const JobParams& J, int32_t owner) {
 
 int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, int spill_per_slot, int64_t maps) {
   if (tbl >= (int64_t(1) << 31) - 2) { set_error("setup: cell tables exceed 2^31 entries"); return SMG_ERR_CAPACITY; }
-  if (slots >= (int64_t(1) << kJBits)) { set_error("setup: more than 2^26 particle slots"); return SMG_ERR_CAPACITY; }
+  if (slots >= (int64_t(1) << 31) - kTile) { set_error("setup: more than 2^31 particle slots"); return SMG_ERR_CAPACITY; }
   cap_total_ = slots;
   n_tbl_ = tbl;
   n_ls_ = ls;
@@ -511,8 +512,8 @@ int64_t RangeSet::high_water() const {
 }
 
 int Engine::setup_pool(int D, int64_t max_slots, int64_t max_table, int64_t max_lset_cells, int max_jobs,
-                       int max_neighbors, int spill_per_slot, hipStream_t stream) {
-  if (max_jobs <= 0 || max_slots < kTile || max_table <= 0 || max_lset_cells <= 0) {
+                       int max_neighbors, int spill_per_slot, hipStream_t stream, int64_t max_lset_maps) {
+  if (max_jobs <= 0 || max_slots < kTile || max_table <= 0 || max_lset_cells <= 0 || max_lset_maps < 0) {
     set_error("setup_pool: capacities must be positive");
     return SMG_ERR_ARG;
   }
@@ -521,11 +522,12 @@ int Engine::setup_pool(int D, int64_t max_slots, int64_t max_table, int64_t max_
   jp_.assign(max_jobs, JobParams{});
   active_.assign(max_jobs, 0);
   tile_job_h_.assign(slots / kTile, -1);
-  SMG_TRY(alloc_buffers_(slots, max_table, max_lset_cells, max_jobs, spill_per_slot));
+  SMG_TRY(alloc_buffers_(slots, max_table, max_lset_cells, max_jobs, spill_per_slot, max_lset_maps));
   is_pool_ = true;
   tiles_free_.reset(slots / kTile);
   tbl_free_.reset(max_table / kScanItems);   // in 4096-entry scan blocks
   ls_free_.reset(max_lset_cells);
+  maps_free_.reset(max_lset_maps);
   ntiles_ = 0;
   W_.n_tbl = 0;
   prerun_done_ = true;
@@ -546,9 +548,13 @@ int Engine::admit(const JobInput& in, int* job) {
   for (int q = 0; q < int(active_.size()); q++)
     if (!active_[q]) { j = q; break; }
   if (j < 0) { set_error("admit: all job slots in use"); return SMG_ERR_CAPACITY; }
-  if (!in.levels.empty()) { set_error("admit: refined level sets are not supported in a pool"); return SMG_ERR_ARG; }
   JobParams J;
   SMG_TRY(make_job_params(in, D_, J));
+  const int64_t nmap = job_lmap_entries(J);
+  if (nmap > 0 && maps_free_.total() == 0) {
+    set_error("admit: refined level sets need a pool with level-set map entries (max_lset_maps)");
+    return SMG_ERR_ARG;
+  }
   const int64_t nt = J.cap / kTile, nls = job_lset_cells(J);
   const int64_t t0 = tiles_free_.take(nt);
   if (t0 < 0) { set_error("admit: no free slot range"); return SMG_ERR_CAPACITY; }
@@ -562,9 +568,18 @@ int Engine::admit(const JobInput& in, int* job) {
     set_error("admit: no free level-set range");
     return SMG_ERR_CAPACITY;
   }
+  const int64_t mo = nmap > 0 ? maps_free_.take(nmap) : 0;
+  if (mo < 0) {
+    tiles_free_.give(t0, nt);
+    tbl_free_.give(tb, tblocks);
+    ls_free_.give(lo, nls);
+    set_error("admit: no free level-set map range");
+    return SMG_ERR_CAPACITY;
+  }
   J.p0 = t0 * kTile;
   J.tbl_base = tb * kScanItems;
   J.lset_off = lo;
+  J.lmap_base = mo;
   jp_[j] = J;
   active_[j] = 1;
   SMG_TRY(upload_job_(j, in));
@@ -637,6 +652,7 @@ int Engine::retire(int j) {
   tiles_free_.give(t0, nt);
   tbl_free_.give(J.tbl_base / kScanItems, table_blocks(J));
   ls_free_.give(J.lset_off, job_lset_cells(J));
+  if (job_lmap_entries(J) > 0) maps_free_.give(J.lmap_base, job_lmap_entries(J));
   SMG_TRY(set_table_owner_(J, -1));
   active_[j] = 0;
   jp_[j].n = 0;
@@ -893,7 +909,16 @@ int Engine::capture_graph(int64_t steps_per_graph) {
   HIP_TRY(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
   graph_steps_ = steps_per_graph;
   graph_par_ = int(save & 1);
+  graph_tiles_ = ntiles_;
+  graph_tbl_ = W_.n_tbl;
   return SMG_OK;
+}
+
+int Engine::step_graphed(int64_t nsteps, int64_t G) {
+  if (G <= 0 || (G & 1) || nsteps % G != 0 || (step_ & 1) || ntiles_ == 0) return step(nsteps);
+  if (!graph_exec_ || graph_steps_ != G || graph_par_ != 0 || graph_tiles_ != ntiles_ || graph_tbl_ != W_.n_tbl)
+    SMG_TRY(capture_graph(G));
+  return run_graph(nsteps / G);
 }
 
 // Each replay runs graph_steps_ steps; the parity and the A/B buffer roles repeat
@@ -1127,7 +1152,7 @@ int Engine::get_neighbors(int job, int64_t* offs, int32_t* ids, int32_t* codes, 
     tmp.clear();
     for (int64_t k = co[s]; k < co[s + 1]; k++) {
       const uint32_t en = uint32_t(ent[k]);
-      tmp.push_back({id[en & kJMask], int32_t(en >> kJBits)});
+      tmp.push_back({id[nb_slot(en, J.p0)], int32_t(en >> kJBits)});
     }
     std::sort(tmp.begin(), tmp.end());
     for (size_t k = 0; k < tmp.size(); k++) {
@@ -1181,6 +1206,7 @@ int Engine::get_stats(int job, smg_stats* st) {
     for (int64_t s = 0; s < J.n; s++) st->n_neighbor_entries += cnt[s];
     st->n_rebuild = R.nrebuild;
     st->drift = bits_to_double(R.drift);
+    st->residual_sum = double(R.rcum) / kResScale / double(J.n);
   }
   return SMG_OK;
 }

@@ -75,7 +75,7 @@ class Engine {
 
   // pool mode: capacities allocated once; jobs enter and leave between steps
   int setup_pool(int D, int64_t max_slots, int64_t max_table, int64_t max_lset_cells, int max_jobs,
-                 int max_neighbors, int spill_per_slot, hipStream_t stream = nullptr);
+                 int max_neighbors, int spill_per_slot, hipStream_t stream = nullptr, int64_t max_lset_maps = 0);
   int admit(const JobInput& in, int* job);   // seed + Pre_run of the new job only
   int retire(int job);
   bool active(int job) const { return job >= 0 && job < int(active_.size()) && active_[job]; }
@@ -92,6 +92,10 @@ class Engine {
   int init_finish();                                            // Set_particle_info + first sort
   int prerun();                                                 // Pre_run
   int step(int64_t nsteps);                                     // SPH_run_mesh_generation x n
+  // the same steps replayed from a captured graph of G steps (G even, nsteps a multiple
+  // of G, even step index), captured again when G or the launch extent changed; any
+  // other case steps eagerly
+  int step_graphed(int64_t nsteps, int64_t G);
   int sync();
   int query(bool* busy);
   int64_t footprint() const { return bytes_; }
@@ -157,7 +161,7 @@ class Engine {
   std::vector<uint8_t> active_;
   bool is_pool_ = false;
   bool alloc_failed_ = false;
-  RangeSet tiles_free_, tbl_free_, ls_free_;
+  RangeSet tiles_free_, tbl_free_, ls_free_, maps_free_;
   std::vector<int32_t> tile_job_h_;
   int64_t ntiles_ = 0, cap_total_ = 0, n_tbl_ = 0, n_ls_ = 0;
   int K_ = 64;
@@ -195,6 +199,7 @@ class Engine {
   hipGraphExec_t graph_exec_ = nullptr;
   int64_t graph_steps_ = 0;
   int graph_par_ = 0;   // step index parity at capture; replays need the same parity
+  int64_t graph_tiles_ = 0, graph_tbl_ = 0;   // launch extent at capture
 
   template <typename T>
   T* alloc_(int64_t count);

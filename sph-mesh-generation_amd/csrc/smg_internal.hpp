@@ -46,4 +46,10 @@ void tet_cell_quality(const double* x, const int32_t* tets, int64_t n, float* ra
 
 void set_error(const std::string& msg);
 
+// cells of a one-cell-halo level-set grid with num_pkg << shift packages per axis
+int64_t halo_cells(const smg_config& c, int shift = 0);
+// maximum_phi over the interior cells of such a grid, as Parallel_get_global_scale
+// (parallelfunc_mesh.cpp:46-54)
+double interior_max_phi(const smg_config& c, const double* phi, int shift);
+
 }  // namespace smg

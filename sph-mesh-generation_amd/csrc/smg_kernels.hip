@@ -1091,7 +1091,7 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
               const float r = fmaxf(rfi, qr.w * cutf);
               const float rhi = r + (J.pf_tol + 1.e-6f * r);
               if (d2 <= rhi * rhi) {
-                emit(cnt, int32_t(uint32_t(q) | (code << kJBits)));
+                emit(cnt, int32_t(nb_pack(q, J.p0, code)));
                 cnt++;
               }
             }
@@ -1299,7 +1299,7 @@ __global__ __launch_bounds__(64) void k_jump_lists(Work W, PState S) {
     int w = total + pre;
     for (int32_t q = lo; q < hi2; q++) {
       if (!cand(q)) continue;
-      if (w < kJumpList) lst[w] = int32_t(uint32_t(q) | (code << kJBits));
+      if (w < kJumpList) lst[w] = int32_t(nb_pack(q, J.p0, code));
       w++;
     }
     total += sum;
@@ -1420,7 +1420,7 @@ __device__ __forceinline__ void for_each_neighbor(const Work& W, const PState& S
       // past the end of the list a lane loads its own record (always valid) and skips it
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        const int64_t q = 4 * g + u < ni ? int64_t(ev[u] & kJMask) : slot;
+        const int64_t q = 4 * g + u < ni ? nb_slot(ev[u], J.p0) : slot;
         qh[u] = W.rec[4 * q + 0];
         ql[u] = W.rec[4 * q + 1];
       }
@@ -1428,14 +1428,15 @@ __device__ __forceinline__ void for_each_neighbor(const Work& W, const PState& S
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         if (4 * g + u >= ni || (__float_as_int(ql[u].w) & kJumpBit)) continue;
-        visit_rec(4 * g + u, ev[u] & kJMask, ev[u] >> kJBits, qh[u], ql[u]);
+        visit_rec(4 * g + u, nb_slot(ev[u], J.p0), ev[u] >> kJBits, qh[u], ql[u]);
       }
     }
     for (int k = ni; k < cnt; k++) {   // the spilled tail
       const uint32_t e = uint32_t(W.spill[int64_t(W.nbo[slot]) + (k - W.K)]);
-      const float4 qh = W.rec[4 * int64_t(e & kJMask) + 0], ql = W.rec[4 * int64_t(e & kJMask) + 1];
+      const int64_t q = nb_slot(e, J.p0);
+      const float4 qh = W.rec[4 * q + 0], ql = W.rec[4 * q + 1];
       if (__float_as_int(ql.w) & kJumpBit) continue;
-      visit_rec(k, e & kJMask, e >> kJBits, qh, ql);
+      visit_rec(k, q, e >> kJBits, qh, ql);
     }
   } else {
     // a jumper walks the list k_jump_lists built for it this step (nbo holds its index)
@@ -1444,7 +1445,7 @@ __device__ __forceinline__ void for_each_neighbor(const Work& W, const PState& S
     const int32_t* lst = W.jl + t * kJumpList;
     for (int k = 0; k < cnt; k++) {
       const uint32_t e = uint32_t(lst[k]);
-      visit(-1, e & kJMask, e >> kJBits);
+      visit(-1, nb_slot(e, J.p0), e >> kJBits);
     }
   }
   if (jt.n == 0) return;
@@ -1470,7 +1471,7 @@ __device__ __forceinline__ void for_each_neighbor(const Work& W, const PState& S
   }
 }
 
-// neighbour sets for download: counts (pass 0) or (slot | code << 26) entries at offs
+// neighbour sets for download: counts (pass 0) or nb_pack entries at offs
 template <int D>
 __global__ __launch_bounds__(kTile) void k_nb_out(Work W, PState S, int pass, const int64_t* offs, int32_t* cnt,
                                                   int32_t* out) {
@@ -1488,7 +1489,7 @@ __global__ __launch_bounds__(kTile) void k_nb_out(Work W, PState S, int pass, co
   const int64_t o = pass ? offs[slot - J.p0] : 0;
   for_each_neighbor<D>(W, S, J, slot, jt, [&](int, int64_t q, uint32_t code, const float4&, const float4&,
                                                const float*, float) {
-    if (pass) out[o + c] = int32_t(uint32_t(q) | (code << kJBits));
+    if (pass) out[o + c] = int32_t(nb_pack(q, J.p0, code));
     c++;
   });
   if (!pass) cnt[slot - J.p0] = c;
@@ -1613,7 +1614,7 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W) {
   uint32_t* hal = W.halo + tile * W.ht;
   const int cap = int(W.ht < int64_t(kMaxHalo) ? W.ht : int64_t(kMaxHalo)) - kTile;
   auto own = [&](uint32_t e) {
-    const int64_t q = e & kJMask;
+    const int64_t q = nb_slot(e, J.p0);
     return (e >> kJBits) == 0u && q >= ts && q < ts + kTile;
   };
   int P = 1, base = 0;
@@ -1676,7 +1677,7 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W) {
     __syncthreads();
     for_entries(W, slot, cnt, off, [&](int k, uint32_t e) {
       if (own(e)) {
-        if (pass == 0) item_at(k) = uint16_t((e & kJMask) - uint32_t(ts));
+        if (pass == 0) item_at(k) = uint16_t(nb_slot(e, J.p0) - ts);
         return;
       }
       if (halo_part(e, P) != uint32_t(pass)) return;
@@ -1692,15 +1693,15 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W) {
 }
 
 // slot and mirror code of halo entry e of a tile
-__device__ __forceinline__ void halo_entry(const Work& W, int64_t tile, int64_t ts, uint32_t e, int64_t& q,
-                                           uint32_t& code) {
+__device__ __forceinline__ void halo_entry(const Work& W, int64_t tile, int64_t ts, int64_t p0, uint32_t e,
+                                           int64_t& q, uint32_t& code) {
   if (e < uint32_t(kTile)) {
     q = ts + e;
     code = 0u;
     return;
   }
   const uint32_t v = W.halo[tile * W.ht + (e - kTile)];
-  q = v & kJMask;
+  q = nb_slot(v, p0);
   code = v >> kJBits;
 }
 
@@ -1767,7 +1768,7 @@ __device__ __forceinline__ void stage_halo(const Work& W, const JobParams& J, in
     const int e = threadIdx.x + u * kTile;
     q[u] = ts;   // a valid slot for the entries past HS (their records are never stored)
     code[u] = 0u;
-    if (e < HS) halo_entry(W, tile, ts, uint32_t(e), q[u], code[u]);
+    if (e < HS) halo_entry(W, tile, ts, J.p0, uint32_t(e), q[u], code[u]);
   }
   float4 rh[NU], rl[NU];
 #pragma unroll
@@ -2015,13 +2016,13 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void
           if (valid && !nb) {
             const float rhi = r * (1.f + 1.e-6f) + o.bq;
             if (d2 <= rhi * rhi) {
-              if (q < 0) halo_entry(W, tile, ts, e, q, code);
+              if (q < 0) halo_entry(W, tile, ts, J.p0, e, q, code);
               nb = exact_pred<D>(W, J, S, slot, q, code);
             }
           }
           nb = valid && nb && !(tw & int(kQJump));   // a jumper: paired through the jumper scan
           const int tq = tw & int(kQType);
-          if (TS && nb && o.surf && tq == SMG_SURFACE && q < 0) halo_entry(W, tile, ts, e, q, code);
+          if (TS && nb && o.surf && tq == SMG_SURFACE && q < 0) halo_entry(W, tile, ts, J.p0, e, q, code);
           const bool f = pair1<D, TS>(W, J, o, dr, d2, qh.w, tq, e == uint32_t(p), q, acc, nb);
           if (f && !(tw & int(kQMir))) {
             st2[(c2 >> 3) * 512 + (c2 & 7)] = uint16_t(e);
@@ -2055,7 +2056,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void
                   qh = hh[e];
                   qc = hq[e];
                 } else {   // past the staged records (a tile whose halo exceeds the LDS)
-                  halo_entry(W, tile, ts, e, q, code);
+                  halo_entry(W, tile, ts, J.p0, e, q, code);
                   float4 rl;
                   halo_rec<D>(W, J, q, code, qh, rl);
                   qc = pack_lo(rl);
@@ -2260,7 +2261,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void
               } else {
                 int64_t q;
                 uint32_t code;
-                halo_entry(W, tile, ts, e, q, code);
+                halo_entry(W, tile, ts, J.p0, e, q, code);
                 float4 rl;
                 halo_rec<D>(W, J, q, code, qh, rl);
                 qc = pack_lo(rl);
@@ -2400,7 +2401,7 @@ __global__ __launch_bounds__(kTile) void k_vor2d(Work W, PState S, int32_t* out_
     double nx[kVorVerts], ny[kVorVerts];
     int32_t nl[kVorVerts];
     int m = 0;
-    const int32_t label = int32_t(uint32_t(q) | (code << kJBits));
+    const int32_t label = int32_t(nb_pack(q, J.p0, code));
     for (int v = 0; v < nv; v++) {
       const int w = v + 1 < nv ? v + 1 : 0;
       const bool vin = f[v] <= 0., win = f[w] <= 0.;
@@ -2430,7 +2431,7 @@ __global__ __launch_bounds__(kTile) void k_vor2d(Work W, PState S, int32_t* out_
   if (!bad) {
     for (int v = 0; v < nv; v++) {
       if (lab[v] < 0) continue;
-      const int32_t src = S.id[uint32_t(lab[v]) & kJMask];
+      const int32_t src = S.id[nb_slot(uint32_t(lab[v]), J.p0)];
       if (src == S.id[slot]) continue;   // the cell face with the particle's own mirror image
       bool seen = false;
       for (int u = 0; u < nf; u++) seen = seen || out_nb[loc * KE + u] == src;

@@ -70,7 +70,7 @@ int smg_server_footprint_of(const smg_server_config* c, int64_t* bytes) {
   if (!c || !bytes) { set_error("smg_server_footprint_of: NULL argument"); return SMG_ERR_ARG; }
   const Sizes z = sizes_for(c->max_neighbors > 0 ? (c->max_neighbors + 3) / 4 * 4 : 128, c->spill_per_slot > 0 ? c->spill_per_slot : 64);
   *bytes = c->max_slots * z.per_slot + c->max_table * z.per_table + c->max_lset_cells * z.per_cell +
-           int64_t(c->max_jobs) * z.per_job;
+           int64_t(c->max_jobs) * z.per_job + std::max<int64_t>(c->max_lset_maps, 1) * int64_t(sizeof(int32_t));
   return SMG_OK;
 }
 
@@ -98,14 +98,14 @@ int smg_server_job_size(const smg_config* cfg, int64_t n, int64_t* slots, int64_
 
 int smg_server_create(const smg_server_config* c, smg_server** server) {
   if (!c || !server) { set_error("smg_server_create: NULL argument"); return SMG_ERR_ARG; }
-  if (c->chunk_steps <= 0 || !(c->tol >= 0.) || c->min_steps < 0 || c->max_steps < 0) {
-    set_error("smg_server_create: chunk_steps > 0, tol >= 0, min_steps >= 0, max_steps >= 0");
+  if (c->chunk_steps <= 0 || !(c->tol >= 0.) || c->min_steps < 0 || c->max_steps < 0 || c->max_lset_maps < 0) {
+    set_error("smg_server_create: chunk_steps > 0, tol >= 0, min_steps >= 0, max_steps >= 0, max_lset_maps >= 0");
     return SMG_ERR_ARG;
   }
   std::unique_ptr<smg_server> s(new smg_server());
   s->cfg = *c;
   const int rc = s->eng.setup_pool(c->dim, c->max_slots, c->max_table, c->max_lset_cells, c->max_jobs,
-                                   c->max_neighbors, c->spill_per_slot, nullptr);
+                                   c->max_neighbors, c->spill_per_slot, nullptr, c->max_lset_maps);
   if (rc != SMG_OK) return rc;
   *server = s.release();
   return SMG_OK;
@@ -118,7 +118,14 @@ int smg_server_destroy(smg_server* server) {
 
 int smg_server_admit(smg_server* s, const smg_config* cfg, const double* phi, const float* normal,
                      const int8_t* tag, int64_t n, int32_t* ticket) {
-  if (!s || !cfg || !phi || !normal || !tag || !ticket || n <= 0) {
+  return smg_server_admit_levels(s, cfg, phi, normal, tag, 0, nullptr, n, ticket);
+}
+
+int smg_server_admit_levels(smg_server* s, const smg_config* cfg, const double* phi, const float* normal,
+                            const int8_t* tag, int32_t nlev, const smg_lset_level* levels, int64_t n,
+                            int32_t* ticket) {
+  if (!s || !cfg || !phi || !normal || !tag || !ticket || n <= 0 || nlev < 0 || nlev > smg::kMaxLsLevels ||
+      (nlev > 0 && !levels)) {
     set_error("smg_server_admit: bad argument");
     return SMG_ERR_ARG;
   }
@@ -131,12 +138,22 @@ int smg_server_admit(smg_server* s, const smg_config* cfg, const double* phi, co
   smg::JobInput in;
   in.cfg = *cfg;
   in.n = n;
-  // maximum_phi of the level set (Levelset::Get_max_phi) for the tanh profile
-  int64_t m = 1;
-  for (int a = 0; a < cfg->dim; a++) m *= 4 * int64_t(cfg->num_pkg[a]) + 2;
-  double mp = 0.;
-  for (int64_t q = 0; q < m; q++) mp = std::max(mp, phi[q]);
-  in.max_phi = mp;
+  for (int l = 0; l < nlev; l++) {
+    const smg_lset_level& L = levels[l];
+    if (!L.phi || !L.normal || !L.tag || !L.active) {
+      set_error("smg_server_admit: NULL array in refined level " + std::to_string(l + 1));
+      return SMG_ERR_ARG;
+    }
+    smg::LevelInput li;
+    li.phi = L.phi;
+    li.normal = L.normal;
+    li.tag = L.tag;
+    li.active = L.active;
+    in.levels.push_back(li);
+  }
+  // maximum_phi of the finest level (Levelset::Get_max_phi) for the tanh profile, as
+  // smg_set_levelset / smg_set_levelset_levels take it
+  in.max_phi = nlev > 0 ? smg::interior_max_phi(*cfg, levels[nlev - 1].phi, nlev) : smg::interior_max_phi(*cfg, phi, 0);
   in.phi = phi;
   in.normal = normal;
   in.tag = tag;
@@ -155,7 +172,12 @@ int smg_server_advance(smg_server* s, int32_t* n_active) {
   if (!s) { set_error("smg_server_advance: NULL server"); return SMG_ERR_ARG; }
   const int64_t chunk = s->cfg.chunk_steps;
   if (!s->by_engine.empty()) {
-    int rc = s->eng.step(chunk);
+    // replayed from a captured graph of G steps: the largest even divisor of the chunk
+    // up to 10 (an odd chunk steps eagerly)
+    int64_t G = 0;
+    for (int64_t g = 10; g >= 2; g -= 2)
+      if (chunk % g == 0) { G = g; break; }
+    int rc = G > 0 ? s->eng.step_graphed(chunk, G) : s->eng.step(chunk);
     if (rc != SMG_OK) return rc;
     rc = s->eng.download_reds();
     if (rc != SMG_OK) return rc;

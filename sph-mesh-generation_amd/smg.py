@@ -57,7 +57,7 @@ class SmgStats(C.Structure):
         ("max_neighbors_seen", C.c_int32), ("num_level", C.c_int32),
         ("dt1", C.c_double), ("dt2", C.c_double), ("max_v", C.c_double),
         ("residual", C.c_double), ("max_disp", C.c_double), ("res_max", C.c_double),
-        ("n_rebuild", C.c_int64), ("drift", C.c_double),
+        ("n_rebuild", C.c_int64), ("drift", C.c_double), ("residual_sum", C.c_double),
     ]
 
     def as_dict(self):
@@ -790,7 +790,7 @@ class SmgServerConfig(C.Structure):
         ("dim", C.c_int32), ("max_jobs", C.c_int32), ("max_slots", C.c_int64), ("max_table", C.c_int64),
         ("max_lset_cells", C.c_int64), ("max_neighbors", C.c_int32), ("spill_per_slot", C.c_int32),
         ("chunk_steps", C.c_int32), ("min_steps", C.c_int32), ("max_steps", C.c_int32), ("reserved", C.c_int32),
-        ("tol", C.c_double), ("max_active_particles", C.c_int64),
+        ("tol", C.c_double), ("max_active_particles", C.c_int64), ("max_lset_maps", C.c_int64),
     ]
 
 
@@ -810,6 +810,8 @@ def _server_sigs(L):
     L.smg_server_destroy.argtypes = [C.c_void_p]
     L.smg_server_admit.argtypes = [C.c_void_p, P(SmgConfig), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                    P(C.c_int32)]
+    L.smg_server_admit_levels.argtypes = [C.c_void_p, P(SmgConfig), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                          P(SmgLsetLevel), C.c_int64, P(C.c_int32)]
     L.smg_server_advance.argtypes = [C.c_void_p, P(C.c_int32)]
     L.smg_server_finished.argtypes = [C.c_void_p, P(SmgServerJob), C.c_int32, P(C.c_int32)]
     L.smg_server_result.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, P(SmgStats)]
@@ -817,9 +819,11 @@ def _server_sigs(L):
 
 
 def server_config(dim, max_jobs, max_slots, max_table, max_lset_cells, chunk_steps=50, min_steps=100,
-                  max_steps=3000, tol=1e-3, max_neighbors=0, spill_per_slot=0, max_active_particles=0):
+                  max_steps=3000, tol=1e-3, max_neighbors=0, spill_per_slot=0, max_active_particles=0,
+                  max_lset_maps=0):
     return SmgServerConfig(dim, max_jobs, int(max_slots), int(max_table), int(max_lset_cells), max_neighbors,
-                           spill_per_slot, chunk_steps, min_steps, max_steps, 0, tol, int(max_active_particles))
+                           spill_per_slot, chunk_steps, min_steps, max_steps, 0, tol, int(max_active_particles),
+                           int(max_lset_maps))
 
 
 def server_footprint(scfg):
@@ -860,11 +864,19 @@ class Server:
         except Exception:
             pass
 
-    def admit(self, cfg, lset, n):
-        """Ticket of the admitted job, or None when it does not fit the pool now."""
+    def admit(self, cfg, lset, n, levels=None):
+        """Ticket of the admitted job, or None when it does not fit the pool now.
+        levels: refined level-set levels as for Job (make_levelset_levels)."""
         t = C.c_int32()
-        rc = lib().smg_server_admit(self.h, C.byref(cfg), _ptr(lset["phi"]), _ptr(lset["normal"]),
-                                    _ptr(lset["tag"]), int(n), C.byref(t))
+        if levels:
+            arr = (SmgLsetLevel * len(levels))()
+            for i, L in enumerate(levels):
+                arr[i] = SmgLsetLevel(_ptr(L["phi"]), _ptr(L["normal"]), _ptr(L["tag"]), _ptr(L["active"]))
+            rc = lib().smg_server_admit_levels(self.h, C.byref(cfg), _ptr(lset["phi"]), _ptr(lset["normal"]),
+                                               _ptr(lset["tag"]), len(levels), arr, int(n), C.byref(t))
+        else:
+            rc = lib().smg_server_admit(self.h, C.byref(cfg), _ptr(lset["phi"]), _ptr(lset["normal"]),
+                                        _ptr(lset["tag"]), int(n), C.byref(t))
         if rc == SMG_ERR_CAPACITY:
             return None
         _check(rc, "smg_server_admit")

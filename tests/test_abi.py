@@ -29,7 +29,7 @@ def test_host_entry_points(smg_mod):
     assert abs(cfg.angle_thres - 0.4 * 3.1415926) < 1e-15
     cfg2 = smg_mod.default_config(2)
     assert cfg2.cell_ratio == 2.0 and cfg2.sym[2] == 0
-    assert smg_mod.lib().smg_abi_version() == 4
+    assert smg_mod.lib().smg_abi_version() == 5
 
 
 def test_bad_arguments_are_rejected(smg_mod):

@@ -128,3 +128,92 @@ def test_admission_by_particle_limit(smg_mod, jobs3):
     assert [f["ticket"] for f in srv.finished()] == [t0]
     assert srv.admit(*jobs3[1]) is not None
     srv.close()
+
+
+def _refined_case(smg_mod):
+    # 3D multi-resolution job: base 16^3, refined twice in a band (partial bands)
+    cfg, sdf = smg_mod.config_c3(num_pkg=4, dx0=0.012, ratio=2.0, band=0.2)
+    base = smg_mod.make_levelset(cfg, sdf)
+    lv = smg_mod.make_levelset_levels(cfg, sdf, 2, band=2.0)
+    n = int(lv[-1]["stats"].num_particle)
+    maps = sum(int(np.prod([L["cfg"].num_pkg[a] + 2 for a in range(3)])) for L in lv)
+    return cfg, base, lv, n, maps
+
+
+def test_refined_level_set_job_in_the_pool(smg_mod, jobs3):
+    """A job with refined level-set levels, admitted next to single-level jobs (and
+    after another refined job retired, so its map range is reused), equals
+    smg_set_levelset_levels + smg_seed_particles + smg_step bit for bit; a pool without
+    map entries refuses it."""
+    cfg, base, lv, n, maps = _refined_case(smg_mod)
+    print(f"refined job: n = {n}, map entries = {maps}", flush=True)
+    no_maps = _server(smg_mod, 3)
+    with pytest.raises(smg_mod.SmgError):
+        no_maps.admit(cfg, base, n, levels=lv)
+    no_maps.close()
+
+    srv = _server(smg_mod, 3, slots=1 << 18, cells=1 << 22, tol=0.0, max_steps=4 * CHUNK, max_lset_maps=2 * maps)
+    first = srv.admit(cfg, base, n, levels=lv)
+    other = srv.admit(*jobs3[0])
+    assert first is not None and other is not None
+    done = _run_until_done(srv, [first, other])
+    assert done[first]["steps"] == 4 * CHUNK
+    srv.result(first)
+    srv.result(other)
+    # a second refined job with another seed, mixed with two single-level jobs
+    cfg2 = smg_mod.SmgConfig.from_buffer_copy(cfg)
+    cfg2.seed = cfg.seed + 17
+    tickets = [srv.admit(*jobs3[1]), srv.admit(cfg2, base, n, levels=lv), srv.admit(*jobs3[2])]
+    assert None not in tickets
+    done = _run_until_done(srv, tickets)
+    r = srv.result(tickets[1])
+    assert done[tickets[1]]["status"] == smg_mod.JOB_MAX_STEPS
+    job = smg_mod.Job(cfg2, base, levels=lv)
+    job.seed(n)
+    job.step(4 * CHUNK)
+    g = job.state()
+    for k in ("x", "type", "h"):
+        assert np.array_equal(r[k], g[k]), k
+    st = job.stats()
+    for k in ("dt1", "dt2", "residual", "residual_sum"):
+        assert r["stats"][k] == st[k], k
+    job.close()
+    srv.close()
+
+
+def test_window_residual_from_residual_sum(smg_mod, jobs3):
+    """The server's chunk window residual is the difference of smg_stats.residual_sum
+    over the chunk divided by its steps: the plain path can apply the same test."""
+    cfg, ls, n = jobs3[4]
+    srv = _server(smg_mod, 3, tol=0.0, max_steps=2 * CHUNK)
+    t = srv.admit(cfg, ls, n)
+    info = _run_until_done(srv, [t])[t]
+    srv.result(t)
+    srv.close()
+    job = smg_mod.Job(cfg, ls)
+    job.seed(n)
+    job.step(CHUNK)
+    s1 = job.stats()["residual_sum"]
+    job.step(CHUNK)
+    s2 = job.stats()["residual_sum"]
+    job.close()
+    assert info["window_residual"] == pytest.approx((s2 - s1) / CHUNK, rel=1e-12, abs=0)
+
+
+def test_odd_chunk_steps_eagerly(smg_mod, jobs3):
+    """An odd chunk cannot replay the two-step graph body; it steps eagerly and still
+    equals the single-job path."""
+    cfg, ls, n = jobs3[0]
+    srv = _server(smg_mod, 3, chunk_steps=7, min_steps=7, tol=0.0, max_steps=21)
+    t = srv.admit(cfg, ls, n)
+    info = _run_until_done(srv, [t])[t]
+    assert info["steps"] == 21
+    r = srv.result(t)
+    srv.close()
+    job = smg_mod.Job(cfg, ls)
+    job.seed(n)
+    job.step(21)
+    g = job.state()
+    job.close()
+    for k in ("x", "type", "h"):
+        assert np.array_equal(r[k], g[k]), k
