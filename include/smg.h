@@ -370,10 +370,10 @@ int smg_tri_cliques(int64_t n_vertices, const int32_t* edges, int64_t n_edges, i
  * ("JrO/1o/100Q", graph_mesh.cpp:1401) is TetGen-internal and not reproduced. */
 typedef struct smg_tet_stats {
   int64_t n_tet;                   /* tetrahedra after the deletion rule                  */
-  int64_t n_delaunay;              /* certified Delaunay tetrahedra of the GPU stage       */
+  int64_t n_delaunay;              /* Delaunay tetrahedra of the GPU stage                 */
   int64_t n_deleted;               /* four non-REAL particles and aspect > delete_thres    */
   int64_t n_surface_tri;           /* boundary triangles                                   */
-  int64_t n_open;                  /* particles whose cell was not closed by the search    */
+  int64_t n_open;                  /* particles whose cell neither pass closed             */
   int64_t n_cell_overflow;         /* particles whose cell could not be built             */
   int64_t n_duplicate;             /* particles on the point of a smaller id (in no tet)   */
   double min_edge, max_edge;
@@ -385,8 +385,10 @@ typedef struct smg_tet_stats {
   int64_t dihedral_hist[45];       /* dihedral angles in 4-degree bins                     */
   int64_t aspect_hist[12];         /* bins bounded by 1.5 2 2.5 3 4 6 10 15 25 50 100      */
   int64_t radius_hist[56];         /* radius ratio in 0.02 bins, the last one open         */
-  double cell_ms;                  /* GPU time of the Voronoi-cell pass                    */
+  double cell_ms;                  /* GPU time of the Voronoi-cell passes                  */
   double total_ms;                 /* the whole call, downloads and host stage included    */
+  int64_t n_second_pass;           /* cells the ring search left open (hull particles,     */
+                                   /* cavity spanners), built by the verifying pass        */
 } smg_tet_stats;
 
 /* Tetrahedralises a 3D job's current particles.  tets: 4 particle ids per tetrahedron
@@ -396,11 +398,12 @@ int smg_tetrahedralize(smg_job* job, double delete_thres, int32_t* tets, int64_t
                        smg_tet_stats* stats);
 /* Boundary triangles (3 ascending particle ids, sorted) of the last smg_tetrahedralize. */
 int smg_tet_surface(smg_job* job, int32_t* tris, int64_t capacity, int64_t* n_tri);
-/* The GPU stage of the last smg_tetrahedralize: the certified Delaunay tetrahedra
- * (before deletion, ascending ids, sorted) and, by particle id, the cell status (0
- * closed, 1 open at the search limit; 2 capacity, 3-5 degenerate cut: not built; 6 on
- * the point of a smaller id) and the certified radius.  status and
- * rcert may be NULL; they hold n_particles entries. */
+/* The GPU stage of the last smg_tetrahedralize: the Delaunay tetrahedra of all
+ * particles (before deletion, ascending ids, sorted) and, by particle id, the cell
+ * status (0 closed; 2 capacity, 3-5 degenerate cut: not built; 6 on the point of a
+ * smaller id) and the certified radius (every point within it was clipped; 1e300 for
+ * a cell closed by the verifying pass).  status and rcert may be NULL; they hold
+ * n_particles entries. */
 int smg_tet_delaunay(smg_job* job, int32_t* tets, int64_t capacity, int64_t* n_tet, int8_t* status, double* rcert);
 /* The host stage alone (no GPU): deletion rule, boundary triangles and statistics of
  * given tetrahedra over positions x (3 per vertex) and particle types. */

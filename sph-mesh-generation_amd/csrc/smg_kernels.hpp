@@ -52,9 +52,24 @@ struct V3Out {
   int64_t cap;
   int8_t* status;               // by id: 0 cell closed, 1 open at rmax; not built: 2 capacity,
                                 //   3 empty cut boundary, 4 non-convex cut, 5 singular vertex
-  double* rcert;                // by id: every point within this radius was clipped
+  double* rcert;                // by id: every point within this radius was clipped (1e300:
+                                //   closed by the second pass, every vertex verified)
+};
+// coarse blocks of the grid for the second pass: the points of each block and the
+// bounding box of those points
+struct V3Blocks {
+  const int32_t* start;   // block starts, g0 g1 g2 + 1 entries (x fastest)
+  const int32_t* order;   // particle ids in block order
+  const double* box;      // 6 per block: min xyz, max xyz of its points
+  int32_t g[3];
+  double lo[3];           // origin (the grid's)
+  double s;               // block size
 };
 void launch_vor3d(hipStream_t st, const V3Grid& G, const V3Out& O);
+// the cells of ids (those pass 1 left open), started from a box of half-size B and
+// verified vertex by vertex; status 0 when closed
+void launch_vor3d_full(hipStream_t st, const V3Grid& G, const V3Blocks& K, const int32_t* ids, int64_t nids, double B,
+                       const V3Out& O);
 void launch_fp64_probe(hipStream_t st, int64_t n, const double* a, const double* b, double* out);
 
 // the no-allocation sampling entry point's staging array (doubles): points then phi,

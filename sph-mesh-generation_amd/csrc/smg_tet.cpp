@@ -339,6 +339,66 @@ int Engine::tetrahedralize(int job, double delete_thres, smg_tet_stats* st) {
   G.x = d_x.p;
   G.start = d_start.p;
   G.order = d_order.p;
+  // pass-2 inputs: blocks of 4^3 grid cells with the bounding box of their points
+  V3Blocks K;
+  K.s = 4. * G.s;
+  int64_t nblk = 1;
+  for (int a = 0; a < 3; a++) {
+    K.lo[a] = G.lo[a];
+    K.g[a] = std::max(1, int(std::ceil((J.box_r[a] - J.box_l[a]) / K.s)));
+    nblk *= K.g[a];
+  }
+  std::vector<int32_t> bstart(nblk + 1, 0), border(static_cast<size_t>(n_cells));
+  std::vector<double> bbox(6 * size_t(nblk));
+  {
+    std::vector<int32_t> blk_of(static_cast<size_t>(n_cells));
+    for (int64_t q = 0; q < n_cells; q++) {
+      const int32_t i = order[q];
+      int64_t b = 0;
+      for (int a = 2; a >= 0; a--) {
+        int k = int(std::floor((x[3 * size_t(i) + a] - K.lo[a]) / K.s));
+        k = k < 0 ? 0 : (k >= K.g[a] ? K.g[a] - 1 : k);
+        b = b * K.g[a] + k;
+      }
+      blk_of[q] = int32_t(b);
+      bstart[b + 1]++;
+    }
+    for (int64_t b = 0; b < nblk; b++) bstart[b + 1] += bstart[b];
+    std::vector<int32_t> fill(bstart.begin(), bstart.end() - 1);
+    for (int64_t b = 0; b < nblk; b++)
+      for (int a = 0; a < 3; a++) {
+        bbox[6 * b + a] = std::numeric_limits<double>::infinity();
+        bbox[6 * b + 3 + a] = -std::numeric_limits<double>::infinity();
+      }
+    for (int64_t q = 0; q < n_cells; q++) {
+      const int32_t i = order[q];
+      const int64_t b = blk_of[q];
+      border[fill[b]++] = i;
+      for (int a = 0; a < 3; a++) {
+        bbox[6 * b + a] = std::min(bbox[6 * b + a], x[3 * size_t(i) + a]);
+        bbox[6 * b + 3 + a] = std::max(bbox[6 * b + 3 + a], x[3 * size_t(i) + a]);
+      }
+    }
+  }
+  // the second pass's starting box: far beyond the point set, so that a circumcentre
+  // it cuts off would be 10^6 box lengths away
+  double ext = 0.;
+  for (int a = 0; a < 3; a++) ext = std::max(ext, J.box_r[a] - J.box_l[a]);
+  const double Bfar = 1.e6 * ext;
+  DevBuf<int32_t> d_bstart, d_border, d_open;
+  DevBuf<double> d_bbox;
+  HIP_TRY_TET(d_bstart.alloc(size_t(nblk) + 1));
+  HIP_TRY_TET(d_border.alloc(size_t(n_cells)));
+  HIP_TRY_TET(d_bbox.alloc(6 * size_t(nblk)));
+  HIP_TRY_TET(d_open.alloc(size_t(n_cells)));
+  HIP_TRY_TET(hipMemcpy(d_bstart.p, bstart.data(), sizeof(int32_t) * (nblk + 1), hipMemcpyHostToDevice));
+  HIP_TRY_TET(hipMemcpy(d_border.p, border.data(), sizeof(int32_t) * n_cells, hipMemcpyHostToDevice));
+  HIP_TRY_TET(hipMemcpy(d_bbox.p, bbox.data(), sizeof(double) * 6 * nblk, hipMemcpyHostToDevice));
+  K.start = d_bstart.p;
+  K.order = d_border.p;
+  K.box = d_bbox.p;
+  std::vector<int8_t> st1(n);
+  std::vector<int32_t> open_ids;
   int64_t cap = 12 * n;
   unsigned long long count = 0ull;
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -364,6 +424,18 @@ int Engine::tetrahedralize(int job, double delete_thres, smg_tet_stats* st) {
     HIP_TRY_TET(hipEventRecord(ev[0], st_));
     launch_vor3d(st_, G, O);
     HIP_TRY_TET(hipGetLastError());
+    // the cells pass 1 left open (hull particles, cavity spanners) go to pass 2
+    HIP_TRY_TET(hipMemcpyAsync(st1.data(), d_status.p, sizeof(int8_t) * n, hipMemcpyDeviceToHost, st_));
+    HIP_TRY_TET(hipStreamSynchronize(st_));
+    open_ids.clear();
+    for (int64_t q = 0; q < n_cells; q++)
+      if (st1[order[q]] != 0) open_ids.push_back(order[q]);
+    if (!open_ids.empty()) {
+      HIP_TRY_TET(hipMemcpyAsync(d_open.p, open_ids.data(), sizeof(int32_t) * open_ids.size(), hipMemcpyHostToDevice,
+                                 st_));
+      launch_vor3d_full(st_, G, K, d_open.p, int64_t(open_ids.size()), Bfar, O);
+      HIP_TRY_TET(hipGetLastError());
+    }
     HIP_TRY_TET(hipEventRecord(ev[1], st_));
     HIP_TRY_TET(hipMemcpyAsync(&count, d_count.p, sizeof(count), hipMemcpyDeviceToHost, st_));
     HIP_TRY_TET(hipStreamSynchronize(st_));
@@ -400,6 +472,7 @@ int Engine::tetrahedralize(int job, double delete_thres, smg_tet_stats* st) {
     st->n_cell_overflow += tet_.status[i] >= 2 && tet_.status[i] <= 5;
     st->n_duplicate += tet_.status[i] == 6;
   }
+  st->n_second_pass = int64_t(open_ids.size());
   st->cell_ms = double(cell_ms);
   st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   return SMG_OK;

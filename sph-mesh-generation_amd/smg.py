@@ -693,7 +693,7 @@ class SmgTetStats(C.Structure):
         ("min_dihedral", C.c_double), ("max_dihedral", C.c_double), ("avg_min_dihedral", C.c_double),
         ("n_min_dihedral_below", C.c_int64 * 4), ("dihedral_hist", C.c_int64 * 45),
         ("aspect_hist", C.c_int64 * 12), ("radius_hist", C.c_int64 * 56),
-        ("cell_ms", C.c_double), ("total_ms", C.c_double),
+        ("cell_ms", C.c_double), ("total_ms", C.c_double), ("n_second_pass", C.c_int64),
     ]
 
     def as_dict(self):
