@@ -260,16 +260,39 @@ def batch(args, smg):
     n_tot = sum(n for _, _, n in jobs)
     log(f"{len(jobs)} jobs, {n_tot} particles, level sets in {time.perf_counter() - t0:.1f} s")
     S, G = args.job_steps, args.graph_steps
-    # batched: all jobs of a dimension in one slot space, graph-replayed step loop
+    # tile shapes from the autotuner (smg_autotune): the list skin, i.e. the cell-size
+    # multiple, and the inline list depth, timed on the largest job of each dimension
+    # from its seeding on (the phase a 200-step job spends its time in)
     t0 = time.perf_counter()
-    b = smg.Batch(jobs)
-    b.seed()
-    b.capture(G)
-    b.run(S // G)
-    b.sync()
-    tb = time.perf_counter() - t0
-    b.close()
-    log(f"batched: {tb:.2f} s")
+    tuned = {}
+    for dim in (2, 3):
+        c, ls, n = max([j for j in jobs if j[0].dim == dim], key=lambda j: j[2])
+        tuned[dim] = smg.autotune(c, ls, n, warmup=0, steps=40)
+        log(f"autotune {dim}D on n={n}: skin {tuned[dim]['skin']}, max_neighbors {tuned[dim]['max_neighbors']}")
+    t_tune = time.perf_counter() - t0
+    tjobs = []
+    for c, ls, n in jobs:
+        cc = smg.SmgConfig.from_buffer_copy(c)
+        cc.skin = tuned[c.dim]["skin"]
+        cc.max_neighbors = tuned[c.dim]["max_neighbors"]
+        tjobs.append((cc, ls, n))
+
+    def batched(js):
+        # all jobs of a dimension in one slot space, graph-replayed step loop
+        t0 = time.perf_counter()
+        b = smg.Batch(js)
+        b.seed()
+        b.capture(G)
+        b.run(S // G)
+        b.sync()
+        t = time.perf_counter() - t0
+        b.close()
+        return t
+
+    tu = batched(jobs)
+    log(f"batched, default shapes: {tu:.2f} s")
+    tb = batched(tjobs)
+    log(f"batched, autotuned shapes: {tb:.2f} s (tuning {t_tune:.2f} s)")
     # plain: one job per launch sequence, eager, one after the other
     t0 = time.perf_counter()
     lat = []
@@ -290,6 +313,9 @@ def batch(args, smg):
                    "particles": n_tot, "graph_steps": G},
         "plain_jobs_per_s": len(jobs) / tp,
         "speedup_vs_plain": tp / tb,
+        "default_shapes_jobs_per_s": len(jobs) / tu,
+        "autotune": {"seconds": t_tune, "2d": {k: tuned[2][k] for k in ("skin", "max_neighbors")},
+                     "3d": {k: tuned[3][k] for k in ("skin", "max_neighbors")}},
         "latency_s": {"batched_p50": tb, "batched_p99": tb, "plain_p50": pct(lat, 50), "plain_p99": pct(lat, 99)},
         "particle_steps_per_s": {"batched": n_tot * S / tb, "plain": n_tot * S / tp},
     }

@@ -573,8 +573,8 @@ int smg_autotune(const smg_config* cfg, const double* phi, const float* normal, 
   const int64_t m = halo_cells(*cfg);
   double mp = 0.;
   for (int64_t q = 0; q < m; q++) mp = std::max(mp, phi[q]);
-  const double skins[] = {0.05, 0.1, 0.15, 0.25};
-  const int32_t depths[] = {64, 128};
+  const double skins[] = {0.05, 0.1, 0.15, 0.25, 0.4};
+  const int32_t depths[] = {64, 96, 128};   // 15 candidates (the result holds 16)
   const int64_t gsteps = 10;   // steps per captured graph (even)
   const int64_t timed = std::max<int64_t>(gsteps, (steps + gsteps - 1) / gsteps * gsteps);
   res->best_ms_per_step = 1.e300;

@@ -915,7 +915,8 @@ class Server:
 def union_job(dim, n_target, rng, seed, max_pkg=None):
     """One batched-config job: a union of 1-5 circles / spheres (radii U[0.05, 0.25],
     centres U[r, 1-r]^dim) in the unit box, the fluid inside the union, uniform dx
-    chosen so that the particle count (Get_num_particles) is about n_target."""
+    chosen so that the particle count (Get_num_particles) is about n_target; the level
+    set covers the unit box and one package around it."""
     k = int(rng.integers(1, 6))
     radii = rng.uniform(0.05, 0.25, size=k)
     centres = np.stack([rng.uniform(r, 1 - r, size=dim) for r in radii])
@@ -939,8 +940,15 @@ def union_job(dim, n_target, rng, seed, max_pkg=None):
     if max_pkg is None:
         max_pkg = 64 if dim == 2 else 16
     npk = int(np.clip(np.ceil(1.0 / (6.0 * dx)), 6, max_pkg))
+    # The level-set box reaches one package beyond the unit box on every side: the
+    # seeding clamps to the level-set box shrunk by one package
+    # (sph_mesh_generation.cpp:1327-1342), which is then the unit box that holds the
+    # union, so no seeds pile up on the clamp box's faces, edges and corners.
+    dpkg = 1.0 / npk
     for a in range(dim):
-        cfg.num_pkg[a] = npk
+        cfg.num_pkg[a] = npk + 2
+        cfg.box_l[a] = -dpkg
+        cfg.domain[a] = 1.0 + 2.0 * dpkg
     cfg.min_dl = cfg.max_dl = dx
     ls = make_levelset(cfg, sdf)
     return cfg, ls, int(ls["stats"].num_particle)

@@ -197,7 +197,7 @@ def test_server_mix_with_retired_job(smg_mod, orc_stored):
     _say("jobs", ns)
     assert ns[0] < 10000 and ns[-1] > 300000
     chunk = 10
-    scfg = smg_mod.server_config(3, 8, 1 << 21, 1 << 24, 1 << 23, chunk_steps=chunk, min_steps=2 * chunk,
+    scfg = smg_mod.server_config(3, 8, 1 << 21, 1 << 25, 1 << 23, chunk_steps=chunk, min_steps=2 * chunk,
                                  max_steps=3 * chunk, tol=0.0)
     srv = smg_mod.Server(scfg)
     first = srv.admit(*jobs[0])

@@ -11,7 +11,7 @@ def test_autotune_picks_the_fastest_candidate(smg_mod, c1):
     cfg, lset = c1
     n = int(lset["stats"].num_particle)
     r = smg_mod.autotune(cfg, lset, n, warmup=10, steps=20)
-    assert len(r["tried"]) == 8
+    assert len(r["tried"]) == 15   # 5 skins x 3 inline depths
     ms = [t[2] for t in r["tried"]]
     assert all(m > 0 for m in ms)
     assert r["ms_per_step"] == min(ms)
