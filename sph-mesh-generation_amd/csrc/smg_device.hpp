@@ -78,6 +78,7 @@ struct JobParams {
   double cell_ratio, cut_off, nu_coeff;
   double min_dl, max_dl, max_phi, tanh25, lin_dx0, lin_dx1, lin_band;
   double c0, clen[kMaxLevels], cinv[kMaxLevels];
+  double cbox_l[3], cdom[3];                  // the cell grid: origin and extent
   float wl_h[3], wl_l[3], wh_h[3], wh_l[3];   // walls as float pairs
   float cos_thres, kdx;                       // cos(angle_thres), cut_off/cell_ratio
   float pf_tol;                               // FP32 prefilter bound on |dr| error

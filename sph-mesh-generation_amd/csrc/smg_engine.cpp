@@ -161,6 +161,21 @@ int make_job_params(const JobInput& in, int D, JobParams& J) {
   const double hmin = prof.dx_min() * c.cell_ratio / c.cut_off;
   if (!(hmin > 0.)) { set_error("setup: spacing must be positive"); return SMG_ERR_ARG; }
   const double R0 = hmax * J.cut_v * (1. + 1.e-9) + 1.e-9;
+  // The cell grid covers the particle box (seeding and re-seeding draw from it, and
+  // the fluid lies in it) grown by one list radius, inside the level-set box: a small
+  // geometry in a large box gets tables of its own size.
+  dmin = 1.e300;
+  for (int a = 0; a < 3; a++) {
+    J.cbox_l[a] = J.box_l[a];
+    J.cdom[a] = J.domain[a];
+    if (a >= D) continue;
+    const double lo = std::max(J.box_l[a], J.pbox_l[a] - R0), hi = std::min(J.box_r[a], J.pbox_r[a] + R0);
+    if (hi > lo) {
+      J.cbox_l[a] = lo;
+      J.cdom[a] = hi - lo;
+    }
+    dmin = std::min(dmin, J.cdom[a]);
+  }
   int64_t nbox = int64_t(std::floor(dmin / R0));
   if (nbox < 1) nbox = 1;
   J.c0 = dmin / double(nbox);
@@ -168,7 +183,7 @@ int make_job_params(const JobInput& in, int D, JobParams& J) {
   int nlev = 1;
   while (nlev < kMaxLevels && std::ldexp(J.c0, -nlev) >= hmin * J.cut_v * (1. + 1.e-9) + 2.e-10) nlev++;
   int bits0 = 0;
-  for (int a = 0; a < D; a++) bits0 = std::max(bits0, ceil_log2(int64_t(std::ceil(J.domain[a] / J.c0))));
+  for (int a = 0; a < D; a++) bits0 = std::max(bits0, ceil_log2(int64_t(std::ceil(J.cdom[a] / J.c0))));
   while (nlev > 1 && bits0 + nlev - 1 > max_bits) nlev--;
   if (bits0 > max_bits) { set_error("setup: domain has too many cells per axis"); return SMG_ERR_CAPACITY; }
   J.nlev = nlev;
@@ -176,7 +191,7 @@ int make_job_params(const JobInput& in, int D, JobParams& J) {
   for (int L = 0; L < nlev; L++) {
     J.clen[L] = std::ldexp(J.c0, -L);
     J.cinv[L] = 1.0 / J.clen[L];
-    for (int a = 0; a < 3; a++) J.lev_n[L][a] = a < D ? int32_t(std::ceil(J.domain[a] / J.clen[L])) : 1;
+    for (int a = 0; a < 3; a++) J.lev_n[L][a] = a < D ? int32_t(std::ceil(J.cdom[a] / J.clen[L])) : 1;
     J.lev_bits[L] = bits0 + L;
     J.lev_off[L] = off;
     off += int64_t(1) << (D * J.lev_bits[L]);

@@ -373,7 +373,7 @@ __device__ __forceinline__ void cell_at(const JobParams& J, int L, const double*
   for (int a = 0; a < D; a++) {
     // clamp in FP64 before the conversion so that a non-finite position still lands
     // in a valid cell of the job's table
-    double t = floor((x[a] - J.box_l[a]) * J.cinv[L]);
+    double t = floor((x[a] - J.cbox_l[a]) * J.cinv[L]);
     const double hi = double(J.lev_n[L][a] - 1);
     t = (t >= 0.) ? t : 0.;
     t = (t <= hi) ? t : hi;

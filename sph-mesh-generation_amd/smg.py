@@ -916,7 +916,7 @@ def union_job(dim, n_target, rng, seed, max_pkg=None):
     """One batched-config job: a union of 1-5 circles / spheres (radii U[0.05, 0.25],
     centres U[r, 1-r]^dim) in the unit box, the fluid inside the union, uniform dx
     chosen so that the particle count (Get_num_particles) is about n_target; the level
-    set covers the unit box and one package around it."""
+    set covers the unit box and one package around it, the particle box the union."""
     k = int(rng.integers(1, 6))
     radii = rng.uniform(0.05, 0.25, size=k)
     centres = np.stack([rng.uniform(r, 1 - r, size=dim) for r in radii])
@@ -949,6 +949,9 @@ def union_job(dim, n_target, rng, seed, max_pkg=None):
         cfg.num_pkg[a] = npk + 2
         cfg.box_l[a] = -dpkg
         cfg.domain[a] = 1.0 + 2.0 * dpkg
+        # seeds are drawn from the union's bounding box (one dx wider)
+        cfg.particle_box_l[a] = max(0.0, float(np.min(centres[:, a] - radii)) - dx)
+        cfg.particle_box_r[a] = min(1.0, float(np.max(centres[:, a] + radii)) + dx)
     cfg.min_dl = cfg.max_dl = dx
     ls = make_levelset(cfg, sdf)
     return cfg, ls, int(ls["stats"].num_particle)
