@@ -88,6 +88,21 @@ def relax(args, smg):
     lset = smg.make_levelset(cfg, sdf)
     n = int(lset["stats"].num_particle)
     log(f"level set built in {time.perf_counter() - t0:.1f} s, n = {n}")
+    tune = None
+    if args.skin is None and not args.no_tune:
+        # list shape (skin, i.e. the cell-size multiple, and inline depth) from the
+        # autotuner, timed on scratch copies over the same window (warmup, steps) as the
+        # measurement below; tuning is not part of the timed region
+        t0 = time.perf_counter()
+        r = smg.autotune(cfg, lset, n, warmup=args.warmup, steps=args.steps)
+        cfg.skin = r["skin"]
+        cfg.max_neighbors = r["max_neighbors"]
+        dflt = [t[2] for t in r["tried"] if t[0] == 0.1 and t[1] == 128]
+        tune = {"skin": r["skin"], "max_neighbors": r["max_neighbors"], "seconds": time.perf_counter() - t0,
+                "tuned_ms_per_step": r["ms_per_step"],
+                "default_shape_ms_per_step": dflt[0] if dflt else None,
+                "candidates": len(r["tried"])}
+        log(f"autotune: {json.dumps(tune)}")
     job = smg.Job(cfg, lset, profile=False)
     t0 = time.perf_counter()
     job.seed(n)
@@ -176,6 +191,7 @@ def relax(args, smg):
             "frac": STEP_BYTES[0 if D == 3 else 1] * value / 1e9 / HBM_PEAK_GBS,
         },
         "list_rebuilds_per_step": rebuild_frac,
+        "autotune": tune,
         "eager_ms_per_step": elapsed_eager / steps * 1e3,
         "kernels_ms": {k: v[0] for k, v in prof.items()},
     }
@@ -659,7 +675,8 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--max-active", type=int, default=4000000)
     ap.add_argument("--c3-max-steps", type=int, default=6000)
-    ap.add_argument("--skin", type=float, default=None, help="relax mode: list skin (default: the config's)")
+    ap.add_argument("--skin", type=float, default=None, help="relax mode: list skin (default: the autotuner's)")
+    ap.add_argument("--no-tune", action="store_true", help="relax mode: keep the config's list shape")
     ap.add_argument("--full-run", type=int, default=1000, help="relax mode: also time a whole run of this many steps (0: off)")
     args = ap.parse_args()
     if args.gpus != 1:
