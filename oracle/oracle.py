@@ -217,7 +217,12 @@ def step(cfg, lset, s, step_index, want_terms=False, want_nb=False):
     return out
 
 
-def neighbors(cfg, x, h, brute=False):
+def neighbors(cfg, x, h, brute=False, method=None):
+    """Neighbour sets (CSR by id, entries sorted by (id, mirror code)).  method: "grid"
+    (default) or "brute" for the predicate form; "tree" / "scll" for the restatement of
+    the reference's own tree search in its default / _SCLL_ build."""
+    if method is not None:
+        brute = {"grid": 0, "brute": 1, "tree": 2, "scll": 3}[method]
     n = len(h)
     x3 = np.zeros((n, 3)); x3[:, :x.shape[1]] = x
     h = np.ascontiguousarray(h, np.float64)

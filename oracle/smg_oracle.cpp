@@ -11,7 +11,8 @@
 //
 // Pinning (see tests/test_oracle_*.py): kernel normalisation and continuity, lattice
 // symmetry (zero force on a perfect lattice), hand two-particle cases, brute-force
-// O(N^2) neighbour sets, closed-form level-set sampling, mirror geometry, and an
+// O(N^2) neighbour sets, the reference's own tree search restated in its default and
+// _SCLL_ builds (neighbors_tree), closed-form level-set sampling, mirror geometry, and an
 // independent long-double evaluation of a step.  Functions with no such check are
 // listed as "parity unpinned" in DESIGN.md.
 #include <cmath>
@@ -501,6 +502,230 @@ void neighbors_brute(const Ctx& c, const std::vector<P>& ps, std::vector<std::ve
   }
 }
 
+// The reference's own neighbour search, restated with its data structures, to pin the
+// predicate form of neighbors() above; scll selects the _SCLL_ build.
+// * The tree: max_scale = maximum_dl CELL_RATIO (= the largest h CUT_OFF),
+//   min_scale likewise (sph_mesh_generation.cpp:781-783); Define_level_infor_tight
+//   (sph.cpp:144-180) fits max_scale to the domain and counts the levels; level L has
+//   ini_num_cell 2^L cells per axis of dcell = domain / num_cell and a margin of 2^L
+//   cells on each mirrored side (Level_info::Initialize, level_infor.cpp:35-102).
+// * A particle's level: Set_particle_scale from max_scale (particle.cpp:957-975,
+//   particle_mesh_generation.cpp:57), matched to the level scales
+//   (Map_the_particle_to_tree, sph.cpp:247-276); its cell: get_cell_id (glbfunc.h).
+// * Mirror ghosts per level, copied from the mirrored cells before finer particles are
+//   merged (Construct_symmetric_BC_particles, level_infor.cpp:786-856, called before
+//   Update_every_level_info at sph_mesh_generation.cpp:2061-2068), then every level's
+//   cells take the finer levels' particles and ghosts (Update_every_level_info,
+//   level_infor.cpp:407-464).
+// * Every particle and every ghost scans (SPH::Refresh_neighbor_info, sph.cpp:345-360):
+//   the default build the 3^D cells around it, adding the neighbour to itself and
+//   itself to a neighbour of another level (level_infor.cpp:469-506); the _SCLL_ build
+//   the sub-cells (cells split in 3 per axis, lists sorted by sub-cell key with start
+//   and end per key: cell_list.cpp:103-137, keys as in Update_cell_list,
+//   level_infor.cpp:325-344) within int((h CUT_OFF + dsub - 1e-10) / dsub) of its own,
+//   adding both ways when h_current > h_neighbour and one way when equal
+//   (level_infor.cpp:508-567).  What lands in a ghost's list is dropped: ghosts are
+//   rebuilt every step and their lists are never read.
+// Returns the real particles' lists (entries: source id and mirror code), sorted.
+void neighbors_tree(const Ctx& c, const std::vector<P>& ps, bool scll, std::vector<std::vector<NbEntry>>& nb) {
+  const int n = int(ps.size()), D = c.D;
+  const double cut = c.cfg->cut_off;
+  double hmax = 0., hmin = 1.e300;
+  for (const P& p : ps) { hmax = std::max(hmax, p.h); hmin = std::min(hmin, p.h); }
+  // Define_level_infor_tight
+  double dmin = 1.e300;
+  for (int a = 0; a < D; a++) dmin = std::min(dmin, c.cfg->domain[a]);
+  const double max0 = hmax * cut, min_scale0 = hmin * cut;
+  const double ini = dmin / std::floor(dmin / max0);
+  double st = ini;
+  int nlev = 1;
+  while ((st + 1.e-6) > min_scale0) { st /= 2.; nlev++; }
+  nlev -= 1;
+  int ini_nc[3];
+  for (int a = 0; a < 3; a++) ini_nc[a] = a < D ? int(c.cfg->domain[a] / ini) : 1;
+  struct Lev { int nc[3], cs[3], ce[3]; double dc[3], scale; };
+  std::vector<Lev> lv(nlev);
+  for (int L = 0; L < nlev; L++) {
+    Lev& v = lv[L];
+    v.scale = 1.e300;
+    for (int a = 0; a < 3; a++) {
+      const bool on = a < D;
+      v.nc[a] = on ? ini_nc[a] << L : 1;
+      v.dc[a] = on ? c.cfg->domain[a] / double(v.nc[a]) : 1.;
+      if (on) v.scale = std::min(v.scale, v.dc[a]);
+      const int dN = (on && c.cfg->sym[a]) ? (1 << L) : 0;
+      v.cs[a] = -dN; v.ce[a] = v.nc[a] + dN;
+    }
+  }
+  // an entry of a cell list: a real particle (code 0) or a ghost, at a level
+  struct Ent { double x[3]; int src, code, level; double h; bool ghost; };
+  std::vector<Ent> ents;
+  auto level_of = [&](double h) {
+    double scale = ini;
+    for (int guard = 0; guard < 64; guard++) {
+      if (h * cut > scale + 1.e-15) scale *= 2.;
+      else if (h * cut < scale / 2. - 1.e-15) scale /= 2.;
+      else break;
+    }
+    int lev = 0;
+    for (int L = nlev - 1; L >= 0; L--)
+      if (std::fabs(scale - lv[L].scale) <= 1.e-10) { lev = L; break; }
+    return lev;
+  };
+  auto cell_id = [&](const Lev& v, const double* x, int* ci) {   // get_cell_id, clamped
+    for (int a = 0; a < 3; a++) {
+      if (a >= D) { ci[a] = 0; continue; }
+      int k = int(std::floor((x[a] - c.cfg->box_l[a]) / v.dc[a]));
+      ci[a] = std::min(std::max(k, v.cs[a]), v.ce[a] - 1);
+    }
+  };
+  std::vector<int> plev(n);
+  for (int i = 0; i < n; i++) {
+    plev[i] = level_of(ps[i].h);
+    Ent e; for (int a = 0; a < 3; a++) e.x[a] = ps[i].x[a];
+    e.src = i; e.code = 0; e.level = plev[i]; e.h = ps[i].h; e.ghost = false;
+    ents.push_back(e);
+  }
+  // ghosts of each level's own particles, from the cells the margin mirrors
+  for (int i = 0; i < n; i++) {
+    const Lev& v = lv[plev[i]];
+    int ci[3]; cell_id(v, ps[i].x, ci);
+    int opt[3][2], nopt[3];
+    for (int a = 0; a < 3; a++) {
+      nopt[a] = 1; opt[a][0] = 0;
+      if (a < D && c.cfg->sym[a]) {
+        const int dN = -v.cs[a];
+        if (ci[a] < dN) opt[a][nopt[a]++] = 1;                    // mirrored into the low margin
+        if (ci[a] >= v.nc[a] - dN) opt[a][nopt[a]++] = 2;         // ... the high margin
+      }
+    }
+    for (int o0 = 0; o0 < nopt[0]; o0++)
+      for (int o1 = 0; o1 < nopt[1]; o1++)
+        for (int o2 = 0; o2 < nopt[2]; o2++) {
+          const int code = opt[0][o0] | (opt[1][o1] << 2) | (opt[2][o2] << 4);
+          if (!code) continue;
+          Ent e; image(c, code, ps[i].x, e.x);
+          e.src = i; e.code = code; e.level = plev[i]; e.h = ps[i].h; e.ghost = true;
+          ents.push_back(e);
+        }
+  }
+  // the cell lists of every level: entries of that level and finer (a finer entry sits
+  // in the coarse cell that holds its finer cell)
+  std::vector<std::vector<std::vector<int>>> cells(nlev);
+  auto cidx = [&](const Lev& v, const int* ci) -> int64_t {
+    return (int64_t(ci[2] - v.cs[2]) * (v.ce[1] - v.cs[1]) + (ci[1] - v.cs[1])) * (v.ce[0] - v.cs[0]) + (ci[0] - v.cs[0]);
+  };
+  for (int L = 0; L < nlev; L++) {
+    const Lev& v = lv[L];
+    cells[L].assign(size_t(v.ce[0] - v.cs[0]) * (v.ce[1] - v.cs[1]) * (v.ce[2] - v.cs[2]), {});
+    for (size_t q = 0; q < ents.size(); q++) {
+      if (ents[q].level < L) continue;
+      int ci[3]; cell_id(v, ents[q].x, ci);
+      // a ghost lies in the margin; an entry outside the grid's margin is not in a cell
+      bool inside = true;
+      for (int a = 0; a < D; a++) {
+        const int k = int(std::floor((ents[q].x[a] - c.cfg->box_l[a]) / v.dc[a]));
+        inside = inside && k >= v.cs[a] && k < v.ce[a];
+      }
+      if (!inside) continue;
+      cells[L][cidx(v, ci)].push_back(int(q));
+    }
+  }
+  // _SCLL_: each cell's list sorted by sub-cell key (a stable sort, as the LSD radix
+  // sort of parallelfunc.h:39-96), start and end per key
+  const int NS = 3;
+  auto subkey = [&](const Lev& v, const double* x, const int* ci) {
+    int ps3[3] = {0, 0, 0};
+    for (int a = 0; a < D; a++) {
+      const double dsub = v.dc[a] / NS;
+      const double cs = (x[a] - c.cfg->box_l[a]) - v.dc[a] * ci[a];
+      int k = int(std::floor(cs / dsub));
+      ps3[a] = std::max(0, std::min(k, NS - 1));
+    }
+    return (ps3[2] * NS + ps3[1]) * NS + ps3[0];
+  };
+  if (scll)
+    for (int L = 0; L < nlev; L++)
+      for (size_t k = 0; k < cells[L].size(); k++) {
+        auto& lst = cells[L][k];
+        if (lst.empty()) continue;
+        const Lev& v = lv[L];
+        int ci[3];
+        int64_t r = int64_t(k);
+        const int wx = v.ce[0] - v.cs[0], wy = v.ce[1] - v.cs[1];
+        ci[0] = int(r % wx) + v.cs[0]; r /= wx;
+        ci[1] = int(r % wy) + v.cs[1]; r /= wy;
+        ci[2] = int(r) + v.cs[2];
+        std::stable_sort(lst.begin(), lst.end(), [&](int a, int b) {
+          return subkey(v, ents[a].x, ci) < subkey(v, ents[b].x, ci); });
+      }
+  std::vector<std::vector<NbEntry>> all(ents.size());
+  auto add = [&](int to, int from) {   // "to" gains "from" (ghost lists are dropped)
+    if (ents[to].ghost) return;
+    all[to].push_back({ents[from].src, ents[from].code});
+  };
+  for (size_t q = 0; q < ents.size(); q++) {
+    const Ent& cur = ents[q];
+    const int L = cur.level;
+    const Lev& v = lv[L];
+    int ci[3]; cell_id(v, cur.x, ci);
+    if (!scll) {
+      const int r[3] = {1, D >= 2 ? 1 : 0, D >= 3 ? 1 : 0};
+      for (int t = std::max(ci[0] - r[0], v.cs[0]); t <= std::min(ci[0] + r[0], v.ce[0] - 1); t++)
+        for (int u = std::max(ci[1] - r[1], v.cs[1]); u <= std::min(ci[1] + r[1], v.ce[1] - 1); u++)
+          for (int w = std::max(ci[2] - r[2], v.cs[2]); w <= std::min(ci[2] + r[2], v.ce[2] - 1); w++) {
+            const int cc[3] = {t, u, w};
+            for (int e : cells[L][cidx(v, cc)]) {
+              const Ent& nbr = ents[e];
+              const double d = dist_of(D, cur.x, nbr.x);
+              if (nb_pred(d, cur.h, nbr.h, cut)) {
+                add(int(q), e);
+                if (nbr.level != L) add(e, int(q));
+              }
+            }
+          }
+    } else {
+      // sub-cell of cur and its search range (level_infor.cpp:510-523)
+      int sc[3], rg[3], s0[3], s1[3];
+      for (int a = 0; a < 3; a++) {
+        if (a >= D) { sc[a] = 0; rg[a] = 0; s0[a] = 0; s1[a] = 1; continue; }
+        const double dsub = v.dc[a] / NS;
+        const double cs = (cur.x[a] - c.cfg->box_l[a]) - v.dc[a] * ci[a];
+        sc[a] = ci[a] * NS + std::max(0, std::min(int(std::floor(cs / dsub)), NS - 1));
+        rg[a] = int((cur.h * cut + dsub - 1.e-10) / dsub);
+        s0[a] = std::max(sc[a] - rg[a], v.cs[a] * NS);
+        s1[a] = std::min(sc[a] + rg[a] + 1, v.ce[a] * NS);
+      }
+      for (int sx = s0[0]; sx < s1[0]; sx++)
+        for (int sy = s0[1]; sy < s1[1]; sy++)
+          for (int sz = s0[2]; sz < s1[2]; sz++) {
+            const int sj[3] = {sx, sy, sz};
+            int tc[3], sub[3] = {0, 0, 0};
+            for (int a = 0; a < 3; a++) {
+              tc[a] = a < D ? int(std::floor(double(sj[a]) / NS)) : 0;
+              if (a < D) sub[a] = sj[a] - NS * tc[a];
+            }
+            const int key = (sub[2] * NS + sub[1]) * NS + sub[0];
+            for (int e : cells[L][cidx(v, tc)]) {
+              if (subkey(v, ents[e].x, tc) != key) continue;   // this key's run of the sorted list
+              const Ent& nbr = ents[e];
+              const double d = dist_of(D, cur.x, nbr.x);
+              if (nb_pred(d, cur.h, nbr.h, cut)) {
+                if (cur.h > nbr.h) { add(int(q), e); add(e, int(q)); }
+                else if (cur.h == nbr.h) add(int(q), e);
+              }
+            }
+          }
+    }
+  }
+  nb.assign(n, {});
+  for (int i = 0; i < n; i++) {
+    nb[i] = all[i];
+    std::sort(nb[i].begin(), nb[i].end(), [](const NbEntry& a, const NbEntry& b) {
+      return a.j != b.j ? a.j < b.j : a.code < b.code; });
+  }
+}
+
 inline double dot(int D, const double* a, const double* b) {   // get_dot_product, glbfunc.h:263-271
   double v = 0.0;
   for (int k = 0; k < D; k++) v += a[k] * b[k];
@@ -833,7 +1058,8 @@ int or_step(const smg_config* cfg, const double* lphi, const float* lnrm, const 
   return o.ok ? SMG_OK : SMG_ERR_REMAP;
 }
 
-// Neighbour sets of given (x, h) with mirror ghosts; brute = 1 for the O(N^2) enumeration.
+// Neighbour sets of given (x, h) with mirror ghosts; brute = 1 for the O(N^2) enumeration,
+// 2 / 3 for the restatement of the reference's tree search (default / _SCLL_ build).
 int or_neighbors(const smg_config* cfg, int64_t n, const double* x, const double* h, int brute,
                  int64_t* offs, int32_t* ids, int32_t* codes, int64_t cap) {
   Ctx c; ctx_init(c, cfg, nullptr, nullptr, nullptr);
@@ -844,7 +1070,9 @@ int or_neighbors(const smg_config* cfg, int64_t n, const double* x, const double
     ps[i].h = h[i]; ps[i].id = int(i);
   }
   std::vector<std::vector<NbEntry>> nb;
-  if (brute) neighbors_brute(c, ps, nb); else neighbors(c, ps, nb);
+  if (brute == 1) neighbors_brute(c, ps, nb);
+  else if (brute == 2 || brute == 3) neighbors_tree(c, ps, brute == 3, nb);
+  else neighbors(c, ps, nb);
   int64_t k = 0; offs[0] = 0;
   for (int64_t i = 0; i < n; i++) {
     for (const NbEntry& e : nb[i]) { if (ids && k < cap) { ids[k] = e.j; codes[k] = e.code; } k++; }

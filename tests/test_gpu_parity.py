@@ -97,6 +97,23 @@ def test_neighbors_match_brute_force(smg_mod, orc):
     assert np.array_equal(offs, bo) and np.array_equal(ids, bi) and np.array_equal(codes, bc)
 
 
+@pytest.mark.parametrize("method", ["tree", "scll"])
+def test_neighbors_match_the_reference_tree_search(smg_mod, orc, method):
+    """The engine's lists equal the restatement of the reference's own tree search, in
+    its default build and in its _SCLL_ build (sub-cell lists, the h-ordered add rule),
+    on a 3D two-level job after a few steps (the CPU tests cover mirror ghosts)."""
+    cfg, lset = _c3d(smg_mod, scale=3.0, num_pkg=8)
+    n = int(lset["stats"].num_particle)
+    job = smg_mod.Job(cfg, lset)
+    job.seed(n)
+    job.step(5)
+    g = job.state()
+    offs, ids, codes = job.neighbors()
+    assert job.stats()["num_level"] >= 2
+    to, ti, tc = orc.neighbors(cfg, g["x"], g["h"], method=method)
+    assert np.array_equal(offs, to) and np.array_equal(ids, ti) and np.array_equal(codes, tc)
+
+
 def test_deterministic_replay(smg_mod, c1):
     cfg, lset = c1
     n = int(lset["stats"].num_particle)

@@ -76,3 +76,44 @@ def test_symmetry_of_real_pairs(smg_mod, orc):
                 pairs.add((i, int(ids[k])))
     assert all((j, i) in pairs for (i, j) in pairs)
     assert all((i, i) in pairs for i in range(400))   # the reference list holds the particle itself
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("scll", [False, True])
+def test_reference_tree_search_gives_the_predicate_sets(smg_mod, orc, dim, scll):
+    """The reference's own search -- tree levels, per-level mirror ghosts, finer
+    particles merged into coarse cells, every particle and ghost scanning (default: 3^D
+    cells, cross-level hits added both ways; _SCLL_: sub-cell lists and the h-ordered
+    add rule) -- gives exactly the predicate sets, so the engine's lists serve both
+    builds.  Spacings spread over two to three tree levels; walls mirror everywhere."""
+    rng = np.random.default_rng(31 + dim + 10 * scll)
+    n = 900 if dim == 2 else 700
+    cfg = _cfg(smg_mod, dim)
+    x = rng.uniform(0.0, 1.0, size=(n, dim))
+    hi = 0.03 if dim == 2 else 0.06
+    h = hi * np.exp(rng.uniform(np.log(0.3), 0.0, size=n))
+    # a quarter of the particles share one spacing (equal-h pairs take the one-way rule)
+    h[: n // 4] = h[0]
+    want = orc.neighbors(cfg, x, h, method="brute")
+    got = orc.neighbors(cfg, x, h, method="scll" if scll else "tree")
+    for u, v in zip(want, got):
+        assert np.array_equal(u, v)
+    offs, ids, codes = got
+    assert np.count_nonzero(codes) > 0
+    # the pairs cross levels: the largest and smallest h differ by more than 2x
+    assert h.max() / h.min() > 2.5
+
+
+def test_tree_search_on_a_lattice_with_walls(smg_mod, orc):
+    """A 2D lattice touching the walls (ties at the cutoff are the predicate's 1e-10
+    margin, not the tree's): both builds give the predicate sets."""
+    cfg = _cfg(smg_mod, 2, sym=(1, 1, 0))
+    g = (np.arange(25) + 0.5) / 25.0
+    X, Y = np.meshgrid(g, g)
+    x = np.stack([X.ravel(), Y.ravel()], axis=1)
+    h = np.full(len(x), 1.0 / 25.0)
+    want = orc.neighbors(cfg, x, h, method="brute")
+    for m in ("tree", "scll"):
+        got = orc.neighbors(cfg, x, h, method=m)
+        for u, v in zip(want, got):
+            assert np.array_equal(u, v), m
