@@ -39,10 +39,11 @@ constexpr int kJumpMax = 64;     // particles per job that may outrun their list
 constexpr int kJumpList = 512;   // candidate entries of one jumper's own list
 constexpr int kJumpBit = 1 << 30;   // flag in the type bits of a record: the slot is a jumper
 
-// Level-set cells live in two arrays of the same (bricked) cell order: phi in FP64
-// (used for projection) and float4 (raw normal xyz, interface tag as int bits).  The
-// interior particles' samples need phi only and read 8 bytes per corner.
-constexpr int64_t kLsCellBytes = 8 + 16;
+// Level-set cells live in three arrays of the same (bricked) cell order: phi in FP64
+// (used for projection), float4 (raw normal xyz, interface tag as int bits) and the
+// tag again as one byte.  The interior particles' samples need phi only and read 8
+// bytes per corner; the tag lookup reads the byte array.
+constexpr int64_t kLsCellBytes = 8 + 16 + 1;
 
 // Refined level-set levels above a job's base grid (a multi-resolution extension: the
 // reference's level set is single-level).  Level l has num_pkg << l packages per
@@ -163,6 +164,8 @@ struct Work {
   JobAcc* acc;      // kAccCopies per job, [job * kAccCopies + copy]
   const double* lsp;   // level-set phi per cell
   const float4* lsn;   // level-set (normal, tag bits) per cell
+  const int8_t* lst;   // the tag alone, one byte per cell: the per-step tag lookup of
+                       // every particle reads a 16x smaller array
   const int32_t* lpm;  // refined level-set package maps (brick index or -1)
   float* nu_a;   // nu read by the stage-2 pass (previous step's value)
   float* nu_b;   // nu written by the stage-2 pass

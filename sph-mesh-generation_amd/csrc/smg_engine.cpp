@@ -243,6 +243,7 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   d_acc_ = alloc_<JobAcc>(int64_t(njobs) * kAccCopies);
   d_lsp_ = alloc_<double>(n_ls_);
   d_lsn_ = alloc_<float4>(n_ls_);
+  d_lst_ = alloc_<int8_t>(n_ls_);
   d_lpm_ = alloc_<int32_t>(std::max<int64_t>(maps, 1));
   PState* S2[2] = {&A_, &B_};
   for (PState* S : S2) {
@@ -312,6 +313,7 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   W_.acc = d_acc_;
   W_.lsp = d_lsp_;
   W_.lsn = d_lsn_;
+  W_.lst = d_lst_;
   W_.lpm = d_lpm_;
   W_.cap_total = cap_total_;
   W_.n_tbl = n_tbl_;
@@ -437,6 +439,13 @@ int Engine::upload_job_(int j, const JobInput& in) {
   }
   HIP_TRY(hipMemcpyAsync(d_lsp_ + J.lset_off, hp.data(), sizeof(double) * m, hipMemcpyHostToDevice, st_));
   HIP_TRY(hipMemcpyAsync(d_lsn_ + J.lset_off, hn.data(), sizeof(float4) * m, hipMemcpyHostToDevice, st_));
+  std::vector<int8_t> ht(m);
+  for (int64_t o = 0; o < m; o++) {
+    int32_t tag;
+    std::memcpy(&tag, &hn[o].w, sizeof(tag));
+    ht[o] = int8_t(tag);
+  }
+  HIP_TRY(hipMemcpyAsync(d_lst_ + J.lset_off, ht.data(), size_t(m), hipMemcpyHostToDevice, st_));
   if (!hm.empty())
     HIP_TRY(hipMemcpyAsync(d_lpm_ + J.lmap_base, hm.data(), sizeof(int32_t) * hm.size(), hipMemcpyHostToDevice, st_));
   HIP_TRY(hipStreamSynchronize(st_));   // the host buffers go out of scope

@@ -177,6 +177,7 @@ class Engine {
   JobAcc* d_acc_ = nullptr;
   double* d_lsp_ = nullptr;
   float4* d_lsn_ = nullptr;
+  int8_t* d_lst_ = nullptr;
   int32_t* d_lpm_ = nullptr;
   PState A_{}, B_{};
   Work W_{};

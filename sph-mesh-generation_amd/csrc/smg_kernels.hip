@@ -289,7 +289,7 @@ __device__ int ls_tag_base(const JobParams& J, const Work& W, const double* x) {
     c = (0 >= c) ? 0 : c;
     g[a] = pk[a] * 4 + c;
   }
-  return __float_as_int(W.lsn[ls_index(J, D, g[0], g[1], g[2])].w);
+  return W.lst[ls_index(J, D, g[0], g[1], g[2])];
 }
 
 // Levelset_levelinfo::Get_id_pkg_cell (lset_level_infor.cpp:350-366) -> tag_interface,
@@ -313,7 +313,7 @@ __device__ int ls_tag(const JobParams& J, const Work& W, const double* x) {
       g[a] = p * 4 + c;
     }
     const int64_t c = lvl_cell<D>(J, W, l, g);
-    if (c >= 0) return __float_as_int(W.lsn[c].w);
+    if (c >= 0) return W.lst[c];
   }
   return ls_tag_base<D>(J, W, x);
 }
