@@ -1051,6 +1051,12 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
     }
   }
   const float cutf = float(J.cut_v);
+  // the job's constants in registers: JobParams sits in global memory, and the list
+  // stores of emit would otherwise make the loops below read them again each time
+  const float pf_tol = J.pf_tol;
+  const int64_t p0 = J.p0;
+  const int nlev = J.nlev;
+  const int64_t tbl = J.tbl_base;
   // one image: every level and stencil cell; the plain image (code 0, the first for
   // every lane) tests xh_i - xh_q directly, a mirror image A + sg xh_q
   auto image = [&](uint32_t code, auto plain) {
@@ -1064,38 +1070,61 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
       A[a] = m == 0 ? xh[a] : (m == 1 ? xh[a] - 2.f * J.wl_h[a] : xh[a] - 2.f * J.wh_h[a]);
       sg[a] = m == 0 ? -1.f : 1.f;
     }
-    for (int L = 0; L < J.nlev; L++) {
+    // one candidate: the FP32 distance of the hi parts against the widened radius
+    auto test = [&](const float4& qr) -> bool {
+      const float qv[3] = {qr.x, qr.y, qr.z};
+      float d2 = 0.f;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        const float t = decltype(plain)::value ? xh[a] - qv[a] : A[a] + sg[a] * qv[a];
+        d2 += t * t;
+      }
+      const float r = fmaxf(rfi, qr.w * cutf);
+      const float rhi = r + (pf_tol + 1.e-6f * r);
+      return d2 <= rhi * rhi;
+    };
+    for (int L = 0; L < nlev; L++) {
       const int s = L < li ? L : li;
       const int shift = D * (L - s);
       int cy[3] = {0, 0, 0};
       cell_at<D>(J, s, y, cy);
-      const int64_t base = J.tbl_base + J.lev_off[L];
+      const int ln[3] = {J.lev_n[s][0], J.lev_n[s][1], D > 2 ? J.lev_n[s][2] : 1};
+      const int32_t* st = W.starts + (tbl + J.lev_off[L]);
       for (int dz = (D > 2 ? -1 : 0); dz <= (D > 2 ? 1 : 0); dz++)
-        for (int dy = -1; dy <= 1; dy++)
-          for (int dx = -1; dx <= 1; dx++) {
-            int c[3] = {cy[0] + dx, cy[1] + dy, cy[2] + dz};
-            if (c[0] < 0 || c[0] >= J.lev_n[s][0] || c[1] < 0 || c[1] >= J.lev_n[s][1]) continue;
-            if (D > 2 && (c[2] < 0 || c[2] >= J.lev_n[s][2])) continue;
-            const uint64_t m = morton<D>(c);
-            const int32_t lo = W.starts[base + int64_t(m << shift)] + soff;
-            const int32_t hi2 = W.starts[base + int64_t((m + 1) << shift)] + soff;
-            for (int32_t q = lo; q < hi2; q++) {
-              const float4 qr = W.recb[q];   // == rec[0] at a build, 16-byte stride
-              const float qv[3] = {qr.x, qr.y, qr.z};
-              float d2 = 0.f;
+        for (int dy = -1; dy <= 1; dy++) {
+          // the slot ranges of the row's three cells, their bounds loaded together
+          int32_t lo[3], hi2[3];
 #pragma unroll
-              for (int a = 0; a < D; a++) {
-                const float t = decltype(plain)::value ? xh[a] - qv[a] : A[a] + sg[a] * qv[a];
-                d2 += t * t;
-              }
-              const float r = fmaxf(rfi, qr.w * cutf);
-              const float rhi = r + (J.pf_tol + 1.e-6f * r);
-              if (d2 <= rhi * rhi) {
-                emit(cnt, int32_t(nb_pack(q, J.p0, code)));
-                cnt++;
+          for (int u = 0; u < 3; u++) {
+            int c[3] = {cy[0] + u - 1, cy[1] + dy, cy[2] + dz};
+            bool ok = c[0] >= 0 && c[0] < ln[0] && c[1] >= 0 && c[1] < ln[1];
+            if (D > 2) ok = ok && c[2] >= 0 && c[2] < ln[2];
+            lo[u] = 0;
+            hi2[u] = 0;
+            if (ok) {
+              const uint64_t m = morton<D>(c);
+              lo[u] = st[int64_t(m << shift)] + soff;
+              hi2[u] = st[int64_t((m + 1) << shift)] + soff;
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 3; u++) {
+            // candidates four at a time: the four records are loaded before any is
+            // tested (an index past the range repeats the range's last slot)
+            for (int32_t q = lo[u]; q < hi2[u]; q += 4) {
+              float4 qr[4];
+#pragma unroll
+              for (int v = 0; v < 4; v++) qr[v] = W.recb[q + v < hi2[u] ? q + v : hi2[u] - 1];   // == rec[0] at a build
+#pragma unroll
+              for (int v = 0; v < 4; v++) {
+                if (q + v < hi2[u] && test(qr[v])) {
+                  emit(cnt, int32_t(nb_pack(q + v, p0, code)));
+                  cnt++;
+                }
               }
             }
           }
+        }
     }
   };
   image(0u, std::true_type{});
@@ -1530,8 +1559,11 @@ __device__ __forceinline__ int block_scan_excl(int v, int* lds4, int& total) {
 template <class Fn>
 __device__ __forceinline__ void for_entries(const Work& W, int64_t slot, int cnt, int32_t off, Fn fn) {
   const int ni = cnt < W.K ? cnt : W.K;
+  // the next group's load is issued before this group is used
+  uint4 vn = ni > 0 ? nb_group(W, slot, 0) : make_uint4(0u, 0u, 0u, 0u);
   for (int g = 0; 4 * g < ni; g++) {
-    const uint4 v = nb_group(W, slot, g);
+    const uint4 v = vn;
+    if (4 * g + 4 < ni) vn = nb_group(W, slot, g + 1);
     const uint32_t ev[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int u = 0; u < 4; u++)
