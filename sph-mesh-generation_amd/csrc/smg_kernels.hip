@@ -1090,15 +1090,40 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
       cell_at<D>(J, s, y, cy);
       const int ln[3] = {J.lev_n[s][0], J.lev_n[s][1], D > 2 ? J.lev_n[s][2] : 1};
       const int32_t* st = W.starts + (tbl + J.lev_off[L]);
+      // Stencil cells that no candidate can reach are skipped.  A slot of the level-L
+      // table has h cut_v + 2e-10 <= clen[L] (level_of), so a candidate passes only
+      // within R = max(rfi, clen[L]) (+ the FP32 slack: pf_tol of the radius, pf_tol of
+      // the FP32 distance, relative roundings) of y.  The slots of stencil cell
+      // cy + o lie beyond the faces y's own cell shares with it (a position clamped
+      // into a border cell lies beyond its inner face too), so their distance from y
+      // is at least the gap gq[a][o_a + 1] to those faces on each axis.
+      float gq[3][3];
+      {
+        const double cl = J.clen[s];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          gq[a][0] = gq[a][1] = gq[a][2] = 0.f;
+          if (a < D) {
+            const double f = J.cbox_l[a] + double(cy[a]) * cl;
+            const float gm = float(fmax(0., y[a] - f)), gp = float(fmax(0., f + cl - y[a]));
+            gq[a][0] = gm * gm;
+            gq[a][2] = gp * gp;
+          }
+        }
+      }
+      const float Rl = fmaxf(rfi, float(J.clen[L])) * (1.f + 1.e-5f) + 2.f * pf_tol + 1.e-9f;
+      const float R2 = Rl * Rl;
       for (int dz = (D > 2 ? -1 : 0); dz <= (D > 2 ? 1 : 0); dz++)
         for (int dy = -1; dy <= 1; dy++) {
           // the slot ranges of the row's three cells, their bounds loaded together
           int32_t lo[3], hi2[3];
+          const float g2 = gq[1][dy + 1] + gq[2][dz + 1];
 #pragma unroll
           for (int u = 0; u < 3; u++) {
             int c[3] = {cy[0] + u - 1, cy[1] + dy, cy[2] + dz};
             bool ok = c[0] >= 0 && c[0] < ln[0] && c[1] >= 0 && c[1] < ln[1];
             if (D > 2) ok = ok && c[2] >= 0 && c[2] < ln[2];
+            ok = ok && g2 + gq[0][u] <= R2;
             lo[u] = 0;
             hi2[u] = 0;
             if (ok) {
