@@ -1616,8 +1616,11 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W) {
   const int j = W.tile_job[tile];
   if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
+  // the job's first slot in a register: JobParams sits in global memory, and the
+  // stores below would otherwise make every entry read it again
+  const int64_t p0 = J.p0;
   const int64_t ts = tile * kTile, slot = ts + threadIdx.x;
-  const bool live = slot - J.p0 < J.n;
+  const bool live = slot - p0 < J.n;
   const int cnt = live ? W.nbc[slot] : 0;
   const int32_t off = cnt > W.K ? W.nbo[slot] : 0;
   // The pair passes' thread order: rank by (list length, slot), so that a wavefront's
@@ -1664,14 +1667,14 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W) {
     fail(kOverItems);
     return;
   }
-  const int64_t tb = J.p0 * W.ips + int64_t(ib);
+  const int64_t tb = p0 * W.ips + int64_t(ib);
   if (threadIdx.x == 0) W.tbase[tile] = tb;
   uint16_t* it0 = W.items + tb + excl;
   auto item_at = [&](int k) -> uint16_t& { return it0[(k >> 3) * 512 + (k & 7)]; };
   uint32_t* hal = W.halo + tile * W.ht;
   const int cap = int(W.ht < int64_t(kMaxHalo) ? W.ht : int64_t(kMaxHalo)) - kTile;
   auto own = [&](uint32_t e) {
-    const int64_t q = nb_slot(e, J.p0);
+    const int64_t q = nb_slot(e, p0);
     return (e >> kJBits) == 0u && q >= ts && q < ts + kTile;
   };
   int P = 1, base = 0;
@@ -1734,7 +1737,7 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W) {
     __syncthreads();
     for_entries(W, slot, cnt, off, [&](int k, uint32_t e) {
       if (own(e)) {
-        if (pass == 0) item_at(k) = uint16_t(nb_slot(e, J.p0) - ts);
+        if (pass == 0) item_at(k) = uint16_t(nb_slot(e, p0) - ts);
         return;
       }
       if (halo_part(e, P) != uint32_t(pass)) return;
