@@ -1552,8 +1552,9 @@ __global__ __launch_bounds__(kTile) void k_nb_out(Work W, PState S, int pass, co
 // ---------------------------------------------------------------------------
 // Neighbour tiles of the pair passes: the halo and item build (list-build steps)
 
-static_assert(kHashN == 8192, "halo_hash yields 13 bits");
-__device__ __forceinline__ uint32_t halo_hash(uint32_t e) { return (e * 0x9E3779B1u) >> 19; }
+static_assert(kHashN >= 1024 && (kHashN & (kHashN - 1)) == 0, "halo_hash yields log2(kHashN) bits");
+constexpr int kHashShift = 32 - __builtin_ctz(unsigned(kHashN));
+__device__ __forceinline__ uint32_t halo_hash(uint32_t e) { return (e * 0x9E3779B1u) >> kHashShift; }
 // the key subset of pass `pass` when a tile's halo takes P passes (P a power of two)
 __device__ __forceinline__ uint32_t halo_part(uint32_t e, int P) { return ((e * 0x85EBCA6Bu) >> 7) & uint32_t(P - 1); }
 
