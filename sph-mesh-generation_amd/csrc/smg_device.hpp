@@ -190,6 +190,7 @@ struct Work {
   // multiple of 4; entries K.. of a longer list in the job's part of the spill pool
   // at spill[nbo[slot] + k - K]
   int32_t* nbl;
+  uint32_t* nbnear;   // near bits of the inline entries: bit k & 31 of word k >> 5 of slot s at [(k >> 5) * cap_total + s]
   int32_t* nbc;     // candidate count per slot (inline + spilled)
   int32_t* nbo;     // spill offset per slot (valid when nbc > K)
   int32_t* spill;   // spill pool, spill_cap entries

@@ -274,6 +274,7 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   const int64_t nb = (n_tbl_ + kScanItems - 1) / kScanItems;
   W_.bsum = alloc_<int32_t>(nb + 1);
   W_.nbl = alloc_<int32_t>(int64_t(K_) * cap_total_);
+  W_.nbnear = alloc_<uint32_t>(int64_t((K_ + 31) / 32) * cap_total_);
   W_.nbc = alloc_<int32_t>(cap_total_);
   W_.nbo = alloc_<int32_t>(cap_total_);
   W_.ovq = alloc_<int32_t>(cap_total_);
@@ -900,7 +901,7 @@ int Engine::step(int64_t nsteps) {
     launch_nb_spill(st_, D_, ntiles_, W_, A_);
     end_();
     begin_("k_halo");
-    launch_halo(st_, D_, ntiles_, W_);
+    launch_halo(st_, D_, ntiles_, W_, A_);
     end_();
     begin_("k_force1");
     launch_force1(st_, D_, ntiles_, W_, A_, par);

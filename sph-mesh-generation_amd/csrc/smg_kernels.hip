@@ -1021,11 +1021,24 @@ __device__ __forceinline__ void image_of(const JobParams& J, uint32_t code, cons
   }
 }
 
+// The FP32 relative position to the image (code) of a slot q: dr_a = A_a + sg_a xh_q,
+// from particle i's hi parts (code 0: A = xh_i, sg = -1, so dr = xh_i - xh_q exactly)
+template <int D>
+__device__ __forceinline__ void image_affine(const JobParams& J, uint32_t code, const float* xh, float* A, float* sg) {
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    const uint32_t m = (code >> (2 * a)) & 3u;
+    A[a] = m == 0 ? xh[a] : (m == 1 ? xh[a] - 2.f * J.wl_h[a] : xh[a] - 2.f * J.wh_h[a]);
+    sg[a] = m == 0 ? -1.f : 1.f;
+  }
+}
+
 // Enumerates the list candidates of one particle in a fixed order (mirror images,
 // then levels, then cells, then slots): every slot (or mirror image) within
 // cut_v * max(h_i, h_q) by the FP32 distance of the hi parts, widened by that
 // distance's error bound -- a superset, so no FP64 decision is needed here.  Hands
-// each entry (k, slot | code << 26) to emit and returns the count.
+// each entry (k, slot | code << 26, near) to emit and returns the count; near: within
+// the cutoff radius itself by that distance (the pair passes take such entries first).
 template <int D, class Emit>
 __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, const JobParams& J, int64_t slot,
                                             Emit emit) {
@@ -1054,6 +1067,7 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
   // the job's constants in registers: JobParams sits in global memory, and the list
   // stores of emit would otherwise make the loops below read them again each time
   const float pf_tol = J.pf_tol;
+  const float nfac = float(J.cut_off / J.cut_v);   // the cutoff radius over the list radius
   const int64_t p0 = J.p0;
   const int nlev = J.nlev;
   const int64_t tbl = J.tbl_base;
@@ -1064,14 +1078,10 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
     image_of<D>(J, code, xi, y);
     // FP32 relative position: dr_a = A_a + s_a * xh_q
     float A[3] = {0.f, 0.f, 0.f}, sg[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-      const uint32_t m = (code >> (2 * a)) & 3u;
-      A[a] = m == 0 ? xh[a] : (m == 1 ? xh[a] - 2.f * J.wl_h[a] : xh[a] - 2.f * J.wh_h[a]);
-      sg[a] = m == 0 ? -1.f : 1.f;
-    }
-    // one candidate: the FP32 distance of the hi parts against the widened radius
-    auto test = [&](const float4& qr) -> bool {
+    image_affine<D>(J, code, xh, A, sg);
+    // one candidate: the FP32 distance of the hi parts against the widened radius (bit
+    // 0), and whether it is within the cutoff radius itself (bit 1: near)
+    auto test = [&](const float4& qr) -> int {
       const float qv[3] = {qr.x, qr.y, qr.z};
       float d2 = 0.f;
 #pragma unroll
@@ -1081,7 +1091,8 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
       }
       const float r = fmaxf(rfi, qr.w * cutf);
       const float rhi = r + (pf_tol + 1.e-6f * r);
-      return d2 <= rhi * rhi;
+      const float rn = r * nfac;
+      return (d2 <= rhi * rhi ? 1 : 0) | (d2 <= rn * rn ? 2 : 0);
     };
     for (int L = 0; L < nlev; L++) {
       const int s = L < li ? L : li;
@@ -1142,8 +1153,9 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
               for (int v = 0; v < 4; v++) qr[v] = W.recb[q + v < hi2[u] ? q + v : hi2[u] - 1];   // == rec[0] at a build
 #pragma unroll
               for (int v = 0; v < 4; v++) {
-                if (q + v < hi2[u] && test(qr[v])) {
-                  emit(cnt, int32_t(nb_pack(q + v, p0, code)));
+                const int tv = q + v < hi2[u] ? test(qr[v]) : 0;
+                if (tv & 1) {
+                  emit(cnt, int32_t(nb_pack(q + v, p0, code)), (tv & 2) != 0);
                   cnt++;
                 }
               }
@@ -1177,9 +1189,11 @@ __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
     const int K = W.K;
     // each group of four inline entries is collected in registers and written with
     // one 16-byte store; a last partial group is written as it stands
+    // the near bits of the inline entries, 32 to a word, likewise
     uint4* nb4 = reinterpret_cast<uint4*>(W.nbl);
     uint4 grp = make_uint4(0u, 0u, 0u, 0u);
-    cnt = nb_enumerate<D>(W, S, J, slot, [&](int k, int32_t e) {
+    uint32_t nw = 0u;
+    cnt = nb_enumerate<D>(W, S, J, slot, [&](int k, int32_t e, bool near) {
       if (k >= K) return;
       const int u = k & 3;
       grp.x = u == 0 ? uint32_t(e) : grp.x;
@@ -1187,8 +1201,17 @@ __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
       grp.z = u == 2 ? uint32_t(e) : grp.z;
       grp.w = u == 3 ? uint32_t(e) : grp.w;
       if (u == 3) nb4[int64_t(k >> 2) * cap + slot] = grp;
+      nw |= near ? 1u << (k & 31) : 0u;
+      if ((k & 31) == 31) {
+        W.nbnear[int64_t(k >> 5) * cap + slot] = nw;
+        nw = 0u;
+      }
     });
     if ((cnt & 3) && cnt < K) nb4[int64_t(cnt >> 2) * cap + slot] = grp;
+    // K is a multiple of four, not of 32: a spilling list can end its inline part
+    // inside a word too
+    const int ni = cnt < K ? cnt : K;
+    if (ni & 31) W.nbnear[int64_t(ni >> 5) * cap + slot] = nw;
     if (cnt > K) {
       const unsigned long long qi = atomicAdd(&W.ctr[0], 1ull);
       W.ovq[qi] = int32_t(slot);   // at most one entry per slot: qi < cap_total
@@ -1220,7 +1243,7 @@ __global__ __launch_bounds__(kTile) void k_nb_spill(Work W, PState S) {
     const int64_t at = J.spill_base + int64_t(off);
     W.nbo[slot] = int32_t(at);
     int32_t* dst = W.spill + at;
-    nb_enumerate<D>(W, S, J, slot, [&](int k, int32_t e) {
+    nb_enumerate<D>(W, S, J, slot, [&](int k, int32_t e, bool) {
       if (k >= K) dst[k - K] = e;
     });
   }
@@ -1598,15 +1621,40 @@ __device__ __forceinline__ void for_entries(const Work& W, int64_t slot, int cnt
   for (int k = ni; k < cnt; k++) fn(k, uint32_t(W.spill[int64_t(off) + (k - W.K)]));
 }
 
+// the same walk with each entry's near bit: k_nb's nbnear words for the inline entries,
+// spill_near(entry) for the spilled ones; fn(k, entry, near)
+template <class Fn, class Near>
+__device__ __forceinline__ void for_entries_near(const Work& W, int64_t slot, int cnt, int32_t off, Fn fn,
+                                                 Near spill_near) {
+  const int ni = cnt < W.K ? cnt : W.K;
+  uint4 vn = ni > 0 ? nb_group(W, slot, 0) : make_uint4(0u, 0u, 0u, 0u);
+  uint32_t nw = 0u;
+  for (int g = 0; 4 * g < ni; g++) {
+    const uint4 v = vn;
+    if (4 * g + 4 < ni) vn = nb_group(W, slot, g + 1);
+    if ((g & 7) == 0) nw = W.nbnear[int64_t(g >> 3) * W.cap_total + slot];
+    const uint32_t ev[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (4 * g + u < ni) fn(4 * g + u, ev[u], ((nw >> ((4 * g + u) & 31)) & 1u) != 0u);
+  }
+  for (int k = ni; k < cnt; k++) {
+    const uint32_t e = uint32_t(W.spill[int64_t(off) + (k - W.K)]);
+    fn(k, e, spill_near(e));
+  }
+}
+
 // One workgroup per tile of a rebuilding job.  The tile's halo: its own slots (halo
 // index = slot - tile start), then every other (slot, mirror code) its candidate lists
 // name, each once, deduplicated in an LDS hash table; the candidate lists become items
-// (halo indices) in list order.  A halo too large for one table is built in P passes,
-// each taking the keys of one hash class.  The halo order is that of the table (it may
-// differ between runs); a particle's items keep the list order, and a halo index only
-// says where the record sits, so every pair sum is formed in the same order each run.
+// (halo indices): the entries within the cutoff at the build positions first, in list
+// order, then the rest in reverse list order.  A halo too large for one table is built
+// in P passes, each taking the keys of one hash class.  The halo order is that of the
+// table (it may differ between runs); a particle's item order depends only on its list
+// and the build positions, and a halo index only says where the record sits, so every
+// pair sum is formed in the same order each run.
 template <int D>
-__global__ __launch_bounds__(kTile) void k_halo(Work W) {
+__global__ __launch_bounds__(kTile) void k_halo(Work W, PState S) {
   __shared__ uint32_t hk[kHashN];
   __shared__ uint16_t hv[kHashN];
   __shared__ int s_scan[4];
@@ -1678,6 +1726,30 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W) {
     const int64_t q = nb_slot(e, p0);
     return (e >> kJBits) == 0u && q >= ts && q < ts + kTile;
   };
+  // k_nb's near test for the spilled entries (its near bits cover the inline ones):
+  // the same FP32 operations on the same records, so a list gets the same near flags
+  // whatever part of it is spilled
+  float xb[3] = {0.f, 0.f, 0.f}, rfi = 0.f;
+  {
+    const float4 rb = W.rec[4 * slot + 0];
+    xb[0] = rb.x; xb[1] = rb.y; xb[2] = rb.z;
+    if (live) rfi = float(S.h[slot] * J.cut_v);
+  }
+  const float cutv = float(J.cut_v), nfac = float(J.cut_off / J.cut_v);
+  auto spill_near = [&](uint32_t e) {
+    float A[3] = {0.f, 0.f, 0.f}, sg[3] = {0.f, 0.f, 0.f};
+    image_affine<D>(J, e >> kJBits, xb, A, sg);
+    const float4 qr = W.recb[nb_slot(e, p0)];
+    const float qv[3] = {qr.x, qr.y, qr.z};
+    float d2 = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      const float t = A[a] + sg[a] * qv[a];
+      d2 += t * t;
+    }
+    const float rn = fmaxf(rfi, qr.w * cutv) * nfac;
+    return d2 <= rn * rn;
+  };
   int P = 1, base = 0;
   for (int pass = 0; pass < P;) {
     for (int s = threadIdx.x; s < kHashN; s += kTile) hk[s] = 0xFFFFFFFFu;
@@ -1736,7 +1808,12 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W) {
       }
     }
     __syncthreads();
-    for_entries(W, slot, cnt, off, [&](int k, uint32_t e) {
+    // Items near-first: the entries k_nb found within the cutoff radius at the build
+    // positions from the front, the others from the back.  Every pass places every
+    // entry, so each entry gets the same position whatever pass writes it.
+    int kn = 0, kf = 0;
+    for_entries_near(W, slot, cnt, off, [&](int, uint32_t e, bool near) {
+      const int k = near ? kn++ : cnt - 1 - kf++;
       if (own(e)) {
         if (pass == 0) item_at(k) = uint16_t(nb_slot(e, p0) - ts);
         return;
@@ -1745,7 +1822,7 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W) {
       uint32_t s = halo_hash(e);
       while (hk[s] != e) s = (s + 1u) & uint32_t(kHashN - 1);
       item_at(k) = hv[s];
-    });
+    }, spill_near);
     base += U;
     pass++;
     __syncthreads();
@@ -1998,7 +2075,7 @@ __device__ __forceinline__ bool pair1(const Work& W, const JobParams& J, const O
 
 // One workgroup per tile; thread t sums for the tile's particle of rank t in the order
 // of list lengths (so each wavefront walks lists of about the same length), over its
-// items in list order, then the pairs with this step's jumpers.  Records come from the
+// items in item order, then the pairs with this step's jumpers.  Records come from the
 // tile's halo, staged in LDS (the first kHL1 entries; the rest read from the records).
 template <int D>
 __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void k_force1(Work W, PState S, int par) {
@@ -2084,10 +2161,15 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void
           nb = valid && nb && !(tw & int(kQJump));   // a jumper: paired through the jumper scan
           const int tq = tw & int(kQType);
           if (TS && nb && o.surf && tq == SMG_SURFACE && q < 0) halo_entry(W, tile, ts, J.p0, e, q, code);
-          const bool f = pair1<D, TS>(W, J, o, dr, d2, qh.w, tq, e == uint32_t(p), q, acc, nb);
-          if (f && !(tw & int(kQMir))) {
-            st2[(c2 >> 3) * 512 + (c2 & 7)] = uint16_t(e);
-            c2++;
+          // An item that is no lane's neighbour adds nothing (pair1 with valid = false
+          // leaves the sums as they are): the wavefront skips the pair arithmetic.  Items
+          // are near-first, so past the near parts of its lists most items are skipped.
+          if (__ballot(nb) != 0ull) {
+            const bool f = pair1<D, TS>(W, J, o, dr, d2, qh.w, tq, e == uint32_t(p), q, acc, nb);
+            if (f && !(tw & int(kQMir))) {
+              st2[(c2 >> 3) * 512 + (c2 & 7)] = uint16_t(e);
+              c2++;
+            }
           }
         };
         // The walk runs to the wavefront's longest list, so its loop and group loads are
@@ -2649,9 +2731,9 @@ void launch_nb_spill(hipStream_t st, int D, int64_t ntiles, const Work& W, const
   if (D == 2) k_nb_spill<2><<<g, kTile, 0, st>>>(W, S);
   else k_nb_spill<3><<<g, kTile, 0, st>>>(W, S);
 }
-void launch_halo(hipStream_t st, int D, int64_t ntiles, const Work& W) {
-  if (D == 2) k_halo<2><<<ntiles, kTile, 0, st>>>(W);
-  else k_halo<3><<<ntiles, kTile, 0, st>>>(W);
+void launch_halo(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S) {
+  if (D == 2) k_halo<2><<<ntiles, kTile, 0, st>>>(W, S);
+  else k_halo<3><<<ntiles, kTile, 0, st>>>(W, S);
 }
 void launch_force1(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int par) {
   if (D == 2) k_force1<2><<<ntiles, kTile, 0, st>>>(W, S, par);

@@ -25,7 +25,7 @@ void launch_segsort(hipStream_t st, int64_t ntiles, const Work& W, const PState&
 void launch_gather(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& A, const PState& B);
 void launch_nb(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S);
 void launch_nb_spill(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S);
-void launch_halo(hipStream_t st, int D, int64_t ntiles, const Work& W);
+void launch_halo(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S);
 void launch_force1(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int par);
 void launch_force2(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int par);
 void launch_residual(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int njobs, int par);

@@ -40,11 +40,12 @@ Sizes sizes_for(int K, int sps) {
   Sizes z;
   const int64_t state = 9 * 4 + 8 + 4 + 4;   // xh, xl, nrm (3 each), h, type, id
   // state (A, B), nu_a / nu_b, a1 / a2, sum, rec, key / skey / perm / sperm / sid,
-  // inline lists, nbc / nbo / ovq / ntrue, recb, spill pool, items and stage-2 items
+  // inline lists and their near bits, nbc / nbo / ovq / ntrue, recb, spill pool, items and stage-2 items
   // (2 B each), halo (ht per 256 slots), ioff, nbc2, perm, and the per-tile words
   const int64_t ips = int64_t(K) + sps;
   const int64_t ht = std::min<int64_t>(smg::kMaxHalo, int64_t(smg::kTile) * std::max<int64_t>(16, ips / 8));
-  z.per_slot = 2 * state + 2 * 4 + 6 * 4 + 4 + 4 * 16 + 5 * 4 + int64_t(K) * 4 + 4 * 4 + 16 + int64_t(sps) * 4 +
+  z.per_slot = 2 * state + 2 * 4 + 6 * 4 + 4 + 4 * 16 + 5 * 4 + int64_t(K) * 4 + int64_t((K + 31) / 32) * 4 + 4 * 4 +
+               16 + int64_t(sps) * 4 +
                ips * 4 + ht / smg::kTile * 4 + 4 + 4 + 1 + 1 + 1;
   z.per_table = 2 * 4 + 1;
   z.per_cell = smg::kLsCellBytes;
