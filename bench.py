@@ -152,6 +152,20 @@ def relax(args, smg):
         if t:
             traffic = t["traffic_bytes"]
             traffic_src = "profiles/pmc_traffic_relax3d_1M.json (FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated)"
+    # the dominant kernel against the VALU issue rate: its wave64 VALU instructions per
+    # launch from the committed rocprofv3 SQ_INSTS_VALU summary of this workload, over
+    # its launch time measured here
+    issue = None
+    sqf = os.path.join(ROOT, "profiles", "sq_counters_relax3d_1M.json")
+    if os.path.exists(sqf) and D == 3:
+        with open(sqf) as f:
+            sq = json.load(f)
+        k = sq.get("kernels", {}).get(dom)
+        if k and dom not in REBUILD_ONLY:
+            rate = k["valu_instr_per_dispatch"] / (avg_ms * 1e-3)
+            issue = {"bound": "valu-issue", "kernel": dom, "achieved": rate, "peak": sq["valu_issue_peak"],
+                     "unit": "wave64 VALU instructions/s", "frac": rate / sq["valu_issue_peak"],
+                     "source": "profiles/sq_counters_relax3d_1M.json (SQ_INSTS_VALU per dispatch)"}
     out = {
         "metric": METRIC,
         "value": value,
@@ -185,6 +199,7 @@ def relax(args, smg):
             "kernel_avg_ms": avg_ms,
             "algorithmic_bytes_per_launch": bpl,
         },
+        "issue_roofline": issue,
         "step_roofline": {
             "bytes_per_particle_step": STEP_BYTES[0 if D == 3 else 1],
             "achieved_GBs": STEP_BYTES[0 if D == 3 else 1] * value / 1e9,
