@@ -343,8 +343,10 @@ typedef struct smg_tri_stats {
   int32_t reserved;
 } smg_tri_stats;
 
-/* Triangulates a 2D job's current particles.  tris: 3 particle ids per triangle
- * (i > j > k), may be NULL to query the count; capacity in triangles. */
+/* Triangulates a 2D job's current particles, after smg_prerun or any step (between
+ * smg_prerun and the first step the candidate lists are built on the placed positions
+ * and nothing moves: the post mode).  tris: 3 particle ids per triangle (i > j > k),
+ * may be NULL to query the count; capacity in triangles. */
 int smg_triangulate(smg_job* job, int32_t* tris, int64_t capacity, int64_t* n_tri, smg_tri_stats* stats);
 /* The Voronoi-stage edges (particle ids, i > j, sorted) of the last smg_triangulate. */
 int smg_tri_edges(smg_job* job, int32_t* edges, int64_t capacity, int64_t* n_edges);

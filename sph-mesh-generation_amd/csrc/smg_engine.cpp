@@ -858,6 +858,21 @@ int Engine::prerun() {
   return SMG_OK;
 }
 
+// The candidate lists on the positions Pre_run left, without a step: its sort left the
+// cell tables, the build records and every job's rebuild flag (k_decide with force), so
+// the list passes alone run.  The 2D mesh stage reads these lists; a step would move
+// the particles (its mapping pass) before they are meshed.
+int Engine::build_lists() {
+  if (!prerun_done_ || step_ != 0) { set_error("build_lists: only between prerun and the first step"); return SMG_ERR_STATE; }
+  if (ntiles_ == 0) return SMG_OK;
+  HIP_TRY(hipMemsetAsync(W_.ctr, 0, sizeof(unsigned long long), st_));   // the spill queue
+  launch_nb(st_, D_, ntiles_, W_, A_);
+  launch_nb_spill(st_, D_, ntiles_, W_, A_);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st_));
+  return check_health();
+}
+
 int Engine::step(int64_t nsteps) {
   if (!prerun_done_) { set_error("step: call prerun (or seed) first"); return SMG_ERR_STATE; }
   if (ntiles_ == 0) return SMG_OK;   // an empty pool

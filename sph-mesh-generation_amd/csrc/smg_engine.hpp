@@ -91,6 +91,7 @@ class Engine {
   int init_positions(int job, const double* x, const int32_t* type);  // upload one job's positions
   int init_finish();                                            // Set_particle_info + first sort
   int prerun();                                                 // Pre_run
+  int build_lists();   // candidate lists on the current positions of a job that has not stepped
   int step(int64_t nsteps);                                     // SPH_run_mesh_generation x n
   // the same steps replayed from a captured graph of G steps (G even, nsteps a multiple
   // of G, even step index), captured again when G or the launch extent changed; any

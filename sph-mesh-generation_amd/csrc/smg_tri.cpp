@@ -146,7 +146,12 @@ int Engine::triangulate(int job, std::vector<std::array<int32_t, 3>>& tris, smg_
   std::memset(st, 0, sizeof(*st));
   tris.clear();
   if (D_ != 2) { set_error("triangulate: 2D only (the reference meshes 3D with TetGen)"); return SMG_ERR_ARG; }
-  if (step_ == 0) { set_error("triangulate: take a step first (the lists come from a step)"); return SMG_ERR_STATE; }
+  if (step_ == 0) {
+    // after Pre_run and before any step (the post mode): lists on the placed positions
+    if (!prerun_done_) { set_error("triangulate: call prerun (or seed and step) first"); return SMG_ERR_STATE; }
+    const int rb = build_lists();
+    if (rb != SMG_OK) return rb;
+  }
   const JobParams& J = jp_[job];
   const int64_t n = J.n, cap = J.cap;
   const int KE = 32;

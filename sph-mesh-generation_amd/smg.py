@@ -1089,10 +1089,11 @@ def read_vtu(path):
 def post_process(dir, cfg, lset, outdir=None):
     """SPH_mesh_generation::Load_data_for_post (sph_mesh_generation.cpp:426-590): for every
     step in dir/list.csv, the particles of all its rank files are placed on the level set
-    (Get_particle_info, Set_particle_scale and Find_and_map_particle, here smg_prerun,
-    whose force passes move nothing) and meshed: triangles in 2D (where the reference's
-    post mode leaves the mesh empty), tetrahedra after the deletion rule in 3D; with outdir, each step's mesh goes to a mesh VTU (output n_post,
-    time = the step).  Returns [(step, cells)]."""
+    (Get_particle_info, Set_particle_scale and Find_and_map_particle, here smg_prerun)
+    and meshed at the placed positions: triangles in 2D (where the reference's post mode
+    leaves the mesh empty; the lists are built on the placed positions, no step is
+    taken), tetrahedra after the deletion rule in 3D; with outdir, each step's mesh goes
+    to a mesh VTU (output n_post, time = the step).  Returns [(step, cells)]."""
     ranks, steps = read_post_list(dir)
     out = []
     for n_post, (step, count) in enumerate(steps):
@@ -1104,9 +1105,6 @@ def post_process(dir, cfg, lset, outdir=None):
             job.init_particles(x[:, :cfg.dim], np.zeros(len(x), np.int32))
             job.prerun()
             if cfg.dim == 2:
-                # the 2D mesh stage reads the neighbour lists of a step; the first step
-                # after Pre_run moves nothing (its velocities are zero) and builds them
-                job.step(1)
                 cells = triangulate(job)[0]
             else:
                 cells = tetrahedralize(job)[0]
