@@ -651,6 +651,9 @@ int Engine::admit(const JobInput& in, int* job) {
 int Engine::sort_job_(int j) {
   const JobParams& J = jp_[j];
   const int64_t t0 = J.p0 / kTile, nt = J.cap / kTile;
+  // the rebuild-only passes' any-job gate (the next step's k_reset clears it)
+  static const unsigned long long kOne = 1ull;
+  HIP_TRY(hipMemcpyAsync(W_.ctr + 1, &kOne, sizeof(kOne), hipMemcpyHostToDevice, st_));
   Work Ws = W_;
   Ws.tile0 = t0;
   launch_key(st_, D_, nt, Ws, A_);

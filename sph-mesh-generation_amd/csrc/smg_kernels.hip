@@ -96,6 +96,12 @@ __device__ __forceinline__ int64_t tile_id(const Work& W) {
   return W.tile0 + x * q + (x < r ? x : r) + i;
 }
 
+// Whether any job rebuilds its lists at this step (W.ctr[1]: set by k_decide, by the
+// host before a single job's sort, cleared by k_reset).  The rebuild-only passes test
+// it first, one scalar load for every workgroup, and return without touching their
+// tiles on the steps where no job rebuilds.
+__device__ __forceinline__ bool rebuild_any(const Work& W) { return W.ctr[1] != 0ull; }
+
 // neighbour entry k of a slot (spill offset off is read only when the list spills)
 __device__ __forceinline__ int64_t nbl_index(const Work& W, int64_t slot, int k) {
   return (int64_t(k >> 2) * W.cap_total + slot) * 4 + (k & 3);
@@ -819,6 +825,7 @@ __global__ __launch_bounds__(64) void k_decide(Work W, int njobs, int force) {
 // cell key and count of every slot of a rebuilding job
 template <int D>
 __global__ __launch_bounds__(kTile) void k_key(Work W, PState S) {
+  if (!rebuild_any(W)) return;
   const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
@@ -930,6 +937,7 @@ __global__ __launch_bounds__(kTile) void k_scan_down(const int32_t* __restrict__
 // A job's cell starts are taken relative to the start of its table segment, so the
 // slot ranges of the jobs may sit anywhere (gaps included) and still sort in place.
 __global__ __launch_bounds__(kTile) void k_scatter(Work W, PState S) {
+  if (!rebuild_any(W)) return;
   const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
@@ -945,6 +953,7 @@ __global__ __launch_bounds__(kTile) void k_scatter(Work W, PState S) {
 // atomics took), a rank sort: slot d of cell [c0, c1) goes to c0 + #{a : sid[a] <
 // sid[d]}; the ids of a job are distinct.  Dead slots (the pad cell) keep their place.
 __global__ __launch_bounds__(kTile) void k_segsort(Work W) {
+  if (!rebuild_any(W)) return;
   const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
@@ -966,6 +975,7 @@ __global__ __launch_bounds__(kTile) void k_segsort(Work W) {
 // current state at every step whichever jobs re-sorted
 template <int D>
 __global__ __launch_bounds__(kTile) void k_gather(Work W, PState A, PState B) {
+  if (!rebuild_any(W)) return;
   const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
   const int64_t d = tile_id(W) * kTile + threadIdx.x;
@@ -984,6 +994,7 @@ __global__ __launch_bounds__(kTile) void k_gather(Work W, PState A, PState B) {
 
 template <int D>
 __global__ __launch_bounds__(kTile) void k_copyback(Work W, PState A, PState B) {
+  if (!rebuild_any(W)) return;
   const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
   const int64_t d = tile_id(W) * kTile + threadIdx.x;
@@ -1178,6 +1189,7 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
 // more joins the spill queue (k_copyback left the build positions and h in recb)
 template <int D>
 __global__ __launch_bounds__(kTile) void k_nb(Work W, PState S) {
+  if (!rebuild_any(W)) return;
   const int j = W.tile_job[tile_id(W)];
   if (j < 0 || !W.red[j].rebuild) return;
   const JobParams& J = W.jp[j];
@@ -1661,6 +1673,7 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W, PState S) {
   __shared__ int s_len[kTile];
   __shared__ int s_cnt, s_over;
   __shared__ unsigned long long s_ibase;
+  if (!rebuild_any(W)) return;
   const int64_t tile = tile_id(W);
   const int j = W.tile_job[tile];
   if (j < 0 || !W.red[j].rebuild) return;
