@@ -28,7 +28,8 @@ constexpr int kScanItems = 4096; // table entries per scan workgroup (256 thread
 // candidate lists are rewritten as items: 16-bit halo indices, one run per slot, all the
 // tile's runs back to back.  The pair passes stage the halo's records in LDS once per
 // tile and walk the items with the work split evenly over the workgroup's threads.
-constexpr int kHashN = 4096;     // LDS hash slots of the halo build (keys of one pass)
+constexpr int kHashN = 4096;     // LDS hash slots of the halo build (keys of one pass) ...
+constexpr int kHashSmallIps = 256;   // ... for up to this many pair items per slot; 2 kHashN above
 constexpr int kHL1 = 1536;       // halo records the stage-1 pass stages in LDS (24 B each)
 constexpr int kHL2 = 1280;       // ... and the stage-2 pass (40 B each)
 constexpr int kMaxHalo = 65536;  // halo entries of a tile (16-bit items)

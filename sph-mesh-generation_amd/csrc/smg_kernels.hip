@@ -1587,9 +1587,12 @@ __global__ __launch_bounds__(kTile) void k_nb_out(Work W, PState S, int pass, co
 // ---------------------------------------------------------------------------
 // Neighbour tiles of the pair passes: the halo and item build (list-build steps)
 
-static_assert(kHashN >= 1024 && (kHashN & (kHashN - 1)) == 0, "halo_hash yields log2(kHashN) bits");
-constexpr int kHashShift = 32 - __builtin_ctz(unsigned(kHashN));
-__device__ __forceinline__ uint32_t halo_hash(uint32_t e) { return (e * 0x9E3779B1u) >> kHashShift; }
+// the hash of a halo key into a table of HN slots (HN a power of two)
+template <int HN>
+__device__ __forceinline__ uint32_t halo_hash(uint32_t e) {
+  static_assert(HN >= 1024 && (HN & (HN - 1)) == 0, "halo_hash yields log2(HN) bits");
+  return (e * 0x9E3779B1u) >> (32 - __builtin_ctz(unsigned(HN)));
+}
 // the key subset of pass `pass` when a tile's halo takes P passes (P a power of two)
 __device__ __forceinline__ uint32_t halo_part(uint32_t e, int P) { return ((e * 0x85EBCA6Bu) >> 7) & uint32_t(P - 1); }
 
@@ -1665,10 +1668,10 @@ __device__ __forceinline__ void for_entries_near(const Work& W, int64_t slot, in
 // table (it may differ between runs); a particle's item order depends only on its list
 // and the build positions, and a halo index only says where the record sits, so every
 // pair sum is formed in the same order each run.
-template <int D>
+template <int D, int HN>
 __global__ __launch_bounds__(kTile) void k_halo(Work W, PState S) {
-  __shared__ uint32_t hk[kHashN];
-  __shared__ uint16_t hv[kHashN];
+  __shared__ uint32_t hk[HN];
+  __shared__ uint16_t hv[HN];
   __shared__ int s_scan[4];
   __shared__ int s_len[kTile];
   __shared__ int s_cnt, s_over;
@@ -1765,7 +1768,7 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W, PState S) {
   };
   int P = 1, base = 0;
   for (int pass = 0; pass < P;) {
-    for (int s = threadIdx.x; s < kHashN; s += kTile) hk[s] = 0xFFFFFFFFu;
+    for (int s = threadIdx.x; s < HN; s += kTile) hk[s] = 0xFFFFFFFFu;
     if (threadIdx.x == 0) {
       s_cnt = 0;
       s_over = 0;
@@ -1776,15 +1779,15 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W, PState S) {
     bool full = false;
     for_entries(W, slot, cnt, off, [&](int, uint32_t e) {
       if (full || own(e) || halo_part(e, P) != uint32_t(pass)) return;
-      uint32_t s = halo_hash(e);
+      uint32_t s = halo_hash<HN>(e);
       while (true) {
         const uint32_t old = atomicCAS(&hk[s], 0xFFFFFFFFu, e);
         if (old == 0xFFFFFFFFu) {
-          full = atomicAdd(&s_cnt, 1) >= kHashN / 4 * 3;
+          full = atomicAdd(&s_cnt, 1) >= HN / 4 * 3;
           break;
         }
         if (old == e) break;
-        s = (s + 1u) & uint32_t(kHashN - 1);
+        s = (s + 1u) & uint32_t(HN - 1);
       }
     });
     if (full) s_over = 1;
@@ -1801,7 +1804,7 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W, PState S) {
       continue;
     }
     // halo indices for the occupied table slots (thread t takes slots t, t + 256, ...)
-    constexpr int kPer = kHashN / kTile;
+    constexpr int kPer = HN / kTile;
     int c = 0;
 #pragma unroll 8
     for (int u = 0; u < kPer; u++) c += hk[u * kTile + threadIdx.x] != 0xFFFFFFFFu ? 1 : 0;
@@ -1832,8 +1835,8 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W, PState S) {
         return;
       }
       if (halo_part(e, P) != uint32_t(pass)) return;
-      uint32_t s = halo_hash(e);
-      while (hk[s] != e) s = (s + 1u) & uint32_t(kHashN - 1);
+      uint32_t s = halo_hash<HN>(e);
+      while (hk[s] != e) s = (s + 1u) & uint32_t(HN - 1);
       item_at(k) = hv[s];
     }, spill_near);
     base += U;
@@ -2745,8 +2748,16 @@ void launch_nb_spill(hipStream_t st, int D, int64_t ntiles, const Work& W, const
   else k_nb_spill<3><<<g, kTile, 0, st>>>(W, S);
 }
 void launch_halo(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S) {
-  if (D == 2) k_halo<2><<<ntiles, kTile, 0, st>>>(W, S);
-  else k_halo<3><<<ntiles, kTile, 0, st>>>(W, S);
+  // Lists of up to 256 items per slot (inline + spill) keep a tile's halo within one
+  // pass of a 4,096-slot table (24 KB, 5 workgroups per CU); longer lists take the
+  // 8,192-slot table (48 KB, 3 per CU), so that their halos take half the passes.
+  if (W.ips <= kHashSmallIps) {
+    if (D == 2) k_halo<2, kHashN><<<ntiles, kTile, 0, st>>>(W, S);
+    else k_halo<3, kHashN><<<ntiles, kTile, 0, st>>>(W, S);
+  } else {
+    if (D == 2) k_halo<2, 2 * kHashN><<<ntiles, kTile, 0, st>>>(W, S);
+    else k_halo<3, 2 * kHashN><<<ntiles, kTile, 0, st>>>(W, S);
+  }
 }
 void launch_force1(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int par) {
   if (D == 2) k_force1<2><<<ntiles, kTile, 0, st>>>(W, S, par);
