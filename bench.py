@@ -360,9 +360,9 @@ def c3(args, smg):
     t0 = time.perf_counter()
     cfg, sdf = smg.config_c3()
     # the seeded start is uniform in space, so the particles of the coarse interior
-    # (h 8x the surface's) start with lists of up to ~1200 entries: 512 inline entries
+    # (h 8x the surface's) start with lists of up to ~1200 entries: 1024 inline entries
     # per particle (walked in groups of four) and a spill pool for the rest
-    cfg.max_neighbors = 512
+    cfg.max_neighbors = 1024
     cfg.spill_per_slot = 256
     base = smg.make_levelset(cfg, sdf)
     lv = smg.make_levelset_levels(cfg, sdf, 2)
