@@ -161,7 +161,12 @@ def relax(args, smg):
         with open(sqf) as f:
             sq = json.load(f)
         k = sq.get("kernels", {}).get(dom)
-        if k and dom not in REBUILD_ONLY:
+        pr = sq.get("profiled_run", {})
+        # the instruction count holds for the profiled list shape and step window only
+        kin = cfg.max_neighbors if cfg.max_neighbors > 0 else 128   # 0: the library's 128
+        same = (pr.get("skin") == cfg.skin and pr.get("max_neighbors") == kin
+                and pr.get("steps") == steps and pr.get("warmup") == args.warmup)
+        if k and dom not in REBUILD_ONLY and same:
             rate = k["valu_instr_per_dispatch"] / (avg_ms * 1e-3)
             issue = {"bound": "valu-issue", "kernel": dom, "achieved": rate, "peak": sq["valu_issue_peak"],
                      "unit": "wave64 VALU instructions/s", "frac": rate / sq["valu_issue_peak"],
