@@ -145,11 +145,19 @@ def relax(args, smg):
     # HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc
     # FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated), if present
     traffic, traffic_src = None, None
+    kin = cfg.max_neighbors if cfg.max_neighbors > 0 else 128   # 0: the library's 128
+
+    def profiled_like_this(pr):
+        # a committed per-launch count holds for the profiled list shape and step window only
+        return (pr.get("skin") == cfg.skin and pr.get("max_neighbors") == kin
+                and pr.get("steps") == steps and pr.get("warmup") == args.warmup)
+
     pmc = os.path.join(ROOT, "profiles", "pmc_traffic_relax3d_1M.json")
     if os.path.exists(pmc) and D == 3:
         with open(pmc) as f:
-            t = json.load(f).get("kernels", {}).get(dom)
-        if t:
+            pj = json.load(f)
+        t = pj.get("kernels", {}).get(dom)
+        if t and profiled_like_this(pj.get("profiled_run", {})):
             traffic = t["traffic_bytes"]
             traffic_src = "profiles/pmc_traffic_relax3d_1M.json (FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated)"
     # the dominant kernel against the VALU issue rate: its wave64 VALU instructions per
@@ -161,12 +169,7 @@ def relax(args, smg):
         with open(sqf) as f:
             sq = json.load(f)
         k = sq.get("kernels", {}).get(dom)
-        pr = sq.get("profiled_run", {})
-        # the instruction count holds for the profiled list shape and step window only
-        kin = cfg.max_neighbors if cfg.max_neighbors > 0 else 128   # 0: the library's 128
-        same = (pr.get("skin") == cfg.skin and pr.get("max_neighbors") == kin
-                and pr.get("steps") == steps and pr.get("warmup") == args.warmup)
-        if k and dom not in REBUILD_ONLY and same:
+        if k and dom not in REBUILD_ONLY and profiled_like_this(sq.get("profiled_run", {})):
             rate = k["valu_instr_per_dispatch"] / (avg_ms * 1e-3)
             issue = {"bound": "valu-issue", "kernel": dom, "achieved": rate, "peak": sq["valu_issue_peak"],
                      "unit": "wave64 VALU instructions/s", "frac": rate / sq["valu_issue_peak"],
