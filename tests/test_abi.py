@@ -50,6 +50,27 @@ def test_step_without_levelset_is_a_state_error(smg_mod):
     smg_mod.lib().smg_destroy(h)
 
 
+def test_empty_and_negative_particle_counts_are_rejected(smg_mod, c1):
+    """A job of no particles is an argument error at seeding and at init_particles, with
+    the level set in place and before any device work; stepping, meshing and reading
+    such a job are state errors."""
+    cfg, lset = c1
+    L = smg_mod.lib()
+    h = C.c_void_p()
+    assert L.smg_create(C.byref(cfg), C.byref(h)) == 0
+    assert L.smg_set_levelset(h, smg_mod._ptr(lset["phi"]), smg_mod._ptr(lset["normal"]), smg_mod._ptr(lset["tag"])) == 0
+    for n in (0, -3):
+        assert L.smg_seed_particles(h, n) == -1
+        assert b"bad argument" in L.smg_last_error()
+        x = np.zeros((1, 3))
+        assert L.smg_init_particles(h, n, smg_mod._ptr(x), None) == -1
+    assert L.smg_step(h, 1) == -4
+    m = C.c_int64()
+    st = smg_mod.SmgTriStats()
+    assert L.smg_triangulate(h, None, 0, C.byref(m), C.byref(st)) == -4
+    L.smg_destroy(h)
+
+
 def test_no_cpu_fallback_without_gpu(smg_mod, c1):
     import subprocess
     import sys
