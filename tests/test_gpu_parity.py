@@ -311,3 +311,15 @@ def test_tiny_jobs(smg_mod, orc_stored, c1, case, n):
         if scale > 0:
             assert np.abs(t[key] - d[key]).max() <= 2e-6 * scale, key
     job.close()
+
+
+def test_job_size_limit_is_a_capacity_error(smg_mod, c1):
+    """A job holds at most 2^26 particles (list entries name slots relative to the job's
+    first slot in 26 bits): one tile more is a capacity error at set-up, before any
+    buffer is allocated."""
+    cfg, lset = c1
+    job = smg_mod.Job(cfg, lset)
+    with pytest.raises(smg_mod.SmgError) as e:
+        job.seed((1 << 26) + 1)
+    assert e.value.rc == -3 and "2^26" in str(e.value)
+    job.close()
