@@ -28,6 +28,7 @@ constexpr int kScanItems = 4096; // table entries per scan workgroup (256 thread
 // candidate lists are rewritten as items: 16-bit halo indices, one run per slot, all the
 // tile's runs back to back.  The pair passes stage the halo's records in LDS once per
 // tile and walk the items with the work split evenly over the workgroup's threads.
+constexpr int kRecbPad = 4;      // records past the last slot of recb (the list build's 4-wide loads)
 constexpr int kHashN = 4096;     // LDS hash slots of the halo build (keys of one pass) ...
 constexpr int kHashSmallIps = 256;   // ... for up to this many pair items per slot; 2 kHashN above
 constexpr int kHL1 = 1536;       // halo records the stage-1 pass stages in LDS (24 B each)

@@ -279,7 +279,7 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   W_.nbo = alloc_<int32_t>(cap_total_);
   W_.ovq = alloc_<int32_t>(cap_total_);
   W_.ntrue = alloc_<int32_t>(cap_total_);
-  W_.recb = alloc_<float4>(cap_total_);
+  W_.recb = alloc_<float4>(cap_total_ + kRecbPad);   // the list build reads up to 3 records past a range
   W_.jset = alloc_<int32_t>(int64_t(njobs) * kJumpMax);
   W_.jl = alloc_<int32_t>(int64_t(njobs) * kJumpMax * kJumpList);
   W_.jcnt = alloc_<int32_t>(int64_t(njobs) * kJumpMax);
@@ -331,7 +331,7 @@ int Engine::alloc_buffers_(int64_t slots, int64_t tbl, int64_t ls, int njobs, in
   HIP_TRY(hipMemsetAsync(W_.ctr, 0, 2 * sizeof(unsigned long long), st_));
   HIP_TRY(hipMemsetAsync(W_.nbo, 0, sizeof(int32_t) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.ntrue, 0, sizeof(int32_t) * cap_total_, st_));
-  HIP_TRY(hipMemsetAsync(W_.recb, 0, sizeof(float4) * cap_total_, st_));
+  HIP_TRY(hipMemsetAsync(W_.recb, 0, sizeof(float4) * (cap_total_ + kRecbPad), st_));
   HIP_TRY(hipMemsetAsync(W_.hcnt, 0, sizeof(int32_t) * std::max<int64_t>(ntl, 1), st_));
   HIP_TRY(hipMemsetAsync(W_.ioff, 0, sizeof(int32_t) * cap_total_, st_));
   HIP_TRY(hipMemsetAsync(W_.nbc2, 0, sizeof(int32_t) * cap_total_, st_));

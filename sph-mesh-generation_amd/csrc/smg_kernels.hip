@@ -1157,11 +1157,13 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
 #pragma unroll
           for (int u = 0; u < 3; u++) {
             // candidates four at a time: the four records are loaded before any is
-            // tested (an index past the range repeats the range's last slot)
+            // tested, from one address (recb has kRecbPad records past its last slot,
+            // so the loads past the range stay inside it; their tests are masked)
             for (int32_t q = lo[u]; q < hi2[u]; q += 4) {
               float4 qr[4];
+              const float4* rq = W.recb + uint32_t(q);   // == rec[0] at a build
 #pragma unroll
-              for (int v = 0; v < 4; v++) qr[v] = W.recb[q + v < hi2[u] ? q + v : hi2[u] - 1];   // == rec[0] at a build
+              for (int v = 0; v < 4; v++) qr[v] = rq[v];
 #pragma unroll
               for (int v = 0; v < 4; v++) {
                 const int tv = q + v < hi2[u] ? test(qr[v]) : 0;
