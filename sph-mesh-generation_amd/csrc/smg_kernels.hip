@@ -596,8 +596,10 @@ __global__ __launch_bounds__(kTile) void k_seed(Work W, PState S) {
 // ---------------------------------------------------------------------------
 // P1: integrate, sample, map, scale, key
 
-template <int D>
-__global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int mode, int par) {
+// mode (kModeInit, kModePrerun, kModeStep) is a template parameter: each launch
+// compiles only its own path (the step pass 114 VGPRs instead of 124)
+template <int D, int mode>
+__global__ __launch_bounds__(kTile) void k_map(Work W, PState S, int par) {
   __shared__ double lds_d[4];
   __shared__ int lds_i[4];
   __shared__ unsigned long long lds_u[4];
@@ -2717,8 +2719,9 @@ void launch_seed(hipStream_t st, int D, int64_t ntiles, const Work& W, const PSt
   else k_seed<3><<<ntiles, kTile, 0, st>>>(W, S);
 }
 void launch_map(hipStream_t st, int D, int64_t ntiles, const Work& W, const PState& S, int mode, int par) {
-  if (D == 2) k_map<2><<<ntiles, kTile, 0, st>>>(W, S, mode, par);
-  else k_map<3><<<ntiles, kTile, 0, st>>>(W, S, mode, par);
+  if (mode == kModeStep) { if (D == 2) k_map<2, kModeStep><<<ntiles, kTile, 0, st>>>(W, S, par); else k_map<3, kModeStep><<<ntiles, kTile, 0, st>>>(W, S, par); }
+  else if (mode == kModePrerun) { if (D == 2) k_map<2, kModePrerun><<<ntiles, kTile, 0, st>>>(W, S, par); else k_map<3, kModePrerun><<<ntiles, kTile, 0, st>>>(W, S, par); }
+  else { if (D == 2) k_map<2, kModeInit><<<ntiles, kTile, 0, st>>>(W, S, par); else k_map<3, kModeInit><<<ntiles, kTile, 0, st>>>(W, S, par); }
 }
 void launch_scan(hipStream_t st, const Work& W, bool gated, bool per_job) {
   const int64_t nb = (W.n_tbl + kScanItems - 1) / kScanItems;
