@@ -1674,8 +1674,15 @@ __device__ __forceinline__ void for_entries_near(const Work& W, int64_t slot, in
 // pair sum is formed in the same order each run.
 template <int D, int HN>
 __global__ __launch_bounds__(kTile) void k_halo(Work W, PState S) {
+  // The long-list variant (the larger table) orders each pass's halo entries by how many
+  // of the tile's list entries name them, most-named first, so that the records the
+  // pair passes stage in LDS (the first ones) serve most item reads of a halo too large
+  // to stage whole; hv holds each key's count while the pass inserts.
+  constexpr bool kOrd = HN > kHashN;
+  using HV = typename std::conditional<kOrd, uint32_t, uint16_t>::type;
   __shared__ uint32_t hk[HN];
-  __shared__ uint16_t hv[HN];
+  __shared__ HV hv[HN];
+  __shared__ int s_bin[kOrd ? kTile : 1];
   __shared__ int s_scan[4];
   __shared__ int s_len[kTile];
   __shared__ int s_cnt, s_over;
@@ -1772,7 +1779,11 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W, PState S) {
   };
   int P = 1, base = 0;
   for (int pass = 0; pass < P;) {
-    for (int s = threadIdx.x; s < HN; s += kTile) hk[s] = 0xFFFFFFFFu;
+    for (int s = threadIdx.x; s < HN; s += kTile) {
+      hk[s] = 0xFFFFFFFFu;
+      if constexpr (kOrd) hv[s] = 0u;
+    }
+    if constexpr (kOrd) s_bin[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
       s_cnt = 0;
       s_over = 0;
@@ -1793,6 +1804,7 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W, PState S) {
         if (old == e) break;
         s = (s + 1u) & uint32_t(HN - 1);
       }
+      if constexpr (kOrd) atomicAdd(&hv[s], 1u);
     });
     if (full) s_over = 1;
     __syncthreads();
@@ -1809,22 +1821,47 @@ __global__ __launch_bounds__(kTile) void k_halo(Work W, PState S) {
     }
     // halo indices for the occupied table slots (thread t takes slots t, t + 256, ...)
     constexpr int kPer = HN / kTile;
-    int c = 0;
-#pragma unroll 8
-    for (int u = 0; u < kPer; u++) c += hk[u * kTile + threadIdx.x] != 0xFFFFFFFFu ? 1 : 0;
     int U = 0;
-    int idx = base + block_scan_excl(c, s_scan, U);
-    if (base + U > cap) {
-      fail(kOverHalo);
-      return;
-    }
-    for (int u = 0; u < kPer; u++) {
-      const int s = u * kTile + threadIdx.x;
-      const uint32_t key = hk[s];
-      if (key != 0xFFFFFFFFu) {
-        hv[s] = uint16_t(kTile + idx);
-        hal[idx] = key;
-        idx++;
+    if constexpr (!kOrd) {   // in table order
+      int c = 0;
+#pragma unroll 8
+      for (int u = 0; u < kPer; u++) c += hk[u * kTile + threadIdx.x] != 0xFFFFFFFFu ? 1 : 0;
+      int idx = base + block_scan_excl(c, s_scan, U);
+      if (base + U > cap) {
+        fail(kOverHalo);
+        return;
+      }
+      for (int u = 0; u < kPer; u++) {
+        const int s = u * kTile + threadIdx.x;
+        const uint32_t key = hk[s];
+        if (key != 0xFFFFFFFFu) {
+          hv[s] = HV(kTile + idx);
+          hal[idx] = key;
+          idx++;
+        }
+      }
+    } else {   // by count, most-named first: a counting sort over the bins 256 - count
+      auto bin_of = [&](uint32_t n) { return kTile - int(n < uint32_t(kTile) ? n : uint32_t(kTile)); };
+      for (int u = 0; u < kPer; u++) {
+        const int s = u * kTile + threadIdx.x;
+        if (hk[s] != 0xFFFFFFFFu) atomicAdd(&s_bin[bin_of(hv[s])], 1);
+      }
+      __syncthreads();
+      const int b0 = block_scan_excl(s_bin[threadIdx.x], s_scan, U);   // thread t: bin t
+      if (base + U > cap) {
+        fail(kOverHalo);
+        return;
+      }
+      s_bin[threadIdx.x] = base + b0;   // the bin's next index
+      __syncthreads();
+      for (int u = 0; u < kPer; u++) {
+        const int s = u * kTile + threadIdx.x;
+        const uint32_t key = hk[s];
+        if (key != 0xFFFFFFFFu) {
+          const int idx = atomicAdd(&s_bin[bin_of(hv[s])], 1);
+          hv[s] = HV(kTile + idx);
+          hal[idx] = key;
+        }
       }
     }
     __syncthreads();
