@@ -1121,20 +1121,31 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
       // cy + o lie beyond the faces y's own cell shares with it (a position clamped
       // into a border cell lies beyond its inner face too), so their distance from y
       // is at least the gap gq[a][o_a + 1] to those faces on each axis.
-      float gq[3][3];
+      // A stencil cell of a finer table (L > s) is a range of level-L cells; its 2^D
+      // halves at level s + 1 are consecutive sub-ranges, each tested the same way on
+      // the one face of it that faces y: gh[a][2 (o_a + 1) + b_a] for half b_a of
+      // offset o_a (the halves' ranges, taken in order, keep the candidates' order).
+      float gq[3][3], gh[3][6];
       {
         const double cl = J.clen[s];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
           gq[a][0] = gq[a][1] = gq[a][2] = 0.f;
+#pragma unroll
+          for (int b = 0; b < 6; b++) gh[a][b] = 0.f;
           if (a < D) {
             const double f = J.cbox_l[a] + double(cy[a]) * cl;
             const float gm = float(fmax(0., y[a] - f)), gp = float(fmax(0., f + cl - y[a]));
             gq[a][0] = gm * gm;
             gq[a][2] = gp * gp;
+            const float g[6] = {float(fmax(0., y[a] - (f - 0.5 * cl))), gm, float(fmax(0., y[a] - (f + 0.5 * cl))),
+                                float(fmax(0., (f + 0.5 * cl) - y[a])), gp, float(fmax(0., (f + 1.5 * cl) - y[a]))};
+#pragma unroll
+            for (int b = 0; b < 6; b++) gh[a][b] = g[b] * g[b];
           }
         }
       }
+      const bool halves = shift >= D;   // the stencil cells of a finer table
       const float Rl = fmaxf(rfi, float(J.clen[L])) * (1.f + 1.e-5f) + 2.f * pf_tol + 1.e-9f;
       const float R2 = Rl * Rl;
       for (int dz = (D > 2 ? -1 : 0); dz <= (D > 2 ? 1 : 0); dz++)
@@ -1142,6 +1153,11 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
           // the slot ranges of the row's three cells, their bounds loaded together
           int32_t lo[3], hi2[3];
           const float g2 = gq[1][dy + 1] + gq[2][dz + 1];
+          // the row's half gaps on axes 1 and 2 (selects: no indexing by dy, dz)
+          const float g1l = dy < 0 ? gh[1][0] : (dy == 0 ? gh[1][2] : gh[1][4]);
+          const float g1h = dy < 0 ? gh[1][1] : (dy == 0 ? gh[1][3] : gh[1][5]);
+          const float g2l = dz < 0 ? gh[2][0] : (dz == 0 ? gh[2][2] : gh[2][4]);
+          const float g2h = dz < 0 ? gh[2][1] : (dz == 0 ? gh[2][3] : gh[2][5]);
 #pragma unroll
           for (int u = 0; u < 3; u++) {
             int c[3] = {cy[0] + u - 1, cy[1] + dy, cy[2] + dz};
@@ -1156,26 +1172,48 @@ __device__ __forceinline__ int nb_enumerate(const Work& W, const PState& S, cons
               hi2[u] = st[int64_t((m + 1) << shift)] + soff;
             }
           }
-#pragma unroll
-          for (int u = 0; u < 3; u++) {
-            // candidates four at a time: the four records are loaded before any is
-            // tested, from one address (recb has kRecbPad records past its last slot,
-            // so the loads past the range stay inside it; their tests are masked)
-            for (int32_t q = lo[u]; q < hi2[u]; q += 4) {
+          // candidates four at a time: the four records are loaded before any is
+          // tested, from one address (recb has kRecbPad records past its last slot, so
+          // the loads past the range stay inside it; their tests are masked)
+          auto scan = [&](int32_t qlo, int32_t qhi) __attribute__((always_inline)) {
+            for (int32_t q = qlo; q < qhi; q += 4) {
               float4 qr[4];
               const float4* rq = W.recb + uint32_t(q);   // == rec[0] at a build
 #pragma unroll
               for (int v = 0; v < 4; v++) qr[v] = rq[v];
 #pragma unroll
               for (int v = 0; v < 4; v++) {
-                const int tv = q + v < hi2[u] ? test(qr[v]) : 0;
+                const int tv = q + v < qhi ? test(qr[v]) : 0;
                 if (tv & 1) {
                   emit(cnt, int32_t(nb_pack(q + v, p0, code)), (tv & 2) != 0);
                   cnt++;
                 }
               }
             }
-          }
+          };
+          // cell u of the row (u a compile-time constant: the row's arrays stay in registers)
+          auto cell = [&](auto u_) __attribute__((always_inline)) {
+            constexpr int u = decltype(u_)::value;
+            if (!halves) {
+              scan(lo[u], hi2[u]);
+            } else if (lo[u] < hi2[u]) {
+              const int c[3] = {cy[0] + u - 1, cy[1] + dy, cy[2] + dz};
+              const uint64_t mc = uint64_t(morton<D>(c)) << D;
+              const int sh = shift - D;
+              const float g0l = gh[0][2 * u], g0h = gh[0][2 * u + 1];
+              int32_t hl = lo[u];
+              for (int b = 0; b < (1 << D); b++) {
+                const int32_t hh2 = b + 1 < (1 << D) ? st[int64_t((mc + uint64_t(b + 1)) << sh)] + soff : hi2[u];
+                float g = ((b & 1) ? g0h : g0l) + ((b & 2) ? g1h : g1l);
+                if (D > 2) g += (b & 4) ? g2h : g2l;
+                if (g <= R2) scan(hl, hh2);
+                hl = hh2;
+              }
+            }
+          };
+          cell(std::integral_constant<int, 0>{});
+          cell(std::integral_constant<int, 1>{});
+          cell(std::integral_constant<int, 2>{});
         }
     }
   };
